@@ -1,0 +1,490 @@
+// pfc_api.cpp - C ABI of libpfc_hip.so: contexts, tables, step driver,
+// energy, graphs and kernel timing.  See include/pfc_hip.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pfc_hip.h"
+#include "pfc_kernels.h"
+#include "pfc_model.hpp"
+#include "pfc_post.h"
+
+using namespace pfc;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fail(PFC_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+struct pfc_ctx {
+    pfc_params p;
+    Model m;
+    int B = 1, prec = 0, L = 0, M = 0, K = 0;
+    size_t rsize = 8, vsize = 32;  // bytes per real / per Vec element
+    hipStream_t stream = nullptr;
+    void *S = nullptr, *AB = nullptr, *IN = nullptr;
+    double* means_d = nullptr;
+    std::vector<double> means_h;
+    void *twH = nullptr, *twM = nullptr, *twL = nullptr;
+    float *kl_d = nullptr, *km_d = nullptr, *c2_d = nullptr;
+    double *erow = nullptr, *ecol[2] = {nullptr, nullptr}, *eout = nullptr;
+    int nrow = 0, ncol = 0;
+    void *stage_a = nullptr, *stage_b = nullptr;
+    float c2a0 = 0.f, c2b0 = 0.f;
+    PhysArgs phys{};
+    DevBufs bufs{};
+    // graphs
+    int graph_chunk = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    // timing
+    int time_kid = -1;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+};
+
+extern "C" const char* pfc_last_error(void) { return g_err.c_str(); }
+
+extern "C" int pfc_params_default(pfc_params* p, int model) {
+    if (!p || (model != PFC_MODEL_MISFIT && model != PFC_MODEL_CVD)) return fail(PFC_E_ARG, "bad model");
+    default_params(*p, model);
+    return PFC_OK;
+}
+
+extern "C" int pfc_init_fields(const pfc_params* p, double* phi, double* con) {
+    if (!p || !phi || !con || p->L <= 0 || p->M <= 0 || p->N <= 0) return fail(PFC_E_ARG, "bad arguments");
+    if (p->model == PFC_MODEL_MISFIT && (p->grainx <= 0 || p->grainy <= 0 || p->grainz <= 0))
+        return fail(PFC_E_ARG, "grain counts must be > 0");
+    Model m = make_model(*p);
+    init_fields(m, phi, con);
+    return PFC_OK;
+}
+
+template <typename R>
+static void* upload_twiddles(int n, hipError_t* err) {
+    std::vector<R> h(2 * (size_t)n);
+    for (int t = 0; t < n; t++) {
+        const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
+        h[2 * t] = (R)cosl(a);
+        h[2 * t + 1] = (R)sinl(a);
+    }
+    void* d = nullptr;
+    *err = hipMalloc(&d, h.size() * sizeof(R));
+    if (*err == hipSuccess) *err = hipMemcpy(d, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice);
+    return d;
+}
+
+static void free_ctx(pfc_ctx* c) {
+    if (!c) return;
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    for (auto e : c->ev) hipEventDestroy(e);
+    void* ptrs[] = {c->S, c->AB, c->IN, c->means_d, c->twH, c->twM, c->twL, c->kl_d, c->km_d, c->c2_d,
+                    c->erow, c->ecol[0], c->ecol[1], c->eout, c->stage_a, c->stage_b};
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
+    if (!p || !out || batch <= 0) return fail(PFC_E_ARG, "bad arguments");
+    *out = nullptr;
+    if (p->model != PFC_MODEL_MISFIT && p->model != PFC_MODEL_CVD) return fail(PFC_E_ARG, "bad model %d", p->model);
+    if (p->precision != PFC_PREC_FP64 && p->precision != PFC_PREC_FP32) return fail(PFC_E_ARG, "bad precision");
+    if (p->N != 1) return fail(PFC_E_UNSUPPORTED, "only 2-D grids (N = 1) are supported, got N = %d", p->N);
+    if (!size_supported(p->L) || !size_supported(p->M))
+        return fail(PFC_E_UNSUPPORTED, "grid %dx%d not supported (sizes: 32,64,128,256,384,512,768,1024)", p->L, p->M);
+    if (p->dt <= 0.0f || p->dx <= 0.0f) return fail(PFC_E_ARG, "dt and dx must be > 0");
+    Model m = make_model(*p);
+    if (p->model == PFC_MODEL_MISFIT && m.b0 != 0.0f)
+        return fail(PFC_E_UNSUPPORTED, "--axx > 0 (noise term) is not supported by the GPU path");
+    const int prec = p->precision == PFC_PREC_FP64 ? PREC_FP64 : PREC_FP32;
+    const int nrow = row_blocks(prec, p->M, p->L), ncol = col_blocks(prec, p->L, p->M);
+    if (nrow <= 0 || ncol <= 0) return fail(PFC_E_UNSUPPORTED, "no launch shape for %dx%d", p->L, p->M);
+
+    pfc_ctx* c = new pfc_ctx();
+    c->p = *p;
+    c->m = m;
+    c->B = batch;
+    c->prec = prec;
+    c->L = p->L; c->M = p->M; c->K = p->M / 2 + 1;
+    c->rsize = prec == PREC_FP64 ? 8 : 4;
+    c->vsize = 4 * c->rsize;
+    c->nrow = nrow; c->ncol = ncol;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    const size_t nvec = (size_t)batch * c->L * c->K;
+    const size_t nreal = (size_t)batch * c->L * c->M;
+    if (e == hipSuccess) e = hipMalloc(&c->S, nvec * c->vsize);
+    if (e == hipSuccess) e = hipMalloc(&c->AB, nvec * c->vsize);
+    if (e == hipSuccess) e = hipMalloc(&c->IN, nvec * c->vsize);
+    if (e == hipSuccess) e = hipMalloc(&c->stage_a, nreal * c->rsize);
+    if (e == hipSuccess) e = hipMalloc(&c->stage_b, nreal * c->rsize);
+    if (e == hipSuccess) e = hipMalloc(&c->means_d, sizeof(double) * 4 * batch);
+    if (e == hipSuccess) e = hipMalloc(&c->erow, sizeof(double) * 2 * nrow * batch);
+    if (e == hipSuccess) e = hipMalloc(&c->ecol[0], sizeof(double) * ncol * batch);
+    if (e == hipSuccess) e = hipMalloc(&c->ecol[1], sizeof(double) * ncol * batch);
+    if (e == hipSuccess) e = hipMalloc(&c->eout, sizeof(double) * 2 * batch);
+    if (e == hipSuccess) e = hipMemsetAsync(c->S, 0, nvec * c->vsize, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->AB, 0, nvec * c->vsize, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->IN, 0, nvec * c->vsize, c->stream);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(PFC_E_NOMEM, "device allocation failed: %s", hipGetErrorString(e));
+    }
+    if (prec == PREC_FP64) {
+        c->twH = upload_twiddles<double>(c->M / 2, &e);
+        if (e == hipSuccess) c->twM = upload_twiddles<double>(c->M, &e);
+        if (e == hipSuccess) c->twL = upload_twiddles<double>(c->L, &e);
+    } else {
+        c->twH = upload_twiddles<float>(c->M / 2, &e);
+        if (e == hipSuccess) c->twM = upload_twiddles<float>(c->M, &e);
+        if (e == hipSuccess) c->twL = upload_twiddles<float>(c->L, &e);
+    }
+    std::vector<float> kl, km, kn, c2;
+    k_tables(m, kl, km, kn);
+    c2_tables_folded(m, c2, &c->c2a0, &c->c2b0);
+    if (e == hipSuccess) e = hipMalloc(&c->kl_d, sizeof(float) * kl.size());
+    if (e == hipSuccess) e = hipMalloc(&c->km_d, sizeof(float) * km.size());
+    if (e == hipSuccess) e = hipMalloc(&c->c2_d, sizeof(float) * c2.size());
+    if (e == hipSuccess) e = hipMemcpy(c->kl_d, kl.data(), sizeof(float) * kl.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->km_d, km.data(), sizeof(float) * km.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->c2_d, c2.data(), sizeof(float) * c2.size(), hipMemcpyHostToDevice);
+    c->means_h.assign(4 * (size_t)batch, 0.0);
+    if (e == hipSuccess) e = hipMemcpy(c->means_d, c->means_h.data(), sizeof(double) * 4 * batch, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(PFC_E_HIP, "table upload failed: %s", hipGetErrorString(e));
+    }
+    PhysArgs& ph = c->phys;
+    ph.niu = m.niu; ph.ksi = m.ksi; ph.w = m.w; ph.c0 = m.c0; ph.alpha = m.alpha;
+    ph.mn_dt = m.mn * m.dt;
+    ph.mc_dt = m.mc * m.dt;
+    ph.mc_dt_a = m.mc * m.dt * m.alpha;
+    ph.narrow = p->narrow ? 1 : 0;
+    DevBufs& b = c->bufs;
+    b.S = c->S; b.AB = c->AB; b.IN = c->IN;
+    b.means = c->means_d;
+    b.twH = c->twH; b.twM = c->twM; b.twL = c->twL;
+    b.kl = c->kl_d; b.km = c->km_d; b.c2 = c->c2_d;
+    b.erow = c->erow; b.ecol = c->ecol[0];
+    b.L = c->L; b.M = c->M; b.K = c->K; b.B = batch;
+    *out = c;
+    return PFC_OK;
+}
+
+extern "C" int pfc_destroy(pfc_ctx* ctx) {
+    if (!ctx) return PFC_OK;
+    hipStreamSynchronize(ctx->stream);
+    free_ctx(ctx);
+    return PFC_OK;
+}
+
+// Buffers of one instance only.
+static DevBufs inst_bufs(const pfc_ctx* c, int inst) {
+    DevBufs b = c->bufs;
+    const size_t off = (size_t)inst * c->L * c->K * c->vsize;
+    b.S = (char*)c->S + off;
+    b.AB = (char*)c->AB + off;
+    b.IN = (char*)c->IN + off;
+    b.means = c->means_d + 4 * inst;
+    b.erow = c->erow + 2 * (size_t)c->nrow * inst;
+    b.ecol = c->ecol[0] + (size_t)c->ncol * inst;
+    b.B = 1;
+    return b;
+}
+
+extern "C" int pfc_set_fields(pfc_ctx* c, int inst, const double* phi, const double* con) {
+    if (!c || !phi || !con || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
+    const size_t P = (size_t)c->L * c->M;
+    double sp = 0.0, sc = 0.0;
+    for (size_t i = 0; i < P; i++) { sp += phi[i]; sc += con[i]; }
+    // The reference rounds every spectral coefficient, the k=0 (mass) mode
+    // included, to float each step (misfit.cpp:1002-1010); the rounded mass is
+    // a fixed point of that map, so one rounding here reproduces it.
+    if (c->p.narrow) { sp = (double)(float)sp; sc = (double)(float)sc; }
+    const double mp = sp / (double)P, mc = sc / (double)P;
+    double* mu = &c->means_h[4 * (size_t)inst];
+    mu[0] = mp; mu[1] = mc;
+    mu[2] = (double)c->c2a0 * mp;
+    mu[3] = (double)c->c2b0 * mp;
+    std::vector<char> ha(P * c->rsize), hb(P * c->rsize);
+    for (size_t i = 0; i < P; i++) {
+        const double dp = phi[i] - mp, dc = con[i] - mc;
+        if (c->prec == PREC_FP64) { ((double*)ha.data())[i] = dp; ((double*)hb.data())[i] = dc; }
+        else { ((float*)ha.data())[i] = (float)dp; ((float*)hb.data())[i] = (float)dc; }
+    }
+    const size_t poff = (size_t)inst * P * c->rsize;
+    HIP_TRY(hipMemcpyAsync((char*)c->stage_a + poff, ha.data(), P * c->rsize, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync((char*)c->stage_b + poff, hb.data(), P * c->rsize, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->means_d + 4 * inst, mu, sizeof(double) * 4, hipMemcpyHostToDevice, c->stream));
+    DevBufs b = inst_bufs(c, inst);
+    HIP_TRY(launch_row_in(c->prec, b, (char*)c->stage_a + poff, (char*)c->stage_b + poff, c->stream));
+    HIP_TRY(launch_col(c->prec, b, c->phys, COL_INIT, 0, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PFC_OK;
+}
+
+static hipEvent_t next_event(pfc_ctx* c) {
+    if (c->ev_used == c->ev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev.push_back(e);
+    }
+    return c->ev[c->ev_used++];
+}
+
+// One step on all instances: row pass then column pass.
+static hipError_t enqueue_step(pfc_ctx* c, int row_flags, int col_flags, int ecol_slot) {
+    DevBufs b = c->bufs;
+    b.ecol = c->ecol[ecol_slot];
+    const bool t0 = c->time_kid == KID_ROW, t1 = c->time_kid == KID_COL;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (t0) { e0 = next_event(c); e1 = next_event(c); if (e0) hipEventRecord(e0, c->stream); }
+    hipError_t e = launch_row_step(c->prec, b, c->phys, row_flags, c->stream);
+    if (t0 && e1) hipEventRecord(e1, c->stream);
+    if (e != hipSuccess) return e;
+    if (t1) { e0 = next_event(c); e1 = next_event(c); if (e0) hipEventRecord(e0, c->stream); }
+    e = launch_col(c->prec, b, c->phys, COL_STEP, col_flags, c->stream);
+    if (t1 && e1) hipEventRecord(e1, c->stream);
+    return e;
+}
+
+extern "C" int pfc_set_graph(pfc_ctx* c, int chunk) {
+    if (!c || chunk < 0) return fail(PFC_E_ARG, "bad arguments");
+    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    c->graph_chunk = chunk;
+    if (chunk == 0) return PFC_OK;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < chunk && e == hipSuccess; i++) e = enqueue_step(c, ROW_WRITE_IN, 0, 0);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        c->graph_chunk = 0;
+        return fail(PFC_E_HIP, "graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    c->graph = g;
+    HIP_TRY(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    return PFC_OK;
+}
+
+extern "C" int pfc_step(pfc_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return fail(PFC_E_ARG, "bad arguments");
+    int done = 0;
+    if (c->gexec && c->time_kid < 0) {
+        while (nsteps - done >= c->graph_chunk) {
+            HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+            done += c->graph_chunk;
+        }
+    }
+    for (; done < nsteps; done++) HIP_TRY(enqueue_step(c, ROW_WRITE_IN, 0, 0));
+    return PFC_OK;
+}
+
+extern "C" int pfc_sync(pfc_ctx* c) {
+    if (!c) return fail(PFC_E_ARG, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PFC_OK;
+}
+
+template <typename R>
+static void widen(const void* src, double* dst, size_t n) {
+    const R* s = (const R*)src;
+    for (size_t i = 0; i < n; i++) dst[i] = (double)s[i];
+}
+
+// S of instance `inst` -> real planes in stage_a/stage_b (device).
+static int fields_to_stage(pfc_ctx* c, int inst) {
+    const size_t P = (size_t)c->L * c->M;
+    const size_t poff = (size_t)inst * P * c->rsize;
+    DevBufs b = inst_bufs(c, inst);
+    HIP_TRY(launch_row_out(c->prec, b, (char*)c->stage_a + poff, (char*)c->stage_b + poff, c->stream));
+    return PFC_OK;
+}
+
+extern "C" int pfc_get_fields(pfc_ctx* c, int inst, double* phi, double* con) {
+    if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
+    int rc = fields_to_stage(c, inst);
+    if (rc) return rc;
+    const size_t P = (size_t)c->L * c->M;
+    const size_t poff = (size_t)inst * P * c->rsize;
+    std::vector<char> h(P * c->rsize);
+    if (phi) {
+        HIP_TRY(hipMemcpyAsync(h.data(), (char*)c->stage_a + poff, P * c->rsize, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->prec == PREC_FP64) widen<double>(h.data(), phi, P); else widen<float>(h.data(), phi, P);
+    }
+    if (con) {
+        HIP_TRY(hipMemcpyAsync(h.data(), (char*)c->stage_b + poff, P * c->rsize, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->prec == PREC_FP64) widen<double>(h.data(), con, P); else widen<float>(h.data(), con, P);
+    }
+    return PFC_OK;
+}
+
+extern "C" int pfc_get_fields_f32(pfc_ctx* c, int inst, float* phi, float* con) {
+    if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
+    const size_t P = (size_t)c->L * c->M;
+    std::vector<double> a(phi ? P : 0), b(con ? P : 0);
+    int rc = pfc_get_fields(c, inst, phi ? a.data() : nullptr, con ? b.data() : nullptr);
+    if (rc) return rc;
+    for (size_t i = 0; phi && i < P; i++) phi[i] = (float)a[i];
+    for (size_t i = 0; con && i < P; i++) con[i] = (float)b[i];
+    return PFC_OK;
+}
+
+extern "C" int pfc_energy(pfc_ctx* c, double* e_out) {
+    if (!c || !e_out) return fail(PFC_E_ARG, "bad arguments");
+    DevBufs b = c->bufs;
+    b.ecol = c->ecol[0];
+    HIP_TRY(launch_row_step(c->prec, b, c->phys, ROW_ENERGY, c->stream));
+    HIP_TRY(launch_col(c->prec, b, c->phys, COL_ENERGY_ONLY, COL_ENERGY, c->stream));
+    HIP_TRY(launch_energy_reduce(b, c->nrow, c->ncol, (double)c->m.alpha, c->eout, c->stream));
+    HIP_TRY(hipMemcpyAsync(e_out, c->eout, sizeof(double) * 2 * c->B, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PFC_OK;
+}
+
+extern "C" int pfc_step_energy(pfc_ctx* c, int nsteps, double* trace) {
+    if (!c || !trace || nsteps < 0) return fail(PFC_E_ARG, "bad arguments");
+    if (nsteps == 0) return PFC_OK;
+    double* dtr = nullptr;
+    HIP_TRY(hipMalloc(&dtr, sizeof(double) * 2 * c->B * (size_t)nsteps));
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < nsteps && e == hipSuccess; i++) {
+        // Row pass of step i also sums the pointwise energy of the state it
+        // reads (after step i-1); the column pass of step i-1 left that
+        // state's gradient sum in ecol[(i-1)&1].
+        const int slot = i & 1;
+        e = enqueue_step(c, ROW_WRITE_IN | (i > 0 ? ROW_ENERGY : 0), COL_ENERGY, slot);
+        if (e == hipSuccess && i > 0) {
+            DevBufs b = c->bufs;
+            b.ecol = c->ecol[(i - 1) & 1];
+            e = launch_energy_reduce(b, c->nrow, c->ncol, (double)c->m.alpha, dtr + 2 * c->B * (size_t)(i - 1), c->stream);
+        }
+    }
+    if (e == hipSuccess) {
+        DevBufs b = c->bufs;
+        b.ecol = c->ecol[(nsteps - 1) & 1];
+        e = launch_row_step(c->prec, b, c->phys, ROW_ENERGY, c->stream);
+        if (e == hipSuccess)
+            e = launch_energy_reduce(b, c->nrow, c->ncol, (double)c->m.alpha, dtr + 2 * c->B * (size_t)(nsteps - 1), c->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(trace, dtr, sizeof(double) * 2 * c->B * (size_t)nsteps, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(dtr);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "step_energy: %s", hipGetErrorString(e));
+    return PFC_OK;
+}
+
+extern "C" int pfc_time_kernel(pfc_ctx* c, int kernel, int enable) {
+    if (!c || (kernel != KID_ROW && kernel != KID_COL)) return fail(PFC_E_ARG, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->ev_used = 0;
+    c->time_kid = enable ? kernel : -1;
+    return PFC_OK;
+}
+
+extern "C" int pfc_kernel_time(pfc_ctx* c, double* mean_ms, int* count) {
+    if (!c || !mean_ms || !count) return fail(PFC_E_ARG, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    int n = 0;
+    for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+        tot += ms;
+        n++;
+    }
+    *mean_ms = n ? tot / n : 0.0;
+    *count = n;
+    return PFC_OK;
+}
+
+extern "C" int pfc_kernel_bytes(pfc_ctx* c, int kernel, double* bytes) {
+    if (!c || !bytes) return fail(PFC_E_ARG, "bad arguments");
+    const double arr = (double)c->B * c->L * c->K * (double)c->vsize;
+    if (kernel == KID_ROW) *bytes = 3.0 * arr;       // read S, AB; write IN
+    else if (kernel == KID_COL) *bytes = 4.0 * arr;  // read S, IN; write S, AB
+    else return fail(PFC_E_ARG, "bad kernel id");
+    return PFC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Post-processing entry points (kernels in pfc_post.hip)
+// ---------------------------------------------------------------------------
+extern "C" int pfc_phimax_misfit(pfc_ctx* c, int inst, double* x, double* y, double* z, double* phi, float* con,
+                                 int cap, int* n) {
+    if (!c || inst < 0 || inst >= c->B || !n || cap < 0) return fail(PFC_E_ARG, "bad arguments");
+    int rc = fields_to_stage(c, inst);
+    if (rc) return rc;
+    const size_t P = (size_t)c->L * c->M;
+    const size_t poff = (size_t)inst * P * c->rsize;
+    PostResult r;
+    hipError_t e = post_phimax_misfit(c->prec, c->L, c->M, c->m.dx, c->m.con0, (char*)c->stage_a + poff,
+                                      (char*)c->stage_b + poff, x, y, z, phi, con, cap, &r, c->stream);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "phimax_misfit: %s", hipGetErrorString(e));
+    *n = r.n;
+    if (r.n > cap) return fail(PFC_E_CAP, "capacity %d < %d atoms", cap, r.n);
+    return PFC_OK;
+}
+
+extern "C" int pfc_phimax_cvd(pfc_ctx* c, int inst, float* pc_x, float* pc_y, float* pc_z, float* pc_phi, int cap,
+                              int* n, float* phic, float* phimax) {
+    if (!c || inst < 0 || inst >= c->B || !n || cap < 0) return fail(PFC_E_ARG, "bad arguments");
+    int rc = fields_to_stage(c, inst);
+    if (rc) return rc;
+    const size_t P = (size_t)c->L * c->M;
+    const size_t poff = (size_t)inst * P * c->rsize;
+    PostResult r;
+    hipError_t e = post_phimax_cvd(c->prec, c->L, c->M, c->m.dx, c->m.u0, (char*)c->stage_a + poff, pc_x, pc_y, pc_z,
+                                   pc_phi, cap, &r, c->stream);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "phimax_cvd: %s", hipGetErrorString(e));
+    *n = r.n;
+    if (phic) *phic = r.phic;
+    if (phimax) *phimax = r.phimax;
+    if (r.n > cap) return fail(PFC_E_CAP, "capacity %d < %d atoms", cap, r.n);
+    return PFC_OK;
+}
+
+extern "C" int pfc_phimax_elastic(int L, int M, int N, float dx, const float* phi, double* x, double* y, double* z,
+                                  double* phimax, int cap, int* n) {
+    if (L <= 0 || M <= 0 || N <= 0 || !phi || !n || cap < 0) return fail(PFC_E_ARG, "bad arguments");
+    PostResult r;
+    hipError_t e = post_phimax_elastic(L, M, N, dx, phi, x, y, z, phimax, cap, &r);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "phimax_elastic: %s", hipGetErrorString(e));
+    *n = r.n;
+    if (r.n > cap) return fail(PFC_E_CAP, "capacity %d < %d atoms", cap, r.n);
+    return PFC_OK;
+}
+
+extern "C" int pfc_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
+                          const double* z, double* strain, unsigned int* nb) {
+    if (L <= 0 || M <= 0 || N <= 0 || n < 0 || (n > 0 && (!x || !y || !z || !strain || !nb)))
+        return fail(PFC_E_ARG, "bad arguments");
+    if (n == 0) return PFC_OK;
+    hipError_t e = post_strain(L, M, N, dx, benchel, n, x, y, z, strain, nb);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "strain: %s", hipGetErrorString(e));
+    return PFC_OK;
+}

@@ -1,0 +1,63 @@
+// pfc_kernels.h - host-side launchers for the spectral step kernels.
+//
+// Device data layout (one batch of B instances of one grid L x M, N = 1):
+//   S   [B][L][K]  row-spectral state: r2c along m of (dpsi, dcon), K = M/2+1,
+//                  one element = {psi_hat_m, con_hat_m} (two complex values);
+//                  the k=0 mode of both fields is held apart, as FP64 means.
+//   AB  [B][L][K]  row-spectral C2A*psi_hat and C2B*psi_hat of the same state
+//                  (the reference's c2aden/c2bden, misfit.cpp:801-833).
+//   IN  [B][L][K]  row-spectral nonlinear terms {inn, inc} (misfit.cpp:835-874).
+// One time step = row pass (S, AB -> IN) + column pass (S, IN -> S, AB).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace pfc {
+
+enum { PREC_FP64 = 0, PREC_FP32 = 1 };
+
+// Scalars the kernels need, with the reference's float products pre-formed.
+struct PhysArgs {
+    float niu, ksi, w, c0, alpha;
+    float mn_dt;    // dynamic_mn*dt            (float, misfit.cpp:1002)
+    float mc_dt;    // dynamic_mc*dt            (float, misfit.cpp:1007)
+    float mc_dt_a;  // dynamic_mc*dt*para_alpha (float, misfit.cpp:1009)
+    int narrow;     // reference float narrowing of Rhat/inc/energy terms
+};
+
+struct DevBufs {
+    void* S;
+    void* AB;
+    void* IN;
+    const double* means;  // [B][4]: mean psi, mean con, C2A(0)*mean psi, C2B(0)*mean psi
+    const void* twH;      // exp(-2 pi i t/H), H = M/2
+    const void* twM;      // exp(-2 pi i t/M)
+    const void* twL;      // exp(-2 pi i t/L)
+    const float* kl;      // [L]  reference krefl
+    const float* km;      // [M]  reference krefm
+    const float* c2;      // [(L/2+1)][K][2] folded {C2A, C2B}
+    double* erow;         // [B][row_blocks][2]  {sum e_point, sum psi}
+    double* ecol;         // [B][col_blocks]     sum w(k)|con_hat|^2
+    int L, M, K, B;
+};
+
+enum { ROW_WRITE_IN = 1, ROW_ENERGY = 2 };
+enum { COL_INIT = 0, COL_STEP = 1, COL_ENERGY_ONLY = 2 };
+enum { COL_ENERGY = 1 };
+
+bool size_supported(int n);
+int row_blocks(int prec, int M, int L);  // row-pass workgroups per instance
+int col_blocks(int prec, int L, int M);  // column-pass workgroups per instance
+
+hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s);
+hipError_t launch_col(int prec, const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s);
+hipError_t launch_row_in(int prec, const DevBufs& b, const void* inphi, const void* incon, hipStream_t s);
+hipError_t launch_row_out(int prec, const DevBufs& b, void* outphi, void* outcon, hipStream_t s);
+// e_out[inst*2 + {0,1}] = {mean energy, mean density}; alpha_over_P = alpha/P.
+hipError_t launch_energy_reduce(const DevBufs& b, int nrow, int ncol, double alpha, double* e_out,
+                                hipStream_t s);
+
+// Named kernels (for timing the dominant one).
+enum { KID_ROW = 0, KID_COL = 1 };
+
+}  // namespace pfc

@@ -1,0 +1,27 @@
+// pfc_post.h - launchers for the Phimax / strain post-processing kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pfc {
+
+struct PostResult {
+    int n = 0;          // entries found (may exceed the caller's capacity)
+    float phic = 0.f;   // CVD threshold
+    float phimax = 0.f; // CVD float max
+};
+
+// Misfit phimax_atmposi on device real fields phi/con ([L][M], prec reals).
+hipError_t post_phimax_misfit(int prec, int L, int M, float dx, float con0, const void* phi, const void* con,
+                              double* x, double* y, double* z, double* ph, float* co, int cap, PostResult* r,
+                              hipStream_t s);
+// CVD atompositionCFGC2 maxima on device real field phi.
+hipError_t post_phimax_cvd(int prec, int L, int M, float dx, float u0, const void* phi, float* px, float* py,
+                           float* pz, float* pphi, int cap, PostResult* r, hipStream_t s);
+// Elastic phimax_atmposi on a host float field.
+hipError_t post_phimax_elastic(int L, int M, int N, float dx, const float* phi, double* x, double* y, double* z,
+                               double* ph, int cap, PostResult* r);
+// elastic3D on host atom positions.
+hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
+                       const double* z, double* strain, unsigned* nb);
+
+}  // namespace pfc

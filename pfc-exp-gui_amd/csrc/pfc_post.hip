@@ -1,0 +1,542 @@
+// pfc_post.hip - Phimax (atom extraction) and strain kernels.
+//
+// The three Phimax variants of the reference differ in threshold, neighbour
+// test and output order; each is a per-point predicate over the field plus an
+// ordered stream compaction (count per block, exclusive scan of the counts,
+// emit with a ballot prefix inside each block), so the atoms come out in the
+// reference's loop order.  The halo semantics are the single-rank MPI stub's:
+// the x neighbour of the first/last row is that row itself (pfc_common.h:
+// 101-110).  Arithmetic on positions follows the reference's float/double
+// expression types and is compiled without contraction.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "pfc_kernels.h"
+#include "pfc_post.h"
+
+namespace pfc {
+
+constexpr int PB = 256;  // threads per compaction block
+
+// Position of this thread's flag among the block's set flags (block order =
+// thread order).  All threads must call.
+__device__ __forceinline__ int block_rank(bool flag, int* wsum, int& total) {
+    const unsigned long long bal = __ballot(flag);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int below = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wid] = __popcll(bal);
+    __syncthreads();
+    int off = 0;
+    total = 0;
+    for (int w = 0; w < PB / 64; w++) {
+        if (w < wid) off += wsum[w];
+        total += wsum[w];
+    }
+    __syncthreads();
+    return off + below;
+}
+
+// ---------------------------------------------------------------------------
+// Predicates / emitters
+// ---------------------------------------------------------------------------
+
+// Misfit (misfit.cpp:3345-3443): scan order p = i + L*m (l fastest).
+template <typename R>
+struct MisfitOp {
+    const R* phi;
+    const R* con;
+    int L, M;
+    float dx, con0;
+    double* x; double* y; double* z; double* ph; float* co;
+
+    __device__ double v(int i, int m) const { return (double)phi[(size_t)i * M + m]; }
+    __device__ double nbv(int i, int m) const {
+        const double t = v(i, m);
+        return isfinite(t) ? t : -INFINITY;
+    }
+    __device__ bool test(long p) const {
+        const int i = (int)(p % L), m = (int)(p / L);
+        const double center = v(i, m);
+        if (!isfinite(center) || center <= -0.07) return false;
+        const int lf = i > 0 ? i - 1 : 0, lb = i < L - 1 ? i + 1 : L - 1;
+        const int mf = m > 0 ? m - 1 : M - 1, mb = m < M - 1 ? m + 1 : 0;
+        const double nb[8] = {nbv(lf, m), nbv(lb, m), nbv(i, mf), nbv(i, mb),
+                              nbv(lb, mb), nbv(lf, mb), nbv(lb, mf), nbv(lf, mf)};
+        bool ge_all = true, gt_any = false;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (center < nb[q]) ge_all = false;
+            if (center > nb[q]) gt_any = true;
+        }
+        return ge_all && gt_any;
+    }
+    __device__ void emit(long p, size_t o) const {
+#pragma clang fp contract(off)
+        const int i = (int)(p % L), m = (int)(p / L);
+        const int lf = i > 0 ? i - 1 : 0, lb = i < L - 1 ? i + 1 : L - 1;
+        const int mf = m > 0 ? m - 1 : M - 1, mb = m < M - 1 ? m + 1 : 0;
+        const double xtemp = (double)((float)(i + 2) * dx);
+        const double ytemp = (double)((float)(m + 1) * dx);
+        const double ztemp = (double)((float)1 * dx);
+        const double z0 = v(i, m), z1 = v(lb, m), z3 = v(lf, m), z4 = v(i, mb), z2 = v(i, mf);
+        double delta_x = 0.0, delta_y = 0.0, phi_local = z0;
+        if (isfinite(z1) && isfinite(z3) && isfinite(z4) && isfinite(z2)) {
+            const double dx_d = (double)dx;
+            const double inv2dx = 0.5 / dx_d;
+            const double invdx2 = 1.0 / (dx_d * dx_d);
+            const double a1 = (z1 - z3) * inv2dx;
+            const double a2 = (z4 - z2) * inv2dx;
+            const double a3 = 0.5 * (z1 + z3 - 2.0 * z0) * invdx2;
+            const double a4 = 0.5 * (z2 + z4 - 2.0 * z0) * invdx2;
+            const double eps = 1e-12;
+            if (isfinite(a3) && fabs(a3) > eps) delta_x = -a1 / (2.0 * a3);
+            if (isfinite(a4) && fabs(a4) > eps) delta_y = -a2 / (2.0 * a4);
+            const double half_dx = 0.5 * dx_d;
+            delta_x = delta_x < -half_dx ? -half_dx : (delta_x > half_dx ? half_dx : delta_x);
+            delta_y = delta_y < -half_dx ? -half_dx : (delta_y > half_dx ? half_dx : delta_y);
+            double phi_interp = z0;
+            if (isfinite(a3) && fabs(a3) > eps && isfinite(a1)) phi_interp -= 0.25 * a1 * a1 / a3;
+            if (isfinite(a4) && fabs(a4) > eps && isfinite(a2)) phi_interp -= 0.25 * a2 * a2 / a4;
+            if (isfinite(phi_interp)) phi_local = phi_interp;
+        }
+        x[o] = xtemp + delta_x;
+        y[o] = ytemp + delta_y;
+        z[o] = ztemp;
+        ph[o] = phi_local;
+        const double c = (double)con[(size_t)i * M + m];
+        co[o] = isfinite(c) ? (float)c : con0;
+    }
+};
+
+// CVD (cvd.cpp:1125-1158): memory order p = i*M + m; float buffer; the m and
+// n neighbours are not wrapped (for N = 1 the n test repeats the m test).
+template <typename R>
+struct CvdOp {
+    const R* phi;
+    int L, M;
+    float dx, phic, fx, fy, fz;
+    float* px; float* py; float* pz; float* pp;
+
+    __device__ float b(int i, int m) const { return (float)phi[(size_t)i * M + m]; }
+    __device__ bool test(long p) const {
+        const int i = (int)(p / M), m = (int)(p % M);
+        const float c = b(i, m);
+        if (!(c >= phic)) return false;
+        const int up = i > 0 ? i - 1 : 0, dn = i < L - 1 ? i + 1 : L - 1;
+        if (!(c >= b(up, m) && c >= b(dn, m))) return false;
+        const float left = m > 0 ? b(i, m - 1) : b(up, M - 1);
+        const float right = m < M - 1 ? b(i, m + 1) : b(dn, 0);
+        return c >= left && c >= right;
+    }
+    __device__ void emit(long p, size_t o) const {
+#pragma clang fp contract(off)
+        const int i = (int)(p / M), m = (int)(p % M);
+        px[o] = ((float)(m + 1) * dx) / fy;
+        py[o] = ((float)1 * dx) / fz;
+        pz[o] = ((float)(i + 1) * dx) / fx;
+        pp[o] = b(i, m);
+    }
+};
+
+// Elastic (elastic.cpp:2853-2917): memory order p = n + N*(m + M*i); float
+// field; 3x3x3 "none strictly greater" test with the reference's wrap
+// arithmetic; stencil reads outside the halo buffer count as not greater.
+struct ElasticOp {
+    const float* phi;
+    int L, M, N;
+    float dx, phic;
+    double* x; double* y; double* z; double* ph;
+
+    // Halo buffer row `bix` (0..L+1) at in-row offset `rem`.
+    __device__ float bufv(long bix, long rem) const {
+        const long NM = (long)N * M;
+        const long fr = bix == 0 ? 0 : (bix == L + 1 ? L - 1 : bix - 1);
+        return phi[fr * NM + rem];
+    }
+    __device__ bool test(long p) const {
+        const long NM = (long)N * M;
+        const int i = (int)(p / NM);
+        const long rem0 = p % NM;
+        const int m = (int)(rem0 / N), n = (int)(rem0 % N);
+        const float c = phi[p];
+        if (!(c > phic)) return false;
+        const long l = i + 1, total = NM * (L + 2);
+        bool mark = true;
+        for (int lx = -1; lx < 2; lx++)
+            for (int ly = -1; ly < 2; ly++)
+                for (int lz = -1; lz < 2; lz++) {
+                    const long ix = l + lx;
+                    long jy = m + ly, kz = n + lz;
+                    if (jy > (M - 1)) jy = jy - M + 1;
+                    if (kz > (N - 1)) kz = kz - N + 1;
+                    if (jy < 0) jy = jy + M - 1;
+                    if (kz < 0) kz = kz + N - 1;
+                    const long idx = kz + (long)N * (jy + (long)M * ix);
+                    if (idx < 0 || idx >= total) continue;
+                    if (bufv(idx / NM, idx % NM) > c) mark = false;
+                }
+        return mark;
+    }
+    __device__ void emit(long p, size_t o) const {
+#pragma clang fp contract(off)
+        const long NM = (long)N * M;
+        const int i = (int)(p / NM);
+        const long rem0 = p % NM;
+        const int m = (int)(rem0 / N), n = (int)(rem0 % N);
+        const long l = i + 1;
+        const int mf = m - 1 < 0 ? M - 1 : m - 1, mb = m + 1 > M - 1 ? 0 : m + 1;
+        const int nf = n - 1 < 0 ? N - 1 : n - 1, nb = n + 1 > N - 1 ? 0 : n + 1;
+        const double xtemp = (double)l, ytemp = (double)m, ztemp = (double)n;
+        const double z0 = bufv(l, n + (long)N * m);
+        const double z1 = bufv(l + 1, n + (long)N * m);
+        const double z3 = bufv(l - 1, n + (long)N * m);
+        const double z4 = bufv(l, n + (long)N * mb);
+        const double z2 = bufv(l, n + (long)N * mf);
+        const double z5 = bufv(l, nb + (long)N * m);
+        const double z6 = bufv(l, nf + (long)N * m);
+        const float dx2 = dx * dx;
+        const double a0 = z0;
+        const double a1 = 0.5 * (z1 - z3) / (double)dx + 10e-8;
+        const double a2 = 0.5 * (z4 - z2) / (double)dx + 10e-8;
+        const double a3 = 0.5 * (z1 + z3 - 2. * z0) / (double)dx2 + 10e-8;
+        const double a4 = 0.5 * (z2 + z4 - 2. * z0) / (double)dx2 + 10e-8;
+        const double a5 = 0.5 * (z5 - z6) / (double)dx + 10e-8;
+        const double a6 = 0.5 * (z5 + z6 - 2. * z0) / (double)dx2 + 10e-8;
+        const double delta_x = -a1 / (2. * a3);
+        const double delta_y = -a2 / (2. * a4);
+        const double delta_z = -a5 / (2. * a6);
+        x[o] = (xtemp + 1) * (double)dx + delta_x;
+        y[o] = (ytemp + 1) * (double)dx + delta_y;
+        z[o] = (ztemp + 1) * (double)dx + delta_z;
+        ph[o] = a0 - 0.25 * a1 * a1 / a3 - 0.25 * a2 * a2 / a4 - 0.25 * a5 * a5 / a6;
+    }
+};
+
+template <typename Op>
+__global__ void __launch_bounds__(PB) k_count(Op op, long P, int* counts) {
+    __shared__ int wsum[PB / 64];
+    const long p = (long)blockIdx.x * PB + threadIdx.x;
+    const bool f = p < P && op.test(p);
+    int total;
+    (void)block_rank(f, wsum, total);
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// Exclusive scan of nb counts by one workgroup; offsets[nb] = total.
+__global__ void __launch_bounds__(1024) k_scan(const int* counts, int nb, int* offsets) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = t * per, b1 = min(nb, b0 + per);
+    int s = 0;
+    for (int i = b0; i < b1; i++) s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int i = 0; i < 1024; i++) { const int v = part[i]; part[i] = acc; acc += v; }
+        offsets[nb] = acc;
+    }
+    __syncthreads();
+    int acc = part[t];
+    for (int i = b0; i < b1; i++) { offsets[i] = acc; acc += counts[i]; }
+}
+
+template <typename Op>
+__global__ void __launch_bounds__(PB) k_emit(Op op, long P, const int* offsets, int cap) {
+    __shared__ int wsum[PB / 64];
+    const long p = (long)blockIdx.x * PB + threadIdx.x;
+    const bool f = p < P && op.test(p);
+    int total;
+    const int r = block_rank(f, wsum, total);
+    const long o = (long)offsets[blockIdx.x] + r;
+    if (f && o < cap) op.emit(p, (size_t)o);
+}
+
+__global__ void k_max(const void* phi, int prec, long P, double* out) {
+    __shared__ double red[256];
+    double m = -INFINITY;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < P; i += (long)gridDim.x * blockDim.x) {
+        const double v = prec == PREC_FP64 ? ((const double*)phi)[i] : (double)((const float*)phi)[i];
+        m = v > m ? v : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + s] ? red[threadIdx.x] : red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Device scratch for one compaction; freed at the end of the call.
+struct Scratch {
+    std::vector<void*> p;
+    ~Scratch() { for (void* q : p) hipFree(q); }
+    template <typename T>
+    T* get(size_t n, hipError_t& e) {
+        void* q = nullptr;
+        if (e == hipSuccess) e = hipMalloc(&q, n * sizeof(T) + 16);
+        if (e == hipSuccess) p.push_back(q);
+        return (T*)q;
+    }
+};
+
+template <typename Op>
+static hipError_t compact(Op op, long P, int cap, int* n_out, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    Scratch sc;
+    const int nb = (int)((P + PB - 1) / PB);
+    int* counts = sc.get<int>(nb, e);
+    int* offs = sc.get<int>(nb + 1, e);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_count<Op>, dim3(nb), dim3(PB), 0, s, op, P, counts);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nb, offs);
+    hipLaunchKernelGGL(k_emit<Op>, dim3(nb), dim3(PB), 0, s, op, P, offs, cap);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    int tot = 0;
+    if ((e = hipMemcpyAsync(&tot, offs + nb, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    *n_out = tot;
+    return hipSuccess;
+}
+
+template <typename T>
+static hipError_t d2h(T* dst, const T* src, int n, hipStream_t s) {
+    if (n <= 0 || !dst) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, s);
+    return e;
+}
+
+template <typename R>
+static hipError_t misfit_t(int L, int M, float dx, float con0, const void* phi, const void* con, double* x,
+                           double* y, double* z, double* ph, float* co, int cap, PostResult* r, hipStream_t s) {
+    const long P = (long)L * M;
+    hipError_t e = hipSuccess;
+    Scratch sc;
+    MisfitOp<R> op;
+    op.phi = (const R*)phi; op.con = (const R*)con; op.L = L; op.M = M; op.dx = dx; op.con0 = con0;
+    op.x = sc.get<double>(P, e); op.y = sc.get<double>(P, e); op.z = sc.get<double>(P, e);
+    op.ph = sc.get<double>(P, e); op.co = sc.get<float>(P, e);
+    if (e != hipSuccess) return e;
+    int n = 0;
+    if ((e = compact(op, P, (int)P, &n, s)) != hipSuccess) return e;
+    if (n == 0) {
+        // Fallback of misfit.cpp:3449-3464: last position of the global max in
+        // memory order, computed on the host copy of the fields.
+        std::vector<R> hp(P), hc(P);
+        if ((e = hipMemcpyAsync(hp.data(), phi, sizeof(R) * P, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(hc.data(), con, sizeof(R) * P, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        double gmax = -INFINITY;
+        int gm = 0, gl = 0;
+        for (int i = 0; i < L; i++)
+            for (int m = 0; m < M; m++) {
+                const double v = (double)hp[(size_t)i * M + m];
+                if (!std::isfinite(v)) continue;
+                gmax = gmax > v ? gmax : v;
+                if (v >= gmax) { gm = m; gl = i; }
+            }
+        r->n = 1;
+        if (cap >= 1) {
+            const double dxd = (double)dx;
+            x[0] = (gl + 2) * dxd; y[0] = (gm + 1) * dxd; z[0] = dxd;
+            ph[0] = std::isfinite(gmax) ? gmax : 0.0;
+            const double c = (double)hc[(size_t)gl * M + gm];
+            co[0] = std::isfinite(c) ? (float)c : con0;
+        }
+        return hipSuccess;
+    }
+    r->n = n;
+    const int k = n < cap ? n : cap;
+    if ((e = d2h(x, op.x, k, s)) != hipSuccess) return e;
+    if ((e = d2h(y, op.y, k, s)) != hipSuccess) return e;
+    if ((e = d2h(z, op.z, k, s)) != hipSuccess) return e;
+    if ((e = d2h(ph, op.ph, k, s)) != hipSuccess) return e;
+    if ((e = d2h(co, op.co, k, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t post_phimax_misfit(int prec, int L, int M, float dx, float con0, const void* phi, const void* con,
+                              double* x, double* y, double* z, double* ph, float* co, int cap, PostResult* r,
+                              hipStream_t s) {
+    return prec == PREC_FP64 ? misfit_t<double>(L, M, dx, con0, phi, con, x, y, z, ph, co, cap, r, s)
+                             : misfit_t<float>(L, M, dx, con0, phi, con, x, y, z, ph, co, cap, r, s);
+}
+
+static hipError_t field_max(const void* phi, int prec, long P, double* out, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    Scratch sc;
+    const int nb = 256;
+    double* part = sc.get<double>(nb, e);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_max, dim3(nb), dim3(256), 0, s, phi, prec, P, part);
+    std::vector<double> h(nb);
+    if ((e = hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    double m = -INFINITY;
+    for (double v : h) m = v > m ? v : m;
+    *out = m;
+    return hipSuccess;
+}
+
+template <typename R>
+static hipError_t cvd_t(int L, int M, float dx, float u0, const void* phi, float* px, float* py, float* pz,
+                        float* pp, int cap, PostResult* r, hipStream_t s) {
+    const long P = (long)L * M;
+    double gmax = 0.0;
+    hipError_t e = field_max(phi, sizeof(R) == 8 ? PREC_FP64 : PREC_FP32, P, &gmax, s);
+    if (e != hipSuccess) return e;
+    // cvd.cpp:1077-1084: running float max starting at phimm = -1.
+    float pmax = -1.0f;
+    if (gmax > (double)pmax) pmax = (float)gmax;
+    const float phic = u0 + (pmax - u0) * 0.618f;
+    r->phic = phic;
+    r->phimax = pmax;
+    Scratch sc;
+    CvdOp<R> op;
+    op.phi = (const R*)phi; op.L = L; op.M = M; op.dx = dx; op.phic = phic;
+    op.fx = L * dx; op.fy = M * dx; op.fz = 1 * dx;
+    op.px = sc.get<float>(P, e); op.py = sc.get<float>(P, e); op.pz = sc.get<float>(P, e); op.pp = sc.get<float>(P, e);
+    if (e != hipSuccess) return e;
+    int n = 0;
+    if ((e = compact(op, P, (int)P, &n, s)) != hipSuccess) return e;
+    r->n = n;
+    const int k = n < cap ? n : cap;
+    if ((e = d2h(px, op.px, k, s)) != hipSuccess) return e;
+    if ((e = d2h(py, op.py, k, s)) != hipSuccess) return e;
+    if ((e = d2h(pz, op.pz, k, s)) != hipSuccess) return e;
+    if ((e = d2h(pp, op.pp, k, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t post_phimax_cvd(int prec, int L, int M, float dx, float u0, const void* phi, float* px, float* py,
+                           float* pz, float* pphi, int cap, PostResult* r, hipStream_t s) {
+    return prec == PREC_FP64 ? cvd_t<double>(L, M, dx, u0, phi, px, py, pz, pphi, cap, r, s)
+                             : cvd_t<float>(L, M, dx, u0, phi, px, py, pz, pphi, cap, r, s);
+}
+
+hipError_t post_phimax_elastic(int L, int M, int N, float dx, const float* phi, double* x, double* y, double* z,
+                               double* ph, int cap, PostResult* r) {
+    const long P = (long)L * M * N;
+    hipError_t e = hipSuccess;
+    Scratch sc;
+    float* dphi = sc.get<float>(P, e);
+    if (e != hipSuccess) return e;
+    hipStream_t s = nullptr;
+    if ((e = hipMemcpy(dphi, phi, sizeof(float) * P, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    // elastic.cpp:2809-2816: float running max from 0.
+    float pmax = 0.0f;
+    for (long i = 0; i < P; i++)
+        if (phi[i] > pmax) pmax = phi[i];
+    ElasticOp op;
+    op.phi = dphi; op.L = L; op.M = M; op.N = N; op.dx = dx;
+    op.phic = (pmax - pmax * 0.40f);
+    op.x = sc.get<double>(P, e); op.y = sc.get<double>(P, e); op.z = sc.get<double>(P, e); op.ph = sc.get<double>(P, e);
+    if (e != hipSuccess) return e;
+    int n = 0;
+    if ((e = compact(op, P, (int)P, &n, s)) != hipSuccess) return e;
+    r->n = n;
+    const int k = n < cap ? n : cap;
+    if ((e = d2h(x, op.x, k, s)) != hipSuccess) return e;
+    if ((e = d2h(y, op.y, k, s)) != hipSuccess) return e;
+    if ((e = d2h(z, op.z, k, s)) != hipSuccess) return e;
+    if ((e = d2h(ph, op.ph, k, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+// ---------------------------------------------------------------------------
+// elastic3D (elastic.cpp:3652-3729): one thread per atom i, all j in order,
+// j tiles staged through LDS.  Float distances with minimum image, at most
+// 16 stored neighbours; terms for 1.3 <= r < benchel (uninitialised reads in
+// the reference) are 0.
+// ---------------------------------------------------------------------------
+constexpr int SB = 256;
+
+__device__ __forceinline__ float min3f(float a, float b, float c) {
+    const float aa = fabsf(a), bb = fabsf(b), cc = fabsf(c);
+    return ((aa < bb ? aa : bb) < cc) ? (aa < bb ? aa : bb) : cc;
+}
+
+__global__ void __launch_bounds__(SB) k_strain(int n, const double* __restrict__ X, const double* __restrict__ Y,
+                                               const double* __restrict__ Z, float fx, float fy, float fz,
+                                               float benchel, double* strain, unsigned* nbo) {
+#pragma clang fp contract(off)
+    __shared__ double sx[SB], sy[SB], sz[SB];
+    const int i = blockIdx.x * SB + threadIdx.x;
+    const bool act = i < n;
+    const double xi = act ? X[i] : 0.0, yi = act ? Y[i] : 0.0, zi = act ? Z[i] : 0.0;
+    unsigned Nb = 0;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    float rjx = 0.f, rjy = 0.f, rjz = 0.f;
+    for (int j0 = 0; j0 < n; j0 += SB) {
+        __syncthreads();
+        const int jl = j0 + threadIdx.x;
+        if (jl < n) { sx[threadIdx.x] = X[jl]; sy[threadIdx.x] = Y[jl]; sz[threadIdx.x] = Z[jl]; }
+        __syncthreads();
+        const int cnt = n - j0 < SB ? n - j0 : SB;
+        if (!act) continue;
+        for (int jj = 0; jj < cnt; jj++) {
+            const int j = j0 + jj;
+            if (j == i) continue;
+            float r3[9];
+            r3[0] = (float)(xi - sx[jj]);
+            r3[1] = (float)(yi - sy[jj]);
+            r3[2] = (float)(zi - sz[jj]);
+            r3[3] = r3[0] + fx; r3[4] = r3[1] + fy; r3[5] = r3[2] + fz;
+            r3[6] = r3[0] - fx; r3[7] = r3[1] - fy; r3[8] = r3[2] - fz;
+            const float drx = min3f(fabsf(r3[0]), fabsf(r3[3]), fabsf(r3[6]));
+            const float dry = min3f(fabsf(r3[1]), fabsf(r3[4]), fabsf(r3[7]));
+            const float drz = min3f(fabsf(r3[2]), fabsf(r3[5]), fabsf(r3[8]));
+            const float rr = sqrtf(drx * drx + dry * dry + drz * drz);
+            if (!(rr < benchel && (double)rr > 0.7)) continue;
+            Nb = Nb + 1;
+            if (Nb > 16u) continue;
+            const double rij = rr;
+#pragma unroll
+            for (int q = 0; q < 7; q += 3)
+                if ((double)fabsf(fabsf(r3[q]) - drx) < 1e-3) rjx = (float)(xi - (double)r3[q]);
+#pragma unroll
+            for (int q = 1; q < 8; q += 3)
+                if ((double)fabsf(fabsf(r3[q]) - dry) < 1e-3) rjy = (float)(yi - (double)r3[q]);
+#pragma unroll
+            for (int q = 2; q < 9; q += 3)
+                if ((double)fabsf(fabsf(r3[q]) - drz) < 1e-3) rjz = (float)(zi - (double)r3[q]);
+            if (rij < 1.3) {
+                ax = ax + 1.0 * (rij - 1) * (((double)rjx - xi) / rij);
+                ay = ay + 1.0 * (rij - 1) * (((double)rjy - yi) / rij);
+                az = az + 1.0 * (rij - 1) * (((double)rjz - zi) / rij);
+            } else {
+                ax = ax + 0.0; ay = ay + 0.0; az = az + 0.0;
+            }
+        }
+    }
+    if (act) {
+        strain[i] = sqrt(ax * ax + ay * ay + az * az);
+        nbo[i] = Nb;
+    }
+}
+
+hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
+                       const double* z, double* strain, unsigned* nb) {
+    hipError_t e = hipSuccess;
+    Scratch sc;
+    double* dx_ = sc.get<double>(n, e);
+    double* dy_ = sc.get<double>(n, e);
+    double* dz_ = sc.get<double>(n, e);
+    double* ds = sc.get<double>(n, e);
+    unsigned* dn = sc.get<unsigned>(n, e);
+    if (e != hipSuccess) return e;
+    if ((e = hipMemcpy(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dz_, z, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    const float fx = L * dx, fy = M * dx, fz = N * dx;
+    hipLaunchKernelGGL(k_strain, dim3((n + SB - 1) / SB), dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, benchel, ds, dn);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpy(strain, ds, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+    return hipMemcpy(nb, dn, sizeof(unsigned) * n, hipMemcpyDeviceToHost);
+}
+
+}  // namespace pfc

@@ -1,0 +1,595 @@
+// pfc_step.hip - fused spectral step kernels for the Misfit / CVD binary PFC
+// models on gfx950.
+//
+// The reference step (misfit.cpp:477-498, cvd.cpp:400-419) performs eight
+// state-changing c2c FFTs over full complex-double arrays plus five
+// elementwise sweeps.  Here a step is two kernels over half spectra:
+//
+//   row pass    per row l: c2r along m of S and AB -> psi, con, a, b in
+//               registers -> in1+in2+in3 and inc (UPDATEINN3, UPDATEINFMIX,
+//               UPDATEINFCMIX second half, UPDATEINC2MIX, UPDATEINC2MIXC)
+//               -> r2c along m -> IN.
+//   column pass per column tile: FFT along l of S and IN -> semi-implicit
+//               update (MAINEQUATIONC2_NOI with b0 = 0 / MAINEQUATIONC2MIX)
+//               -> inverse FFT along l of the new state -> S, and of
+//               C2A/C2B times the new psi_hat -> AB (UPDATEINFCMIX first half).
+//
+// Both passes stream each array through HBM once; all transforms and the
+// pointwise algebra happen in LDS and VGPRs.
+#include "pfc_fft.hpp"
+#include "pfc_kernels.h"
+
+namespace pfc {
+
+// ---------------------------------------------------------------------------
+// Pointwise physics
+// ---------------------------------------------------------------------------
+
+// Nonlinear terms at one point (misfit.cpp:777, 792, 826, 840, 858-867).
+// Returns inn, inc and in3.
+__device__ __forceinline__ void nonlinear(const PhysArgs& p, double ph, double c, double a, double b,
+                                          double& inn, double& inc, double& in3) {
+#pragma clang fp contract(off)
+    const double c0 = (double)p.c0;
+    const double lc = log(c / c0);
+    const double lc1 = log((1.0 - c) / (1.0 - c0));
+    const double ph2 = ph * ph;
+    const double in1 = -(double)p.niu * ph2 / 2.0 + (double)p.ksi * (ph2 * ph) / 3.0;
+    const double in2 = (double)p.w * (c * lc + (1.0 - c) * lc1);
+    const double cc = c * c;
+    const double cm = 1.0 - c;
+    const double h1 = 1.0 - 3.0 * cc + 2.0 * (cc * c);
+    const double h2 = 1.0 - 3.0 * (cm * cm) + 2.0 * (cm * cm * cm);
+    in3 = -h1 * a - h2 * b;
+    inn = in1 + in2 + in3;
+    const double wp = (double)p.w * (ph + 1.0);
+    const double t1 = wp * lc;
+    const double t2 = wp * lc1;
+    const double t3 = 3.0 * ph * (a - b) * cc;
+    const double t4 = 3.0 * ph * (-1.0 * a + b) * c;
+    if (p.narrow)
+        inc = (double)((((float)t1 - (float)t2) - (float)t3) - (float)t4);
+    else
+        inc = t1 - t2 - t3 - t4;
+}
+
+__device__ __forceinline__ void nonlinear(const PhysArgs& p, float ph, float c, float a, float b, float& inn,
+                                          float& inc, float& in3) {
+    const float c0 = p.c0;
+    const float lc = logf(c / c0);
+    const float lc1 = logf((1.0f - c) / (1.0f - c0));
+    const float ph2 = ph * ph;
+    const float in1 = -p.niu * ph2 * 0.5f + p.ksi * (ph2 * ph) * (1.0f / 3.0f);
+    const float in2 = p.w * (c * lc + (1.0f - c) * lc1);
+    const float cc = c * c;
+    const float cm = 1.0f - c;
+    const float h1 = 1.0f - 3.0f * cc + 2.0f * (cc * c);
+    const float h2 = 1.0f - 3.0f * (cm * cm) + 2.0f * (cm * cm * cm);
+    in3 = -h1 * a - h2 * b;
+    inn = in1 + in2 + in3;
+    const float wp = p.w * (ph + 1.0f);
+    inc = wp * lc - wp * lc1 - 3.0f * ph * (a - b) * cc - 3.0f * ph * (b - a) * c;
+}
+
+// Per-point energy without the gradient term (ENERGYC2, misfit.cpp:1054-1058);
+// the alpha*grac term is summed in k space by the column pass (Parseval).
+template <typename R>
+__device__ __forceinline__ double energy_point(const PhysArgs& p, R phr, R cr, R in3r) {
+#pragma clang fp contract(off)
+    const double ph = (double)phr, c = (double)cr, in3 = (double)in3r;
+    const double c0 = (double)p.c0;
+    const double ph2 = ph * ph;
+    const double eid = ph2 / 2.0 - (double)p.niu * (ph2 * ph) / 6.0 + (double)p.ksi * (ph2 * ph2) / 12.0;
+    const double emix = (ph + 1.0) * (double)p.w * (c * log(c / c0) + (1.0 - c) * log((1.0 - c) / (1.0 - c0)));
+    const double eex = 0.5 * ph * in3;
+    if (p.narrow) return (double)((float)eid + (float)emix + (float)eex);
+    return eid + emix + eex;
+}
+
+// Semi-implicit update of one mode (misfit.cpp:1002-1010, cvd.cpp:872-880).
+__device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<double, 2>& s,
+                                            const Vec<double, 2>& o) {
+#pragma clang fp contract(off)
+    const float mk = p.mn_dt * ks;
+    double r1 = s.c[0].x - (double)mk * o.c[0].x;
+    double r2 = s.c[0].y - (double)mk * o.c[0].y;
+    const float mkc = p.mc_dt * ks;
+    const float den = (p.mc_dt_a * ks) * ks;
+    double r3 = s.c[1].x - (double)mkc * o.c[1].x;
+    double r4 = s.c[1].y - (double)mkc * o.c[1].y;
+    if (p.narrow) {
+        r1 = (double)(float)r1; r2 = (double)(float)r2;
+        r3 = (double)(float)r3; r4 = (double)(float)r4;
+    }
+    const double d1 = 1.0 + (double)mk;
+    const double d2 = 1.0 + (double)den;
+    s.c[0].x = r1 / d1; s.c[0].y = r2 / d1;
+    s.c[1].x = r3 / d2; s.c[1].y = r4 / d2;
+}
+
+__device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<float, 2>& s,
+                                            const Vec<float, 2>& o) {
+    const float mk = p.mn_dt * ks;
+    const float mkc = p.mc_dt * ks;
+    const float den = (p.mc_dt_a * ks) * ks;
+    const float d1 = 1.0f / (1.0f + mk);
+    const float d2 = 1.0f / (1.0f + den);
+    s.c[0].x = (s.c[0].x - mk * o.c[0].x) * d1;
+    s.c[0].y = (s.c[0].y - mk * o.c[0].y) * d1;
+    s.c[1].x = (s.c[1].x - mkc * o.c[1].x) * d2;
+    s.c[1].y = (s.c[1].y - mkc * o.c[1].y) * d2;
+}
+
+__device__ __forceinline__ float ksq(float a, float b) {
+#pragma clang fp contract(off)
+    return a * a + b * b + 0.0f * 0.0f;
+}
+
+// Deterministic workgroup sum (fixed shuffle tree, then waves in order).
+// Result valid in thread 0.  All threads must call.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    constexpr int W = NT < 64 ? NT : 64;
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
+    const int lane = threadIdx.x % W, wid = threadIdx.x / W;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NT / W; i++) tot += red[i];
+    return tot;
+}
+
+// ---------------------------------------------------------------------------
+// Launch shapes
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+struct RowCfg {
+    static constexpr int E = elems_for(H);
+    static constexpr int G = H / E;
+    static constexpr int cap = sizeof(R) == 8 ? 8 : 16;
+    static constexpr int want = (sizeof(R) == 8 ? 128 : 256) / G;
+    static constexpr int RPW = want < 1 ? 1 : (want > cap ? cap : want);
+    static constexpr int NT = RPW * G;
+    static constexpr int K = H + 1;
+};
+
+template <typename R, int LL>
+struct ColCfg {
+    static constexpr int E = elems_for(LL);
+    static constexpr int G = LL / E;
+    static constexpr int T = sizeof(R) == 8 ? (LL <= 128 ? 8 : (LL <= 512 ? 4 : 2))
+                                            : (LL <= 128 ? 16 : (LL <= 512 ? 8 : 4));
+    static constexpr int NT = T * G;
+};
+
+// ---------------------------------------------------------------------------
+// Row-transform helpers (one row of length M = 2H held as K = H+1 elements)
+// ---------------------------------------------------------------------------
+
+// c2r: lds[0..H] holds X[k]; on return z[q] = M*(x[2j] + i x[2j+1]) per field,
+// j = t + G*q.  Uses z = IFFT_H(Z), Z[k] = (X[k] + X*[H-k]) + i (X[k] - X*[H-k]) e^{+2 pi i k/M}.
+template <typename R, int H>
+__device__ __forceinline__ void row_c2r(Vec<R, 2>* lds, int t, const Cx<R>* __restrict__ twH,
+                                        const Cx<R>* __restrict__ twM, Vec<R, 2> (&z)[elems_for(H)]) {
+    constexpr int E = elems_for(H), G = H / E;
+    Vec<R, 2> zz[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int k = t + G * q;
+        const Vec<R, 2> xk = lds[k], xc = lds[H - k];
+        const Cx<R> w = twM[k];
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            const R ax = xk.c[f].x + xc.c[f].x, ay = xk.c[f].y - xc.c[f].y;
+            const R dx = xk.c[f].x - xc.c[f].x, dy = xk.c[f].y + xc.c[f].y;
+            // B = D * conj(w)
+            const R bx = dx * w.x + dy * w.y, by = dy * w.x - dx * w.y;
+            zz[q].c[f] = Cx<R>{ax - by, ay + bx};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; q++) lds[t + G * q] = zz[q];
+    __syncthreads();
+    lds_fft<R, 2, H, +1, 1>(lds, t, true, twH);
+#pragma unroll
+    for (int q = 0; q < E; q++) z[q] = lds[t + G * q];
+}
+
+// r2c of w[q] (= x[2j] + i x[2j+1] per field, j = t + G*q); writes K = H+1
+// output elements to dst.  Leaves lds busy; ends without a trailing barrier.
+template <typename R, int H>
+__device__ __forceinline__ void row_r2c_store(Vec<R, 2>* lds, int t, const Vec<R, 2> (&w)[elems_for(H)],
+                                              const Cx<R>* __restrict__ twH, const Cx<R>* __restrict__ twM,
+                                              Vec<R, 2>* __restrict__ dst) {
+    constexpr int E = elems_for(H), G = H / E;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; q++) lds[t + G * q] = w[q];
+    __syncthreads();
+    lds_fft<R, 2, H, -1, 1>(lds, t, true, twH);
+    const R half = R(0.5);
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int k = t + G * q;
+        const Vec<R, 2> wk = lds[k], wc = lds[(H - k) % H];
+        const Cx<R> tw = twM[k];
+        Vec<R, 2> X;
+        Vec<R, 2> XH;
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            const R ex = half * (wk.c[f].x + wc.c[f].x), ey = half * (wk.c[f].y - wc.c[f].y);
+            const R dx = wk.c[f].x - wc.c[f].x, dy = wk.c[f].y + wc.c[f].y;
+            const R ox = half * dy, oy = -half * dx;  // D / (2i)
+            X.c[f] = Cx<R>{ex + (tw.x * ox - tw.y * oy), ey + (tw.x * oy + tw.y * ox)};
+            XH.c[f] = Cx<R>{ex - ox, R(0)};
+        }
+        dst[k] = X;
+        if (k == 0) dst[H] = XH;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row pass: S, AB -> IN (+ per-row energy partials)
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, PhysArgs p, int flags) {
+    using C = RowCfg<R, H>;
+    constexpr int E = C::E, G = C::G, RPW = C::RPW, K = C::K;
+    __shared__ Vec<R, 2> sm[RPW][K];
+    __shared__ double red[(C::NT + 63) / 64 + 1];
+    const int tid = threadIdx.x, rl = tid / G, t = tid % G;
+    const int inst = blockIdx.y, row = blockIdx.x * RPW + rl;
+    const size_t rbase = ((size_t)inst * b.L + row) * K;
+    const Vec<R, 2>* S = (const Vec<R, 2>*)b.S;
+    const Vec<R, 2>* AB = (const Vec<R, 2>*)b.AB;
+    const Cx<R>* twH = (const Cx<R>*)b.twH;
+    const Cx<R>* twM = (const Cx<R>*)b.twM;
+    Vec<R, 2>* lds = sm[rl];
+
+    for (int k = t; k < K; k += G) lds[k] = S[rbase + k];
+    __syncthreads();
+    Vec<R, 2> zs[E];
+    row_c2r<R, H>(lds, t, twH, twM, zs);
+    __syncthreads();
+    for (int k = t; k < K; k += G) lds[k] = AB[rbase + k];
+    __syncthreads();
+    Vec<R, 2> zab[E];
+    row_c2r<R, H>(lds, t, twH, twM, zab);
+
+    const double* mu = b.means + 4 * inst;
+    const R mph = (R)mu[0], mco = (R)mu[1], ma = (R)mu[2], mb = (R)mu[3];
+    const R invM = R(1) / R(2 * H);
+    Vec<R, 2> w[E];
+    double esum = 0.0, dsum = 0.0;
+    const bool want_e = (flags & ROW_ENERGY) != 0;
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        R ph[2] = {mph + zs[q].c[0].x * invM, mph + zs[q].c[0].y * invM};
+        R co[2] = {mco + zs[q].c[1].x * invM, mco + zs[q].c[1].y * invM};
+        R aa[2] = {ma + zab[q].c[0].x * invM, ma + zab[q].c[0].y * invM};
+        R bb[2] = {mb + zab[q].c[1].x * invM, mb + zab[q].c[1].y * invM};
+        R inn[2], inc[2], in3[2];
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            nonlinear(p, ph[s], co[s], aa[s], bb[s], inn[s], inc[s], in3[s]);
+            if (want_e) {
+                esum += energy_point<R>(p, ph[s], co[s], in3[s]);
+                dsum += (double)ph[s];
+            }
+        }
+        w[q].c[0] = Cx<R>{inn[0], inn[1]};
+        w[q].c[1] = Cx<R>{inc[0], inc[1]};
+    }
+    if (flags & ROW_WRITE_IN) row_r2c_store<R, H>(lds, t, w, twH, twM, (Vec<R, 2>*)b.IN + rbase);
+    if (want_e) {
+        const double se = block_sum<C::NT>(esum, red);
+        const double sd = block_sum<C::NT>(dsum, red);
+        if (tid == 0) {
+            const size_t slot = (size_t)inst * gridDim.x + blockIdx.x;
+            b.erow[2 * slot] = se;
+            b.erow[2 * slot + 1] = sd;
+        }
+    }
+}
+
+// Real planes (dpsi, dcon) -> S.
+template <typename R, int H>
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_in(DevBufs b, const R* __restrict__ inphi,
+                                                             const R* __restrict__ incon) {
+    using C = RowCfg<R, H>;
+    constexpr int E = C::E, G = C::G, RPW = C::RPW, K = C::K;
+    constexpr int M = 2 * H;
+    __shared__ Vec<R, 2> sm[RPW][K];
+    const int tid = threadIdx.x, rl = tid / G, t = tid % G;
+    const int inst = blockIdx.y, row = blockIdx.x * RPW + rl;
+    const size_t rbase = ((size_t)inst * b.L + row) * K;
+    const size_t pbase = ((size_t)inst * b.L + row) * M;
+    Vec<R, 2> w[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int j = t + G * q;
+        w[q].c[0] = Cx<R>{inphi[pbase + 2 * j], inphi[pbase + 2 * j + 1]};
+        w[q].c[1] = Cx<R>{incon[pbase + 2 * j], incon[pbase + 2 * j + 1]};
+    }
+    row_r2c_store<R, H>(sm[rl], t, w, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM, (Vec<R, 2>*)b.S + rbase);
+}
+
+// S -> real planes (psi, con) including the means.
+template <typename R, int H>
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __restrict__ outphi,
+                                                              R* __restrict__ outcon) {
+    using C = RowCfg<R, H>;
+    constexpr int E = C::E, G = C::G, RPW = C::RPW, K = C::K;
+    constexpr int M = 2 * H;
+    __shared__ Vec<R, 2> sm[RPW][K];
+    const int tid = threadIdx.x, rl = tid / G, t = tid % G;
+    const int inst = blockIdx.y, row = blockIdx.x * RPW + rl;
+    const size_t rbase = ((size_t)inst * b.L + row) * K;
+    const size_t pbase = ((size_t)inst * b.L + row) * M;
+    const Vec<R, 2>* S = (const Vec<R, 2>*)b.S;
+    Vec<R, 2>* lds = sm[rl];
+    for (int k = t; k < K; k += G) lds[k] = S[rbase + k];
+    __syncthreads();
+    Vec<R, 2> z[E];
+    row_c2r<R, H>(lds, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM, z);
+    const double* mu = b.means + 4 * inst;
+    const R mph = (R)mu[0], mco = (R)mu[1];
+    const R invM = R(1) / R(M);
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int j = t + G * q;
+        outphi[pbase + 2 * j] = mph + z[q].c[0].x * invM;
+        outphi[pbase + 2 * j + 1] = mph + z[q].c[0].y * invM;
+        outcon[pbase + 2 * j] = mco + z[q].c[1].x * invM;
+        outcon[pbase + 2 * j + 1] = mco + z[q].c[1].y * invM;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass
+// ---------------------------------------------------------------------------
+template <typename R, int LL>
+__global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs p, int mode, int flags) {
+    using C = ColCfg<R, LL>;
+    constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT;
+    __shared__ Vec<R, 2> sm[LL * T];
+    __shared__ double red[(NT + 63) / 64 + 1];
+    const int tid = threadIdx.x, col = tid % T, t = tid / T;
+    const int inst = blockIdx.y;
+    const int K = b.K;
+    const int kc0 = blockIdx.x * T, kc = kc0 + col;
+    const bool valid = kc < K;
+    const size_t ibase = (size_t)inst * LL * K;
+    Vec<R, 2>* S = (Vec<R, 2>*)b.S;
+    const Cx<R>* twL = (const Cx<R>*)b.twL;
+
+    for (int e = tid; e < LL * T; e += NT) {
+        const int i = e / T, c = e % T, kk = kc0 + c;
+        sm[e] = (kk < K) ? S[ibase + (size_t)i * K + kk] : vzero<R, 2>();
+    }
+    __syncthreads();
+    lds_fft<R, 2, LL, -1, T>(sm + col, t, true, twL);
+    Vec<R, 2> sh[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) sh[q] = sm[(t + G * q) * T + col];
+
+    if (mode == COL_STEP) {
+        const Vec<R, 2>* IN = (const Vec<R, 2>*)b.IN;
+        __syncthreads();
+        for (int e = tid; e < LL * T; e += NT) {
+            const int i = e / T, c = e % T, kk = kc0 + c;
+            sm[e] = (kk < K) ? IN[ibase + (size_t)i * K + kk] : vzero<R, 2>();
+        }
+        __syncthreads();
+        lds_fft<R, 2, LL, -1, T>(sm + col, t, true, twL);
+        if (valid) {
+            const float kmv = b.km[kc];
+#pragma unroll
+            for (int q = 0; q < E; q++) {
+                const int i = t + G * q;
+                const Vec<R, 2> oh = sm[i * T + col];
+                update_mode(p, ksq(b.kl[i], kmv), sh[q], oh);
+            }
+        }
+    }
+    if (kc == 0) {
+#pragma unroll
+        for (int q = 0; q < E; q++)
+            if (t + G * q == 0) sh[q] = vzero<R, 2>();
+    }
+    if (flags & COL_ENERGY) {
+        // sum_k (kl'^2 + 2 km'^2) |con_hat|^2 over this tile, with the own-direction
+        // Nyquist removed (Re of the reference's i k con_hat transform) and the
+        // half-spectrum columns 0 < kc < M/2 counted twice.
+        double g = 0.0;
+        if (valid) {
+            const int Mh = b.M / 2;
+            const double kmv = (kc == Mh) ? 0.0 : (double)b.km[kc];
+            const double mult = (kc == 0 || kc == Mh) ? 1.0 : 2.0;
+#pragma unroll
+            for (int q = 0; q < E; q++) {
+                const int i = t + G * q;
+                const double klv = (i == LL / 2) ? 0.0 : (double)b.kl[i];
+                const double cx = (double)sh[q].c[1].x, cy = (double)sh[q].c[1].y;
+                g += mult * (klv * klv + 2.0 * kmv * kmv) * (cx * cx + cy * cy);
+            }
+        }
+        const double gs = block_sum<NT>(g, red);
+        if (tid == 0) b.ecol[(size_t)inst * gridDim.x + blockIdx.x] = gs;
+    }
+    if (mode == COL_ENERGY_ONLY) return;
+
+    const R invL = R(1) / R(LL);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; q++) sm[(t + G * q) * T + col] = sh[q];
+    __syncthreads();
+    lds_fft<R, 2, LL, +1, T>(sm + col, t, true, twL);
+    for (int e = tid; e < LL * T; e += NT) {
+        const int i = e / T, c = e % T, kk = kc0 + c;
+        if (kk < K) S[ibase + (size_t)i * K + kk] = vscale(sm[e], invL);
+    }
+    __syncthreads();
+    {
+        const float* c2 = b.c2;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            const int i = t + G * q;
+            const int il = i <= LL / 2 ? i : LL - i;
+            Vec<R, 2> v;
+            if (valid) {
+                const float ca = c2[2 * ((size_t)il * K + kc)], cb = c2[2 * ((size_t)il * K + kc) + 1];
+                v.c[0] = Cx<R>{sh[q].c[0].x * (R)ca, sh[q].c[0].y * (R)ca};
+                v.c[1] = Cx<R>{sh[q].c[0].x * (R)cb, sh[q].c[0].y * (R)cb};
+            } else {
+                v = vzero<R, 2>();
+            }
+            sm[i * T + col] = v;
+        }
+    }
+    __syncthreads();
+    lds_fft<R, 2, LL, +1, T>(sm + col, t, true, twL);
+    Vec<R, 2>* AB = (Vec<R, 2>*)b.AB;
+    for (int e = tid; e < LL * T; e += NT) {
+        const int i = e / T, c = e % T, kk = kc0 + c;
+        if (kk < K) AB[ibase + (size_t)i * K + kk] = vscale(sm[e], invL);
+    }
+}
+
+// Per-instance energy: e_out = {(sum e + alpha/P sum_k w|c|^2)/L/M/N, sum psi/L/M/N}.
+__global__ void k_energy_reduce(DevBufs b, int nrow, int ncol, double alpha, double* e_out) {
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= b.B) return;
+    double se = 0.0, sd = 0.0, sg = 0.0;
+    for (int i = 0; i < nrow; i++) {
+        se += b.erow[2 * ((size_t)inst * nrow + i)];
+        sd += b.erow[2 * ((size_t)inst * nrow + i) + 1];
+    }
+    for (int i = 0; i < ncol; i++) sg += b.ecol[(size_t)inst * ncol + i];
+    const double P = (double)b.L * (double)b.M;
+    const double tot = se + alpha * sg / P;
+    e_out[2 * inst] = tot / b.L / b.M / 1;
+    e_out[2 * inst + 1] = sd / b.L / b.M / 1;
+}
+
+// ---------------------------------------------------------------------------
+// Host dispatch
+// ---------------------------------------------------------------------------
+bool size_supported(int n) {
+    switch (n) {
+        case 32: case 64: case 128: case 256: case 384: case 512: case 768: case 1024: return true;
+        default: return false;
+    }
+}
+
+template <typename R>
+static int row_blocks_t(int M, int L) {
+    int rpw = 1;
+    switch (M / 2) {
+#define RB(H) case H: rpw = RowCfg<R, H>::RPW; break;
+        RB(16) RB(32) RB(64) RB(128) RB(192) RB(256) RB(384) RB(512)
+#undef RB
+        default: return -1;
+    }
+    if (L % rpw) return -1;
+    return L / rpw;
+}
+
+template <typename R>
+static int col_blocks_t(int L, int M) {
+    int T = 1;
+    switch (L) {
+#define CB(LL) case LL: T = ColCfg<R, LL>::T; break;
+        CB(32) CB(64) CB(128) CB(256) CB(384) CB(512) CB(768) CB(1024)
+#undef CB
+        default: return -1;
+    }
+    const int K = M / 2 + 1;
+    return (K + T - 1) / T;
+}
+
+int row_blocks(int prec, int M, int L) {
+    return prec == PREC_FP64 ? row_blocks_t<double>(M, L) : row_blocks_t<float>(M, L);
+}
+int col_blocks(int prec, int L, int M) {
+    return prec == PREC_FP64 ? col_blocks_t<double>(L, M) : col_blocks_t<float>(L, M);
+}
+
+template <typename R>
+static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s) {
+    const int nb = row_blocks_t<R>(b.M, b.L);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.M / 2) {
+#define RS(H) case H: hipLaunchKernelGGL((k_row_step<R, H>), grid, dim3(RowCfg<R, H>::NT), 0, s, b, p, flags); break;
+        RS(16) RS(32) RS(64) RS(128) RS(192) RS(256) RS(384) RS(512)
+#undef RS
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s) {
+    const int nb = col_blocks_t<R>(b.L, b.M);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.L) {
+#define CS(LL) case LL: hipLaunchKernelGGL((k_col<R, LL>), grid, dim3(ColCfg<R, LL>::NT), 0, s, b, p, mode, flags); break;
+        CS(32) CS(64) CS(128) CS(256) CS(384) CS(512) CS(768) CS(1024)
+#undef CS
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t row_in_t(const DevBufs& b, const void* inphi, const void* incon, hipStream_t s) {
+    const int nb = row_blocks_t<R>(b.M, b.L);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.M / 2) {
+#define RI(H) case H: hipLaunchKernelGGL((k_row_in<R, H>), grid, dim3(RowCfg<R, H>::NT), 0, s, b, (const R*)inphi, (const R*)incon); break;
+        RI(16) RI(32) RI(64) RI(128) RI(192) RI(256) RI(384) RI(512)
+#undef RI
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t row_out_t(const DevBufs& b, void* outphi, void* outcon, hipStream_t s) {
+    const int nb = row_blocks_t<R>(b.M, b.L);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.M / 2) {
+#define RO(H) case H: hipLaunchKernelGGL((k_row_out<R, H>), grid, dim3(RowCfg<R, H>::NT), 0, s, b, (R*)outphi, (R*)outcon); break;
+        RO(16) RO(32) RO(64) RO(128) RO(192) RO(256) RO(384) RO(512)
+#undef RO
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s) {
+    return prec == PREC_FP64 ? row_step_t<double>(b, p, flags, s) : row_step_t<float>(b, p, flags, s);
+}
+hipError_t launch_col(int prec, const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s) {
+    return prec == PREC_FP64 ? col_t<double>(b, p, mode, flags, s) : col_t<float>(b, p, mode, flags, s);
+}
+hipError_t launch_row_in(int prec, const DevBufs& b, const void* inphi, const void* incon, hipStream_t s) {
+    return prec == PREC_FP64 ? row_in_t<double>(b, inphi, incon, s) : row_in_t<float>(b, inphi, incon, s);
+}
+hipError_t launch_row_out(int prec, const DevBufs& b, void* outphi, void* outcon, hipStream_t s) {
+    return prec == PREC_FP64 ? row_out_t<double>(b, outphi, outcon, s) : row_out_t<float>(b, outphi, outcon, s);
+}
+hipError_t launch_energy_reduce(const DevBufs& b, int nrow, int ncol, double alpha, double* e_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_energy_reduce, dim3((b.B + 63) / 64), dim3(64), 0, s, b, nrow, ncol, alpha, e_out);
+    return hipGetLastError();
+}
+
+}  // namespace pfc

@@ -1,0 +1,190 @@
+// pfc_io.cpp - file formats of pfc-exp-cli.  The byte layouts are the
+// reference's, so the Qt GUI's Visualizer and the elastic reader accept them.
+#include "pfc_io.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <filesystem>
+#include <set>
+
+namespace pfcio {
+
+std::string g6(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%g", v);
+    return b;
+}
+
+bool write_vti(const std::string& path, const char* name, int L, int M, int N, const float* field) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    std::fprintf(f, "<?xml version=\"1.0\"?>\n");
+    std::fprintf(f, "<VTKFile type=\"ImageData\" version=\"0.1\"  byte_order=\"LittleEndian\">\n");
+    std::fprintf(f, "<ImageData WholeExtent=\"%d %d %d %d %d %d\"   Origin=\"%d %d %d\" Spacing=\"%d %d %d\">\n", 0,
+                 L - 1, 0, M - 1, 0, N - 1, 0, 0, 0, 1, 1, 1);
+    std::fprintf(f, "<Piece Extent=\"%d %d %d %d %d %d\">\n", 0, L - 1, 0, M - 1, 0, N - 1);
+    std::fprintf(f, "<PointData Scalars=\"%s\">\n", name);
+    std::fprintf(f, "<DataArray type=\"Float32\" Name=\"%s\"   format=\"appended\" offset=\"%d\" />\n", name, 0);
+    std::fprintf(f, "</PointData>\n<CellData>\n</CellData>\n</Piece>\n</ImageData>\n");
+    std::fprintf(f, "<AppendedData encoding=\"raw\">\n_");
+    const int bytes = static_cast<int>(static_cast<size_t>(N) * M * L * sizeof(float)) + static_cast<int>(sizeof(int));
+    std::fwrite(&bytes, sizeof(int), 1, f);
+    // Payload order n, m, l (l fastest): a transpose of the n+N*(m+M*l) layout.
+    std::vector<float> buf(static_cast<size_t>(L));
+    for (int n = 0; n < N; n++)
+        for (int m = 0; m < M; m++) {
+            for (int l = 0; l < L; l++) buf[l] = field[n + static_cast<size_t>(N) * (m + static_cast<size_t>(M) * l)];
+            std::fwrite(buf.data(), sizeof(float), L, f);
+        }
+    std::fprintf(f, "\n</AppendedData>\n</VTKFile>\n");
+    return std::fclose(f) == 0;
+}
+
+bool write_pvti(const std::string& path, const char* name, const char* prefix, int kd, int L, int M, int N) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::fprintf(f, "<?xml version=\"1.0\"?>\n");
+    std::fprintf(f, "<VTKFile type=\"PImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n");
+    std::fprintf(f,
+                 "  <PImageData WholeExtent=\"0 %d 0 %d %d %d\" GhostLevel=\"#\" Origin=\"0  0  0\"  Spacing=\"1  1  1\">\n",
+                 L - 1, M - 1, 0, N - 1);
+    std::fprintf(f, "    <PPointData>\n      <PDataArray type=\"Float32\" Name=\"%s\"/>\n    </PPointData>\n", name);
+    std::fprintf(f, "    <Piece Extent=\"0 %d 0 %d %d %d\" Source=\"%s%d_0.vti\"/>\n", L - 1, M - 1, 0, N - 1, prefix, kd);
+    std::fprintf(f, "  </PImageData>\n</VTKFile>\n");
+    return std::fclose(f) == 0;
+}
+
+long read_vti(const std::string& path, int L, int M, int N, float* field) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return -1;
+    char line[100];
+    for (int i = 1; i < 13; i++)
+        if (!std::fgets(line, 100, f)) break;
+    (void)std::fgetc(f);
+    int bytes = 0;
+    if (std::fread(&bytes, sizeof(int), 1, f) != 1) bytes = 0;
+    long got = 0;
+    std::vector<float> row(static_cast<size_t>(L));
+    for (int n = 0; n < N; n++)
+        for (int m = 0; m < M; m++) {
+            const size_t r = std::fread(row.data(), sizeof(float), L, f);
+            for (size_t l = 0; l < r; l++) field[n + static_cast<size_t>(N) * (m + static_cast<size_t>(M) * l)] = row[l];
+            got += static_cast<long>(r);
+        }
+    std::fclose(f);
+    return got;
+}
+
+static bool finite_atom(double x, double y, double z, double v, double c) {
+    return std::isfinite(x) && std::isfinite(y) && std::isfinite(z) && std::isfinite(v) && std::isfinite(c);
+}
+
+bool write_phimax_misfit(const std::string& path, int n, const double* x, const double* y, const double* z,
+                         const double* phi, const float* con) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    int cnt = 0;
+    for (int i = 0; i < n; i++)
+        if (finite_atom(x[i], y[i], z[i], phi[i], con[i]) && !(x[i] < 0)) cnt++;
+    std::fprintf(f, "%d\nKommentar\n", 2 * cnt);
+    for (int i = 0; i < n; i++) {
+        const double c = con[i];
+        if (!finite_atom(x[i], y[i], z[i], phi[i], c) || x[i] < 0) continue;
+        std::fprintf(f, "%.4f %.4f %.4f %.4f %.4f\n", x[i], y[i], z[i], phi[i], c);
+        std::fprintf(f, "%.4f %.4f %.4f %.4f %.4f\n", x[i], y[i], z[i] + 20, phi[i], c);
+    }
+    return std::fclose(f) == 0;
+}
+
+bool write_phimax_elastic(const std::string& path, int n, const double* x, const double* y, const double* z,
+                          const double* phi) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::fprintf(f, "%d\nKommentar\n", n);
+    for (int i = 0; i < n; i++)
+        if (x[i] >= 0) std::fprintf(f, "%.4f %.4f %.4f %.4f\n", x[i], y[i], z[i], phi[i]);
+    return std::fclose(f) == 0;
+}
+
+bool write_strain(const std::string& path, int n, const double* x, const double* y, const double* z,
+                  const double* strain, const unsigned* nb) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::fprintf(f, "%d\nKommentar\n", n);
+    for (int i = 0; i < n; i++)
+        if (x[i] >= 0) std::fprintf(f, "st %.4f %.4f %.4f %.4f %u\n", x[i], y[i], z[i], strain[i], nb[i]);
+    return std::fclose(f) == 0;
+}
+
+bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float phimax, int n, const float* px,
+                     const float* py, const float* pz, const float* pphi) {
+    FILE* f = std::fopen("phimax.txt", "a");
+    if (!f) return false;
+    std::fprintf(f, "%d , %s , %s , %d\n", kd, g6(phic).c_str(), g6(phimax).c_str(), 0);
+    std::fprintf(f, "phimax is done myid=0\n buf is done \n pc_xyz is initializing \n pc_xyz is done \n");
+    std::fprintf(f, " pc_xyz has been gathered \n");
+    std::fclose(f);
+    const float frame_x = L * dx, frame_y = M * dx, frame_z = N * dx;
+    char name[64];
+    std::snprintf(name, sizeof name, "phiball_%d.cfg", kd);
+    f = std::fopen(name, "w");
+    if (!f) return false;
+    std::fprintf(f, "Number of particles = %d\n", n);
+    std::fprintf(f, "A = %.4f Angstrom (basic length-scale)\n", 10.0);
+    std::fprintf(f, "H0(1,1) = %.4f A \n", frame_y);
+    std::fprintf(f, "H0(1,2) = 0 A \nH0(1,3) = 0 A \nH0(2,1) = 0 A \n");
+    std::fprintf(f, "H0(2,2) = %.4f A \n", frame_z);
+    std::fprintf(f, "H0(2,3) = 0 A \nH0(3,1) = 0 A \nH0(3,2) = 0 A \n");
+    std::fprintf(f, "H0(3,3) = %.4f A \n", frame_x);
+    std::fprintf(f, ".NO_VELOCITY.\nentry_count = 3\n63.546\nCu\n");
+    for (int i = 0; i < n; i++) std::fprintf(f, "%.4f %.4f %.4f\n", px[i], py[i], pz[i]);
+    std::fclose(f);
+    std::snprintf(name, sizeof name, "Phimax_%d.txt", kd);
+    f = std::fopen(name, "w");
+    if (!f) return false;
+    std::fprintf(f, "%d\nKommentar\n", n);
+    for (int i = 0; i < n; i++)
+        std::fprintf(f, "%.4f %.4f %.4f %.4f\n", pz[i] * frame_x, px[i] * frame_y, py[i] * frame_z, pphi[i]);
+    return std::fclose(f) == 0;
+}
+
+std::vector<int> checkpoint_steps(const std::string& dir) {
+    std::vector<int> out;
+    namespace fs = std::filesystem;
+    const fs::path d = dir.empty() ? fs::current_path() : fs::path(dir);
+    std::error_code ec;
+    if (!fs::exists(d, ec) || ec) return out;
+    std::set<int> ps, cs;
+    for (const auto& e : fs::directory_iterator(d, ec)) {
+        if (ec) break;
+        if (!e.is_regular_file()) continue;
+        const std::string nm = e.path().filename().string();
+        int st = 0, rk = 0;
+        if (std::sscanf(nm.c_str(), "phi_%d_%d.vti", &st, &rk) == 2) { ps.insert(st); continue; }
+        if (std::sscanf(nm.c_str(), "con_%d_%d.vti", &st, &rk) == 2) cs.insert(st);
+    }
+    bool pos = false;
+    for (int s : ps) {
+        if (!cs.count(s)) continue;
+        if (s > 0) pos = true;
+        out.push_back(s);
+    }
+    std::sort(out.begin(), out.end());
+    if (pos) out.erase(std::remove(out.begin(), out.end(), 0), out.end());
+    return out;
+}
+
+bool make_dirs(const std::string& path) {
+    std::error_code ec;
+    std::filesystem::create_directories(path, ec);
+    return !ec;
+}
+
+long long now_ms() {
+    return std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::system_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace pfcio

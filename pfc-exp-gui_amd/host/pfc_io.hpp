@@ -1,0 +1,41 @@
+// pfc_io.hpp - file formats of pfc-exp-cli (what the Qt GUI reads).
+#pragma once
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace pfcio {
+
+// phi_<kd>_0.vti: 12 XML lines, '_', int32 (payload bytes + 4), float32
+// payload with l fastest, then the closing tags (misfit.cpp:1616-1698).
+bool write_vti(const std::string& path, const char* name, int L, int M, int N, const float* field);
+// phi_<kd>.pvti (misfit.cpp:1588-1614).
+bool write_pvti(const std::string& path, const char* name, const char* prefix, int kd, int L, int M, int N);
+// READPHIVTIN (elastic.cpp:2562-2608): returns values read, -1 if unopenable.
+long read_vti(const std::string& path, int L, int M, int N, float* field);
+
+// Phimax_<kd>.txt of the Misfit model: header 2*count, each atom twice
+// (z and z+20), columns x y z phi con (misfit.cpp:3495-3536).
+bool write_phimax_misfit(const std::string& path, int n, const double* x, const double* y, const double* z,
+                         const double* phi, const float* con);
+// Elastic Phimax_<kd>.txt (elastic.cpp:2953-2968).
+bool write_phimax_elastic(const std::string& path, int n, const double* x, const double* y, const double* z,
+                          const double* phi);
+// strain_<kd>.txt (elastic.cpp:3738-3752).
+bool write_strain(const std::string& path, int n, const double* x, const double* y, const double* z,
+                  const double* strain, const unsigned* nb);
+// CVD checkpoint outputs of atompositionCFGC2 (cvd.cpp:1086-1249):
+// phimax.txt (append), phiball_<kd>.cfg and Phimax_<kd>.txt.
+bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float phimax, int n, const float* px,
+                     const float* py, const float* pz, const float* pphi);
+
+// Default iostream formatting of a double (%g, 6 significant digits).
+std::string g6(double v);
+
+// CollectAvailableCheckpointSteps (elastic.cpp:37-74).
+std::vector<int> checkpoint_steps(const std::string& dir);
+
+bool make_dirs(const std::string& path);
+long long now_ms();
+
+}  // namespace pfcio

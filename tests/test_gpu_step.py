@@ -1,0 +1,133 @@
+"""GPU step parity: the HIP path against the CPU oracle on the reference's
+own seeded initial fields (INITIAL_dis / INITIAL_C2), at sizes the oracle
+finishes in seconds.
+
+Tolerances (field L-infinity, after n steps):
+  FP64, narrowing off  : 1e-11  (only FFT round-off differs)
+  FP64, narrowing on   : 1e-6   (a 1e-16 difference can flip one float
+                                 rounding of a spectral coefficient, 6e-8 rel.)
+  FP32                 : 2e-4 at 50 steps
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(hip, oracle, kind, L, M, narrow, prec, **kw):
+    p = hip.default_params(model=kind, L=L, M=M, precision=prec, **kw)
+    p.narrow = narrow
+    m = oracle.model(kind, L, M, 1, narrow=narrow, **kw)
+    phi, con = hip.init_fields(p)
+    ophi, ocon = oracle.init_fields(m)
+    assert np.array_equal(phi, ophi) and np.array_equal(con, ocon), "initial fields differ"
+    sim = hip.Sim(p)
+    sim.set_fields(phi, con)
+    osim = oracle.Sim(m, phi, con)
+    return sim, osim
+
+
+@pytest.mark.parametrize("kind,L,M,kw", [
+    (0, 64, 64, dict(grainx=4, grainy=4)),
+    (1, 64, 64, {}),
+    (0, 128, 384, dict(grainx=4, grainy=8)),
+    (1, 384, 64, {}),
+])
+def test_fp64_exact_math(hip, oracle, kind, L, M, kw):
+    sim, osim = _pair(hip, oracle, kind, L, M, 0, hip.FP64, **kw)
+    for n in (1, 9, 20):
+        sim.step(n)
+        osim.run(n)
+        phi, con = sim.get_fields()
+        ophi, ocon = osim.fields()
+        err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
+        print(f"kind={kind} {L}x{M} t={osim.t} fp64 exact err={err:.3e}", flush=True)
+        assert err < 1e-11
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_fp64_narrow(hip, oracle, kind):
+    kw = dict(grainx=4, grainy=4) if kind == 0 else {}
+    sim, osim = _pair(hip, oracle, kind, 64, 64, 1, hip.FP64, **kw)
+    sim.step(50)
+    osim.run(50)
+    phi, con = sim.get_fields()
+    ophi, ocon = osim.fields()
+    err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
+    print(f"kind={kind} fp64 narrow err={err:.3e}", flush=True)
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_fp32(hip, oracle, kind):
+    kw = dict(grainx=4, grainy=4) if kind == 0 else {}
+    sim, osim = _pair(hip, oracle, kind, 64, 64, 1, hip.FP32, **kw)
+    sim.step(50)
+    osim.run(50)
+    phi, con = sim.get_fields()
+    ophi, ocon = osim.fields()
+    err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
+    print(f"kind={kind} fp32 err={err:.3e}", flush=True)
+    assert err < 2e-4
+    # k=0 modes held in FP64: means conserved exactly as the oracle's
+    assert abs(phi.mean() - ophi.mean()) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_energy(hip, oracle, prec):
+    sim, osim = _pair(hip, oracle, 0, 64, 64, 1, prec, grainx=4, grainy=4)
+    sim.step(25)
+    osim.run(25)
+    e = sim.energy()[0]
+    oe, orho = osim.energy()
+    print(f"prec={prec} energy {e} oracle {oe} {orho}", flush=True)
+    assert abs(e[0] - oe) < 1e-6 * max(1.0, abs(oe)) + (1e-9 if prec == 0 else 1e-6)
+    assert abs(e[1] - orho) < 1e-9
+
+
+def test_step_energy_trace(hip, oracle):
+    sim, osim = _pair(hip, oracle, 1, 64, 64, 1, hip.FP64)
+    tr = sim.step_energy(6)
+    for t in range(6):
+        osim.run(1)
+        oe, orho = osim.energy()
+        assert abs(tr[t, 0, 0] - oe) < 1e-9 + 1e-6 * abs(oe), (t, tr[t, 0], oe)
+        assert abs(tr[t, 0, 1] - orho) < 1e-9
+
+
+def test_graph_replay_matches_eager(hip):
+    p = hip.default_params(model=0, L=64, M=64, precision=hip.FP64, grainx=4, grainy=4)
+    phi, con = hip.init_fields(p)
+    a = hip.Sim(p)
+    a.set_fields(phi, con)
+    a.step(24)
+    ra = a.get_fields()
+    b = hip.Sim(p)
+    b.set_fields(phi, con)
+    b.set_graph(8)
+    for _ in range(3):          # three replays with host syncs between them
+        b.step(8)
+        b.sync()
+    rb = b.get_fields()
+    assert np.array_equal(ra[0], rb[0]) and np.array_equal(ra[1], rb[1])
+
+
+def test_batch_equals_single(hip):
+    """Each instance of a batch gives bit-identical results to running alone."""
+    p = hip.default_params(model=0, L=64, M=64, precision=hip.FP32, grainx=4, grainy=4)
+    fields = []
+    for s in range(3):
+        p.seed = 1000 + s
+        p.u0 = 0.02 * (s + 1)
+        fields.append(hip.init_fields(p))
+    b = hip.Sim(p, batch=3)
+    for i, (phi, con) in enumerate(fields):
+        b.set_fields(phi, con, inst=i)
+    b.step(30)
+    for i, (phi, con) in enumerate(fields):
+        s1 = hip.Sim(p)
+        s1.set_fields(phi, con)
+        s1.step(30)
+        x = s1.get_fields()
+        y = b.get_fields(i)
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
