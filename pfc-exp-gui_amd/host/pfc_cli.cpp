@@ -143,7 +143,7 @@ int parse(Kind k, int argc, char** argv, Opts& o) {
         // Flags a model does not know behave as unknown flags: rejected by
         // misfit (misfit.cpp:269-272), ignored by cvd/elastic (cvd.cpp:192-193).
         const bool known = key >= 0 && !((k != ELASTIC && key >= 18) || (k == CVD && key >= 8 && key <= 11) ||
-                                          (k == ELASTIC && key == 11) || (k != MISFIT && key == 17));
+                                          (k == ELASTIC && (key == 11 || key == 17)));
         if (!known) {
             if (k == MISFIT) {
                 std::fprintf(stderr, "Unknown option: %s\n", raw);

@@ -1,0 +1,100 @@
+"""CPU tests of the product's host layer: the C ABI library loads and exports
+every symbol include/pfc_hip.h declares, the host-side reference
+initialisers equal the oracle's bit for bit, and the CLI honours the
+reference's argument / exit-code contract on paths that end before any GPU
+call (main_cli.cpp:24-88, misfit.cpp:181-309, elastic.cpp:188-289,
+pfc_common.h:249-269)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "pfc-exp-gui_amd", "pfc-exp-cli")
+HEADER = os.path.join(ROOT, "include", "pfc_hip.h")
+
+
+def test_library_exports_header_symbols(hip):
+    text = open(HEADER).read()
+    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(pfc_\w+)\s*\(", text, re.M))
+    assert len(declared) >= 19
+    lib = hip.load()
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared == set(hip.SYMBOLS)
+
+
+def test_params_defaults(hip):
+    p = hip.default_params(hip.MISFIT)
+    assert (p.u0, p.con0, p.sig, p.dt, p.dx) == tuple(float(np.float32(v)) for v in (0.05, 0.2, 0.05, 0.05, 0.125))
+    assert (p.grainx, p.grainy, p.grainz, p.seed, p.narrow) == (16, 16, 1, 20200604, 1)
+    q = hip.default_params(hip.CVD)
+    assert (q.u0, q.sig) == (float(np.float32(0.01)), float(np.float32(0.03)))
+
+
+@pytest.mark.parametrize("kind,L,M,kw", [
+    (0, 256, 256, {}),
+    (0, 128, 384, dict(grainx=4, grainy=8, seed=7, u0=0.04, con0=0.3)),
+    (0, 64, 64, dict(grainx=4, grainy=4, seed=123456)),
+    (1, 128, 64, dict(u0=0.05, con0=0.25)),
+])
+def test_init_fields_match_oracle(hip, oracle, kind, L, M, kw):
+    p = hip.default_params(model=kind, L=L, M=M, **kw)
+    m = oracle.model(kind, L, M, 1, **kw)
+    phi, con = hip.init_fields(p)
+    ophi, ocon = oracle.init_fields(m)
+    assert np.array_equal(phi, ophi) and np.array_equal(con, ocon)
+
+
+def _cli(args, cwd):
+    return subprocess.run([CLI] + args, cwd=cwd, capture_output=True, text=True, timeout=60)
+
+
+def test_cli_global_help(tmp_path):
+    r = _cli(["--help"], tmp_path)
+    assert r.returncode == 0 and "Models:" in r.stderr
+
+
+@pytest.mark.parametrize("args", [
+    ["--model", "nosuch"],
+    ["--model"],
+    ["--model", "misfit", "--bogus", "1"],
+    ["--steps", "0"],
+    ["--dt=-1"],
+    ["--u0", "abc"],
+    ["--steps"],
+    ["--grainx", "512"],
+    ["--mod", "0"],
+])
+def test_cli_bad_arguments_exit_2(tmp_path, args):
+    r = _cli(args, tmp_path)
+    assert r.returncode == 2, (args, r.stderr)
+    assert not (tmp_path / "run_config.txt").exists()
+
+
+def test_cli_model_help(tmp_path):
+    r = _cli(["--model", "cvd", "--help"], tmp_path)
+    assert r.returncode == 0 and "--model cvd" in r.stderr
+
+
+def test_cli_outdir_failure(tmp_path):
+    blocker = tmp_path / "file"
+    blocker.write_text("x")
+    r = _cli(["--outdir", str(blocker / "sub")], tmp_path)
+    assert r.returncode == 2 and "outdir" in r.stderr
+
+
+def test_cli_elastic_ignores_unknown_and_missing_input(tmp_path):
+    """elastic ignores unknown flags (elastic.cpp:279); with no checkpoint
+    files it falls back to mod-spaced steps and exits 2 on the missing VTI
+    (elastic.cpp:2572-2575) after writing run_config.txt."""
+    out = tmp_path / "out"
+    r = _cli(["--model", "elastic", "--whatever", "3", "--input-dir", str(tmp_path / "empty"),
+              "--outdir", str(out), "--steps", "400", "--mod", "200", "--benchel", "1.5"], tmp_path)
+    assert r.returncode == 2 and "can't open input file" in r.stderr
+    cfg = (out / "run_config.txt").read_text().splitlines()
+    assert cfg[0] == "model elastic" and "benchel 1.5" in cfg
+    assert (out / "checkpoint_timestamps.txt").read_text().startswith("# Checkpoint timestamps")
+    assert "PFC_PROGRESS step=1 total=2" in r.stderr
