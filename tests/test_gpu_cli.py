@@ -1,0 +1,114 @@
+"""End-to-end: the product pfc-exp-cli (GPU) against the oracle CLI (CPU) on
+GUI-style argv (MainWindow.cpp:2401-2428 plus the runtime-grid flags).
+Same file names, byte-identical run_config / PVTI / initial VTI / VTI
+headers, field payloads within FP64 round-off, energy.txt within its printed
+precision, Phimax atoms matched, and the elastic post-processor byte-identical
+on the same VTI input (integer/index work and unfused IEEE arithmetic)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "pfc-exp-gui_amd", "pfc-exp-cli")
+OCLI = os.path.join(ROOT, "oracle", "pfc-oracle-cli")
+PROGRESS = re.compile(r"^PFC_PROGRESS\s+step=(\d+)\s+total=(\d+)\s*$")
+
+
+def run(exe, args, cwd):
+    r = subprocess.run([exe] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (exe, args, r.stderr[-2000:])
+    return r
+
+
+def vti_split(path):
+    raw = open(path, "rb").read()
+    i = raw.index(b"\n_") + 2
+    n = int(np.frombuffer(raw[i:i + 4], np.int32)[0])
+    return raw[:i + 4], np.frombuffer(raw[i + 4:i + n], np.float32), raw[i + n:], n
+
+
+def read_table(path):
+    lines = open(path).read().splitlines()
+    return int(lines[0]), lines[1], np.array([[float(t) for t in l.split() if t != "st"] for l in lines[2:]])
+
+
+def test_misfit_cli_matches_oracle(tmp_path):
+    args = ["--u0", "0.05", "--con0", "0.2", "--sig", "0.05", "--dt", "0.05", "--dx", "0.125", "--steps", "50",
+            "--mod", "25", "--seed", "20200604", "--L", "64", "--M", "64", "--grainx", "4", "--grainy", "4"]
+    g, o = tmp_path / "gpu", tmp_path / "cpu"
+    r = run(CLI, ["--model", "misfit"] + args + ["--outdir", str(g)], tmp_path)
+    run(OCLI, ["--model", "misfit"] + args + ["--outdir", str(o)], tmp_path)
+    steps = [int(PROGRESS.match(l).group(1)) for l in r.stdout.splitlines() if PROGRESS.match(l)]
+    assert steps[0] == 1 and steps[-1] == 50
+    assert sorted(os.listdir(g)) == sorted(os.listdir(o))
+    for f in ("run_config.txt", "phi_0_0.vti", "con_0_0.vti", "phi_25.pvti", "con_50.pvti"):
+        assert (g / f).read_bytes() == (o / f).read_bytes(), f
+    for kd in (25, 50):
+        for fld in ("phi", "con"):
+            hg, pg, tg, ng = vti_split(g / f"{fld}_{kd}_0.vti")
+            ho, po, to, no = vti_split(o / f"{fld}_{kd}_0.vti")
+            assert hg == ho and tg == to and ng == no
+            assert np.abs(pg - po).max() < 1e-6
+        cg, _, ag = read_table(g / f"Phimax_{kd}.txt")
+        co, _, ao = read_table(o / f"Phimax_{kd}.txt")
+        assert cg == co
+        assert np.abs(ag - ao).max() <= 2e-4
+    eg = np.loadtxt(g / "energy.txt")
+    eo = np.loadtxt(o / "energy.txt")
+    assert np.array_equal(eg[:, 0], eo[:, 0])
+    assert np.allclose(eg[:, 1:], eo[:, 1:], rtol=2e-5, atol=1e-9)
+    ck = [l.split()[1] for l in (g / "checkpoint_timestamps.txt").read_text().splitlines()[1:]]
+    assert ck == ["25", "50"]
+
+
+def test_cvd_cli_matches_oracle(tmp_path):
+    args = ["--steps", "21", "--mod", "10", "--L", "64", "--M", "64", "--u0", "0.05", "--con0", "0.25"]
+    g, o = tmp_path / "gpu", tmp_path / "cpu"
+    r = run(CLI, ["--model", "cvd"] + args + ["--outdir", str(g)], tmp_path)
+    run(OCLI, ["--model", "cvd"] + args + ["--outdir", str(o)], tmp_path)
+    prog = [l for l in r.stderr.splitlines() if PROGRESS.match(l)]
+    assert len(prog) == 21
+    assert sorted(os.listdir(g)) == sorted(os.listdir(o))
+    eg = np.loadtxt(g / "energy.txt")
+    eo = np.loadtxt(o / "energy.txt")
+    assert eg.shape == (21, 3) and np.array_equal(eg[:, 0], np.arange(21))
+    assert np.allclose(eg[:, 1:], eo[:, 1:], rtol=2e-5, atol=1e-9)
+    for kd in (0, 10, 20):
+        cg, _, ag = read_table(g / f"Phimax_{kd}.txt")
+        co, _, ao = read_table(o / f"Phimax_{kd}.txt")
+        assert cg == co and np.abs(ag - ao).max() <= 2e-4
+        assert (g / f"phiball_{kd}.cfg").read_text().splitlines()[:15] == \
+            (o / f"phiball_{kd}.cfg").read_text().splitlines()[:15]
+    pg = (g / "phimax.txt").read_text().splitlines()
+    po = (o / "phimax.txt").read_text().splitlines()
+    assert len(pg) == len(po) == 18
+    assert [l for l in pg if " , " not in l] == [l for l in po if " , " not in l]
+
+
+def test_elastic_cli_bit_identical(tmp_path):
+    src = tmp_path / "src"
+    run(OCLI, ["--model", "misfit", "--L", "128", "--M", "128", "--grainx", "4", "--grainy", "4", "--steps", "40",
+               "--mod", "20", "--outdir", str(src)], tmp_path)
+    g, o = tmp_path / "g", tmp_path / "o"
+    common = ["--model", "elastic", "--L", "128", "--M", "128", "--input-dir", str(src)]
+    r = run(CLI, common + ["--outdir", str(g)], tmp_path)
+    run(OCLI, common + ["--outdir", str(o)], tmp_path)
+    assert "PFC_PROGRESS step=2 total=2" in r.stderr     # steps 20, 40 (0 dropped)
+    for kd in (20, 40):
+        for f in (f"Phimax_{kd}.txt", f"strain_{kd}.txt"):
+            assert (g / f).read_bytes() == (o / f).read_bytes(), f
+    assert not (g / "Phimax_0.txt").exists()
+
+
+def test_fp32_cli_runs(tmp_path):
+    g = tmp_path / "g"
+    run(CLI, ["--model", "misfit", "--precision", "fp32", "--L", "128", "--M", "128", "--steps", "30",
+              "--mod", "15", "--grainx", "8", "--grainy", "8", "--outdir", str(g)], tmp_path)
+    assert (g / "Phimax_30.txt").exists() and (g / "phi_30_0.vti").exists()
+    e = np.loadtxt(g / "energy.txt")
+    assert e.shape == (2, 3) and np.all(np.isfinite(e))
