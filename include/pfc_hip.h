@@ -65,6 +65,11 @@ int pfc_destroy(pfc_ctx* ctx);
 /* Load real fields of instance `inst` (host doubles, L*M*N each). */
 int pfc_set_fields(pfc_ctx* ctx, int inst, const double* phi, const double* con);
 
+/* Seed of instance `inst` for the Misfit noise stream (Gauss,
+ * misfit.cpp:951-956); defaults to the creation parameters' seed.  Loading
+ * fields with pfc_set_fields restarts the instance's step index at 0. */
+int pfc_set_seed(pfc_ctx* ctx, int inst, unsigned int seed);
+
 /* Advance every instance by nsteps reference time steps (the `for t` body,
  * misfit.cpp:477-498 / cvd.cpp:400-419).  Enqueues and returns; see pfc_sync. */
 int pfc_step(pfc_ctx* ctx, int nsteps);

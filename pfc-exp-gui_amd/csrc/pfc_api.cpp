@@ -48,6 +48,12 @@ struct pfc_ctx {
     double *erow = nullptr, *ecol[2] = {nullptr, nullptr}, *eout = nullptr;
     int nrow = 0, ncol = 0;
     void *stage_a = nullptr, *stage_b = nullptr;
+    long long* tcount = nullptr;  // [B] step index per instance (the noise RNG's t)
+    // noise term (misfit --axx > 0)
+    bool noise = false;
+    unsigned* seeds_d = nullptr;
+    void *nplanes = nullptr, *nz = nullptr, *nscratch = nullptr;
+    int* nerr = nullptr;
     float c2a0 = 0.f, c2b0 = 0.f;
     PhysArgs phys{};
     DevBufs bufs{};
@@ -98,7 +104,8 @@ static void free_ctx(pfc_ctx* c) {
     if (c->graph) hipGraphDestroy(c->graph);
     for (auto e : c->ev) hipEventDestroy(e);
     void* ptrs[] = {c->S, c->AB, c->IN, c->means_d, c->twH, c->twM, c->twL, c->kl_d, c->km_d, c->c2_d,
-                    c->erow, c->ecol[0], c->ecol[1], c->eout, c->stage_a, c->stage_b};
+                    c->erow, c->ecol[0], c->ecol[1], c->eout, c->stage_a, c->stage_b, c->tcount,
+                    c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -115,8 +122,6 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
         return fail(PFC_E_UNSUPPORTED, "grid %dx%d not supported (sizes: 32,64,128,256,384,512,768,1024)", p->L, p->M);
     if (p->dt <= 0.0f || p->dx <= 0.0f) return fail(PFC_E_ARG, "dt and dx must be > 0");
     Model m = make_model(*p);
-    if (p->model == PFC_MODEL_MISFIT && m.b0 != 0.0f)
-        return fail(PFC_E_UNSUPPORTED, "--axx > 0 (noise term) is not supported by the GPU path");
     const int prec = p->precision == PFC_PREC_FP64 ? PREC_FP64 : PREC_FP32;
     const int nrow = row_blocks(prec, p->M, p->L), ncol = col_blocks(prec, p->L, p->M);
     if (nrow <= 0 || ncol <= 0) return fail(PFC_E_UNSUPPORTED, "no launch shape for %dx%d", p->L, p->M);
@@ -143,6 +148,19 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&c->ecol[0], sizeof(double) * ncol * batch);
     if (e == hipSuccess) e = hipMalloc(&c->ecol[1], sizeof(double) * ncol * batch);
     if (e == hipSuccess) e = hipMalloc(&c->eout, sizeof(double) * 2 * batch);
+    if (e == hipSuccess) e = hipMalloc(&c->tcount, sizeof(long long) * batch);
+    if (e == hipSuccess) e = hipMemsetAsync(c->tcount, 0, sizeof(long long) * batch, c->stream);
+    c->noise = (p->model == PFC_MODEL_MISFIT && m.b0 != 0.0f);
+    if (c->noise) {
+        std::vector<unsigned> sd(batch, p->seed);
+        if (e == hipSuccess) e = hipMalloc(&c->seeds_d, sizeof(unsigned) * batch);
+        if (e == hipSuccess) e = hipMemcpy(c->seeds_d, sd.data(), sizeof(unsigned) * batch, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&c->nplanes, nreal * c->rsize);
+        if (e == hipSuccess) e = hipMalloc(&c->nz, nvec * c->vsize / 2);
+        if (e == hipSuccess) e = hipMalloc(&c->nscratch, noise_scratch_bytes((long)c->L * c->M, batch));
+        if (e == hipSuccess) e = hipMalloc(&c->nerr, sizeof(int));
+        if (e == hipSuccess) e = hipMemsetAsync(c->nerr, 0, sizeof(int), c->stream);
+    }
     if (e == hipSuccess) e = hipMemsetAsync(c->S, 0, nvec * c->vsize, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->AB, 0, nvec * c->vsize, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->IN, 0, nvec * c->vsize, c->stream);
@@ -180,12 +198,15 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     ph.mc_dt = m.mc * m.dt;
     ph.mc_dt_a = m.mc * m.dt * m.alpha;
     ph.narrow = p->narrow ? 1 : 0;
+    ph.db = c->noise ? m.dt * m.b0 : 0.0f;
     DevBufs& b = c->bufs;
     b.S = c->S; b.AB = c->AB; b.IN = c->IN;
     b.means = c->means_d;
     b.twH = c->twH; b.twM = c->twM; b.twL = c->twL;
     b.kl = c->kl_d; b.km = c->km_d; b.c2 = c->c2_d;
     b.erow = c->erow; b.ecol = c->ecol[0];
+    b.NZ = c->noise ? c->nz : nullptr;
+    b.tcount = c->tcount;
     b.L = c->L; b.M = c->M; b.K = c->K; b.B = batch;
     *out = c;
     return PFC_OK;
@@ -208,6 +229,8 @@ static DevBufs inst_bufs(const pfc_ctx* c, int inst) {
     b.means = c->means_d + 4 * inst;
     b.erow = c->erow + 2 * (size_t)c->nrow * inst;
     b.ecol = c->ecol[0] + (size_t)c->ncol * inst;
+    b.NZ = c->noise ? (char*)c->nz + off / 2 : nullptr;
+    b.tcount = c->tcount + inst;
     b.B = 1;
     return b;
 }
@@ -236,6 +259,8 @@ extern "C" int pfc_set_fields(pfc_ctx* c, int inst, const double* phi, const dou
     HIP_TRY(hipMemcpyAsync((char*)c->stage_a + poff, ha.data(), P * c->rsize, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync((char*)c->stage_b + poff, hb.data(), P * c->rsize, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->means_d + 4 * inst, mu, sizeof(double) * 4, hipMemcpyHostToDevice, c->stream));
+    const long long zero = 0;
+    HIP_TRY(hipMemcpyAsync(c->tcount + inst, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
     DevBufs b = inst_bufs(c, inst);
     HIP_TRY(launch_row_in(c->prec, b, (char*)c->stage_a + poff, (char*)c->stage_b + poff, c->stream));
     HIP_TRY(launch_col(c->prec, b, c->phys, COL_INIT, 0, c->stream));
@@ -258,6 +283,10 @@ static hipError_t enqueue_step(pfc_ctx* c, int row_flags, int col_flags, int eco
     b.ecol = c->ecol[ecol_slot];
     const bool t0 = c->time_kid == KID_ROW, t1 = c->time_kid == KID_COL;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->noise) {
+        hipError_t en = launch_noise(c->prec, b, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr, c->stream);
+        if (en != hipSuccess) return en;
+    }
     if (t0) { e0 = next_event(c); e1 = next_event(c); if (e0) hipEventRecord(e0, c->stream); }
     hipError_t e = launch_row_step(c->prec, b, c->phys, row_flags, c->stream);
     if (t0 && e1) hipEventRecord(e1, c->stream);
@@ -304,6 +333,19 @@ extern "C" int pfc_step(pfc_ctx* c, int nsteps) {
 
 extern "C" int pfc_sync(pfc_ctx* c) {
     if (!c) return fail(PFC_E_ARG, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->noise) {
+        int err = 0;
+        HIP_TRY(hipMemcpy(&err, c->nerr, sizeof(int), hipMemcpyDeviceToHost));
+        if (err) return fail(PFC_E_HIP, "noise stream: too few accepted Marsaglia pairs");
+    }
+    return PFC_OK;
+}
+
+extern "C" int pfc_set_seed(pfc_ctx* c, int inst, unsigned int seed) {
+    if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
+    if (!c->noise) return PFC_OK;
+    HIP_TRY(hipMemcpyAsync(c->seeds_d + inst, &seed, sizeof(seed), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return PFC_OK;
 }

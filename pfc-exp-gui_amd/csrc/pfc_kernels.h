@@ -23,6 +23,7 @@ struct PhysArgs {
     float mc_dt;    // dynamic_mc*dt            (float, misfit.cpp:1007)
     float mc_dt_a;  // dynamic_mc*dt*para_alpha (float, misfit.cpp:1009)
     int narrow;     // reference float narrowing of Rhat/inc/energy terms
+    float db;       // dt*b0 (float, misfit.cpp:1002); 0 = no noise term
 };
 
 struct DevBufs {
@@ -38,6 +39,8 @@ struct DevBufs {
     const float* c2;      // [(L/2+1)][K][2] folded {C2A, C2B}
     double* erow;         // [B][row_blocks][2]  {sum e_point, sum psi}
     double* ecol;         // [B][col_blocks]     sum w(k)|con_hat|^2
+    const void* NZ;       // [B][L][K] complex: row-spectral noise (nullptr: no noise)
+    long long* tcount;    // [B] step counter per instance (advanced by the column pass)
     int L, M, K, B;
 };
 
@@ -53,9 +56,18 @@ hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int fl
 hipError_t launch_col(int prec, const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s);
 hipError_t launch_row_in(int prec, const DevBufs& b, const void* inphi, const void* incon, hipStream_t s);
 hipError_t launch_row_out(int prec, const DevBufs& b, void* outphi, void* outcon, hipStream_t s);
+// One real plane per instance [B][L][M] -> row spectrum [B][L][K] (single field).
+hipError_t launch_row_r2c1(int prec, const DevBufs& b, const void* in, void* out, hipStream_t s);
 // e_out[inst*2 + {0,1}] = {mean energy, mean density}; alpha_over_P = alpha/P.
 hipError_t launch_energy_reduce(const DevBufs& b, int nrow, int ncol, double alpha, double* e_out,
                                 hipStream_t s);
+
+// Misfit noise (misfit.cpp:951-981): real noise planes [B][L][M] for the
+// current step of each instance, then NZ = r2c along m of those planes.
+// seeds: [B] run seeds; scratch sized by noise_scratch_bytes(P, B).
+size_t noise_scratch_bytes(long P, int B);
+hipError_t launch_noise(int prec, const DevBufs& b, const unsigned* seeds, void* planes, void* nz, void* scratch,
+                        int* err_flag, hipStream_t s);
 
 // Named kernels (for timing the dominant one).
 enum { KID_ROW = 0, KID_COL = 1 };

@@ -224,9 +224,12 @@ __global__ void __launch_bounds__(PB) k_count(Op op, long P, int* counts) {
     if (threadIdx.x == 0) counts[blockIdx.x] = total;
 }
 
-// Exclusive scan of nb counts by one workgroup; offsets[nb] = total.
+// Exclusive scan of nb counts by one workgroup per blockIdx.y batch;
+// offsets[nb] = total.  Batch y uses counts + y*nb and offsets + y*(nb+1).
 __global__ void __launch_bounds__(1024) k_scan(const int* counts, int nb, int* offsets) {
     __shared__ int part[1024];
+    counts += (size_t)blockIdx.y * nb;
+    offsets += (size_t)blockIdx.y * (nb + 1);
     const int t = threadIdx.x;
     const int per = (nb + 1023) / 1024;
     const int b0 = t * per, b1 = min(nb, b0 + per);
@@ -537,6 +540,111 @@ hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, cons
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpy(strain, ds, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
     return hipMemcpy(nb, dn, sizeof(unsigned) * n, hipMemcpyDeviceToHost);
+}
+
+// ---------------------------------------------------------------------------
+// Misfit noise field (Gauss, misfit.cpp:951-981).  The reference draws, point
+// by point in memory order, Marsaglia polar pairs (v1, v2) = two successive
+// u11 draws of one splitmix64 stream until 0 < v1^2 + v2^2 < 1.  splitmix64's
+// k-th output is mix(s0 + k*gamma), so pair j is decided independently; point
+// p takes the p-th accepted pair.  Count accepted pairs per block, scan, emit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t sm64_at(uint64_t s0, uint64_t k) {
+    uint64_t z = s0 + k * 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ float u11f(uint64_t x) {
+#pragma clang fp contract(off)
+    const double u = (double)(x >> 11) * (1.0 / 9007199254740992.0);
+    return (float)(2.0 * u - 1.0);
+}
+
+__device__ __forceinline__ uint64_t noise_state(unsigned seed, long long t) {
+    uint64_t st = (uint64_t)seed;
+    st ^= (uint64_t)(unsigned)(0 + 1) * 2654435761ULL;
+    st ^= (uint64_t)(unsigned)t * 374761393ULL;
+    return st;
+}
+
+struct NoisePair {
+    float v1, r;
+    bool ok;
+};
+
+__device__ __forceinline__ NoisePair noise_pair(uint64_t s0, long j) {
+#pragma clang fp contract(off)
+    NoisePair q;
+    q.v1 = u11f(sm64_at(s0, (uint64_t)(2 * j + 1)));
+    const float v2 = u11f(sm64_at(s0, (uint64_t)(2 * j + 2)));
+    q.r = q.v1 * q.v1 + v2 * v2;
+    q.ok = !(q.r >= 1.0f || q.r <= 0.0f);
+    return q;
+}
+
+__global__ void __launch_bounds__(PB) k_noise_count(const unsigned* seeds, const long long* tcount, long NP,
+                                                    int nb, int* counts) {
+    __shared__ int wsum[PB / 64];
+    const int inst = blockIdx.y;
+    const long j = (long)blockIdx.x * PB + threadIdx.x;
+    const uint64_t s0 = noise_state(seeds[inst], tcount[inst]);
+    const bool f = j < NP && noise_pair(s0, j).ok;
+    int total;
+    (void)block_rank(f, wsum, total);
+    if (threadIdx.x == 0) counts[(size_t)inst * nb + blockIdx.x] = total;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(PB) k_noise_emit(const unsigned* seeds, const long long* tcount, long NP, long P,
+                                                   int nb, const int* offsets, R* planes, int* err) {
+    __shared__ int wsum[PB / 64];
+    const int inst = blockIdx.y;
+    const long j = (long)blockIdx.x * PB + threadIdx.x;
+    const uint64_t s0 = noise_state(seeds[inst], tcount[inst]);
+    NoisePair q{0.f, 0.f, false};
+    if (j < NP) q = noise_pair(s0, j);
+    int total;
+    const int rk = block_rank(q.ok, wsum, total);
+    const int* off = offsets + (size_t)inst * (nb + 1);
+    const long o = (long)off[blockIdx.x] + rk;
+    if (q.ok && o < P) {
+        const float f = (float)sqrt(-2. * (double)logf(q.r) / (double)q.r);
+        planes[(size_t)inst * P + o] = (R)(f * q.v1);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && off[nb] < P) atomicOr(err, 1);
+}
+
+static long noise_pairs(long P) {
+    // Acceptance probability pi/4; the margin is > 30 standard deviations.
+    return (long)(P / 0.785398163397448 + 40.0 * sqrt((double)P) + 4096.0);
+}
+
+size_t noise_scratch_bytes(long P, int B) {
+    const long NP = noise_pairs(P);
+    const long nb = (NP + PB - 1) / PB;
+    return sizeof(int) * (size_t)B * (2 * nb + 1) + 64;
+}
+
+hipError_t launch_noise(int prec, const DevBufs& b, const unsigned* seeds, void* planes, void* nz, void* scratch,
+                        int* err_flag, hipStream_t s) {
+    const long P = (long)b.L * b.M;
+    const long NP = noise_pairs(P);
+    const int nb = (int)((NP + PB - 1) / PB);
+    int* counts = (int*)scratch;
+    int* offs = counts + (size_t)b.B * nb;
+    hipLaunchKernelGGL(k_noise_count, dim3(nb, b.B), dim3(PB), 0, s, seeds, (const long long*)b.tcount, NP, nb, counts);
+    hipLaunchKernelGGL(k_scan, dim3(1, b.B), dim3(1024), 0, s, counts, nb, offs);
+    if (prec == PREC_FP64)
+        hipLaunchKernelGGL(k_noise_emit<double>, dim3(nb, b.B), dim3(PB), 0, s, seeds, (const long long*)b.tcount, NP,
+                           P, nb, offs, (double*)planes, err_flag);
+    else
+        hipLaunchKernelGGL(k_noise_emit<float>, dim3(nb, b.B), dim3(PB), 0, s, seeds, (const long long*)b.tcount, NP,
+                           P, nb, offs, (float*)planes, err_flag);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_row_r2c1(prec, b, planes, nz, s);
 }
 
 }  // namespace pfc

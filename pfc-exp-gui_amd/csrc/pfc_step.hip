@@ -86,13 +86,34 @@ __device__ __forceinline__ double energy_point(const PhysArgs& p, R phr, R cr, R
     return eid + emix + eex;
 }
 
-// Semi-implicit update of one mode (misfit.cpp:1002-1010, cvd.cpp:872-880).
+// Noise term of MAINEQUATIONC2_NOI for one mode (misfit.cpp:1002-1003):
+// dt*b0*noisf*sqrt(ks)*sqrt(|0.5 +- 0.5 sel|)*(0.5 - 0.5 tanh((sqrt(ks) - 2 pi)/0.1)),
+// sel = 1 on the four self-conjugate bins (l in {0, L/2}, m in {0, M/2}).
+__device__ __forceinline__ Cx<double> noise_term(const PhysArgs& p, float ks, bool sel, Cx<double> nz) {
+#pragma clang fp contract(off)
+    const float sks = sqrtf(ks);
+    const double damp = 0.5 - 0.5 * tanh(((double)sks - 2.0 * 3.1415926) / 0.1);
+    const double s = sel ? 1.0 : 0.0;
+    const double a = (double)p.db * nz.x * (double)sks * sqrt(fabs(0.5 + 0.5 * s)) * damp;
+    const double b = (double)p.db * nz.y * (double)sks * sqrt(fabs(0.5 - 0.5 * s)) * damp;
+    return Cx<double>{a, b};
+}
+
+__device__ __forceinline__ Cx<float> noise_term(const PhysArgs& p, float ks, bool sel, Cx<float> nz) {
+    const float sks = sqrtf(ks);
+    const float damp = 0.5f - 0.5f * tanhf((sks - 6.2831852f) * 10.0f);
+    const float ca = sel ? 1.0f : 0.70710678f, cb = sel ? 0.0f : 0.70710678f;
+    return Cx<float>{p.db * nz.x * sks * ca * damp, p.db * nz.y * sks * cb * damp};
+}
+
+// Semi-implicit update of one mode (misfit.cpp:1002-1010, cvd.cpp:872-880);
+// nt is the noise term (zero when b0 = 0).
 __device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<double, 2>& s,
-                                            const Vec<double, 2>& o) {
+                                            const Vec<double, 2>& o, Cx<double> nt) {
 #pragma clang fp contract(off)
     const float mk = p.mn_dt * ks;
-    double r1 = s.c[0].x - (double)mk * o.c[0].x;
-    double r2 = s.c[0].y - (double)mk * o.c[0].y;
+    double r1 = s.c[0].x - (double)mk * o.c[0].x + nt.x;
+    double r2 = s.c[0].y - (double)mk * o.c[0].y + nt.y;
     const float mkc = p.mc_dt * ks;
     const float den = (p.mc_dt_a * ks) * ks;
     double r3 = s.c[1].x - (double)mkc * o.c[1].x;
@@ -108,14 +129,14 @@ __device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<dou
 }
 
 __device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<float, 2>& s,
-                                            const Vec<float, 2>& o) {
+                                            const Vec<float, 2>& o, Cx<float> nt) {
     const float mk = p.mn_dt * ks;
     const float mkc = p.mc_dt * ks;
     const float den = (p.mc_dt_a * ks) * ks;
     const float d1 = 1.0f / (1.0f + mk);
     const float d2 = 1.0f / (1.0f + den);
-    s.c[0].x = (s.c[0].x - mk * o.c[0].x) * d1;
-    s.c[0].y = (s.c[0].y - mk * o.c[0].y) * d1;
+    s.c[0].x = (s.c[0].x - mk * o.c[0].x + nt.x) * d1;
+    s.c[0].y = (s.c[0].y - mk * o.c[0].y + nt.y) * d1;
     s.c[1].x = (s.c[1].x - mkc * o.c[1].x) * d2;
     s.c[1].y = (s.c[1].y - mkc * o.c[1].y) * d2;
 }
@@ -201,26 +222,26 @@ __device__ __forceinline__ void row_c2r(Vec<R, 2>* lds, int t, const Cx<R>* __re
 
 // r2c of w[q] (= x[2j] + i x[2j+1] per field, j = t + G*q); writes K = H+1
 // output elements to dst.  Leaves lds busy; ends without a trailing barrier.
-template <typename R, int H>
-__device__ __forceinline__ void row_r2c_store(Vec<R, 2>* lds, int t, const Vec<R, 2> (&w)[elems_for(H)],
+template <typename R, int H, int NC>
+__device__ __forceinline__ void row_r2c_store(Vec<R, NC>* lds, int t, const Vec<R, NC> (&w)[elems_for(H)],
                                               const Cx<R>* __restrict__ twH, const Cx<R>* __restrict__ twM,
-                                              Vec<R, 2>* __restrict__ dst) {
+                                              Vec<R, NC>* __restrict__ dst) {
     constexpr int E = elems_for(H), G = H / E;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < E; q++) lds[t + G * q] = w[q];
     __syncthreads();
-    lds_fft<R, 2, H, -1, 1>(lds, t, true, twH);
+    lds_fft<R, NC, H, -1, 1>(lds, t, true, twH);
     const R half = R(0.5);
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int k = t + G * q;
-        const Vec<R, 2> wk = lds[k], wc = lds[(H - k) % H];
+        const Vec<R, NC> wk = lds[k], wc = lds[(H - k) % H];
         const Cx<R> tw = twM[k];
-        Vec<R, 2> X;
-        Vec<R, 2> XH;
+        Vec<R, NC> X;
+        Vec<R, NC> XH;
 #pragma unroll
-        for (int f = 0; f < 2; f++) {
+        for (int f = 0; f < NC; f++) {
             const R ex = half * (wk.c[f].x + wc.c[f].x), ey = half * (wk.c[f].y - wc.c[f].y);
             const R dx = wk.c[f].x - wc.c[f].x, dy = wk.c[f].y + wc.c[f].y;
             const R ox = half * dy, oy = -half * dx;  // D / (2i)
@@ -284,7 +305,7 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
         w[q].c[0] = Cx<R>{inn[0], inn[1]};
         w[q].c[1] = Cx<R>{inc[0], inc[1]};
     }
-    if (flags & ROW_WRITE_IN) row_r2c_store<R, H>(lds, t, w, twH, twM, (Vec<R, 2>*)b.IN + rbase);
+    if (flags & ROW_WRITE_IN) row_r2c_store<R, H, 2>(lds, t, w, twH, twM, (Vec<R, 2>*)b.IN + rbase);
     if (want_e) {
         const double se = block_sum<C::NT>(esum, red);
         const double sd = block_sum<C::NT>(dsum, red);
@@ -315,7 +336,28 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_in(DevBufs b, const 
         w[q].c[0] = Cx<R>{inphi[pbase + 2 * j], inphi[pbase + 2 * j + 1]};
         w[q].c[1] = Cx<R>{incon[pbase + 2 * j], incon[pbase + 2 * j + 1]};
     }
-    row_r2c_store<R, H>(sm[rl], t, w, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM, (Vec<R, 2>*)b.S + rbase);
+    row_r2c_store<R, H, 2>(sm[rl], t, w, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM, (Vec<R, 2>*)b.S + rbase);
+}
+
+// One real plane per instance -> its row spectrum (the noise field).
+template <typename R, int H>
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_r2c1(DevBufs b, const R* __restrict__ in,
+                                                                 Vec<R, 1>* __restrict__ out) {
+    using C = RowCfg<R, H>;
+    constexpr int E = C::E, G = C::G, RPW = C::RPW, K = C::K;
+    constexpr int M = 2 * H;
+    __shared__ Vec<R, 1> sm[RPW][K];
+    const int tid = threadIdx.x, rl = tid / G, t = tid % G;
+    const int inst = blockIdx.y, row = blockIdx.x * RPW + rl;
+    const size_t rbase = ((size_t)inst * b.L + row) * K;
+    const size_t pbase = ((size_t)inst * b.L + row) * M;
+    Vec<R, 1> w[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int j = t + G * q;
+        w[q].c[0] = Cx<R>{in[pbase + 2 * j], in[pbase + 2 * j + 1]};
+    }
+    row_r2c_store<R, H, 1>(sm[rl], t, w, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM, out + rbase);
 }
 
 // S -> real planes (psi, con) including the means.
@@ -386,15 +428,39 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
         }
         __syncthreads();
         lds_fft<R, 2, LL, -1, T>(sm + col, t, true, twL);
+        Vec<R, 2> oh[E];
+#pragma unroll
+        for (int q = 0; q < E; q++) oh[q] = sm[(t + G * q) * T + col];
+        Cx<R> nz[E];
+#pragma unroll
+        for (int q = 0; q < E; q++) nz[q] = Cx<R>{R(0), R(0)};
+        if (b.NZ) {
+            // Noise spectrum along l (MAINEQUATIONC2_NOI's noisf).
+            const Vec<R, 1>* NZ = (const Vec<R, 1>*)b.NZ;
+            Vec<R, 1>* sm1 = (Vec<R, 1>*)sm;
+            __syncthreads();
+            for (int e = tid; e < LL * T; e += NT) {
+                const int i = e / T, c = e % T, kk = kc0 + c;
+                sm1[e] = (kk < K) ? NZ[ibase + (size_t)i * K + kk] : vzero<R, 1>();
+            }
+            __syncthreads();
+            lds_fft<R, 1, LL, -1, T>(sm1 + col, t, true, twL);
+#pragma unroll
+            for (int q = 0; q < E; q++) nz[q] = sm1[(t + G * q) * T + col].c[0];
+        }
         if (valid) {
             const float kmv = b.km[kc];
+            const bool mself = (kc == 0 || kc == b.M / 2);
 #pragma unroll
             for (int q = 0; q < E; q++) {
                 const int i = t + G * q;
-                const Vec<R, 2> oh = sm[i * T + col];
-                update_mode(p, ksq(b.kl[i], kmv), sh[q], oh);
+                const float ks = ksq(b.kl[i], kmv);
+                Cx<R> nt{R(0), R(0)};
+                if (b.NZ) nt = noise_term(p, ks, mself && (i == 0 || i == LL / 2), nz[q]);
+                update_mode(p, ks, sh[q], oh[q], nt);
             }
         }
+        if (b.tcount && blockIdx.x == 0 && tid == 0) b.tcount[inst] += 1;
     }
     if (kc == 0) {
 #pragma unroll
@@ -573,6 +639,24 @@ static hipError_t row_out_t(const DevBufs& b, void* outphi, void* outcon, hipStr
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t row_r2c1_t(const DevBufs& b, const void* in, void* out, hipStream_t s) {
+    const int nb = row_blocks_t<R>(b.M, b.L);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.M / 2) {
+#define R1(H) case H: hipLaunchKernelGGL((k_row_r2c1<R, H>), grid, dim3(RowCfg<R, H>::NT), 0, s, b, (const R*)in, (Vec<R, 1>*)out); break;
+        R1(16) R1(32) R1(64) R1(128) R1(192) R1(256) R1(384) R1(512)
+#undef R1
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_row_r2c1(int prec, const DevBufs& b, const void* in, void* out, hipStream_t s) {
+    return prec == PREC_FP64 ? row_r2c1_t<double>(b, in, out, s) : row_r2c1_t<float>(b, in, out, s);
 }
 
 hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s) {

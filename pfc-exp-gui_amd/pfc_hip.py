@@ -20,7 +20,7 @@ KID_ROW, KID_COL = 0, 1
 
 # Exported symbols (checked by the CPU test suite against include/pfc_hip.h).
 SYMBOLS = [
-    "pfc_params_default", "pfc_init_fields", "pfc_create", "pfc_destroy", "pfc_set_fields",
+    "pfc_params_default", "pfc_init_fields", "pfc_create", "pfc_destroy", "pfc_set_fields", "pfc_set_seed",
     "pfc_step", "pfc_sync", "pfc_get_fields", "pfc_get_fields_f32", "pfc_energy",
     "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic",
     "pfc_strain", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
@@ -65,6 +65,7 @@ def load():
         "pfc_destroy": [vp],
         "pfc_set_fields": [vp, ctypes.c_int, dp, dp],
         "pfc_step": [vp, ctypes.c_int],
+        "pfc_set_seed": [vp, ctypes.c_int, ctypes.c_uint],
         "pfc_sync": [vp],
         "pfc_get_fields": [vp, ctypes.c_int, dp, dp],
         "pfc_get_fields_f32": [vp, ctypes.c_int, fp, fp],
@@ -150,6 +151,9 @@ class Sim:
         con = np.ascontiguousarray(con, np.float64).ravel()
         assert phi.size == self.P and con.size == self.P
         _check(self.lib.pfc_set_fields(self.h, inst, _dp(phi), _dp(con)))
+
+    def set_seed(self, seed, inst=0):
+        _check(self.lib.pfc_set_seed(self.h, inst, seed))
 
     def step(self, n=1):
         _check(self.lib.pfc_step(self.h, n))

@@ -131,3 +131,23 @@ def test_batch_equals_single(hip):
         x = s1.get_fields()
         y = b.get_fields(i)
         assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-7), (1, 5e-4)])
+def test_misfit_noise_term(hip, oracle, prec, tol):
+    """--axx > 0: the Marsaglia noise stream (misfit.cpp:951-981) regenerated
+    on the GPU from the counter-based splitmix64 and added in k space."""
+    kw = dict(grainx=4, grainy=4, axx=0.001)
+    sim, osim = _pair(hip, oracle, 0, 64, 64, 0, prec, **kw)
+    sim.step(5)
+    sim.sync()
+    osim.run(5)
+    phi, con = sim.get_fields()
+    ophi, ocon = osim.fields()
+    err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
+    # the same run without noise differs by O(1e-2): the term is really applied
+    q, _ = _pair(hip, oracle, 0, 64, 64, 0, prec, grainx=4, grainy=4)
+    q.step(5)
+    quiet = np.abs(q.get_fields()[0] - ophi).max()
+    print(f"prec={prec} noise err={err:.3e} (no-noise distance {quiet:.3e})", flush=True)
+    assert err < tol and quiet > 100 * tol
