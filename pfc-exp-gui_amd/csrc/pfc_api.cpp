@@ -131,7 +131,7 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     c->m = m;
     c->B = batch;
     c->prec = prec;
-    c->L = p->L; c->M = p->M; c->K = p->M / 2 + 1;
+    c->L = p->L; c->M = p->M; c->K = p->M / 2;  // packed: X[M/2] rides in Im X[0]
     c->rsize = prec == PREC_FP64 ? 8 : 4;
     c->vsize = 4 * c->rsize;
     c->nrow = nrow; c->ncol = ncol;

@@ -1,17 +1,16 @@
-// pfc_fft.hpp - LDS-resident Stockham FFT building blocks for gfx950.
+// pfc_fft.hpp - register/LDS Stockham FFT building blocks for gfx950.
 //
 // Replaces the FFTW c2c plans of the reference (misfit.cpp:438-452,
 // cvd.cpp:334-348).  Every transform in the step is a batch of 1-D transforms
-// executed inside one workgroup: the data sits in LDS, each thread owns E
-// elements, and each radix pass reads its butterfly inputs from LDS, applies
-// twiddles, runs the radix-R butterfly in registers and writes the outputs back
-// in Stockham (auto-sorted) order.  Lengths are compile-time so every loop
-// unrolls and all per-thread data lives in VGPRs.
+// executed inside one workgroup: each thread owns E elements in VGPRs, each
+// radix pass runs the radix-R butterfly in registers, and consecutive passes
+// exchange data through LDS in Stockham (auto-sorted) order.  Lengths are
+// compile-time so every loop unrolls and all per-thread data lives in VGPRs.
 //
-// An element is a Vec: NC complex values that share one index (the two fields
-// psi/con, or the two convolutions a/b), so one pass of index arithmetic and
-// twiddle loads serves both fields, and global memory moves 16 B (fp32) or
-// 32 B (fp64) per element.
+// An element is a Vec: NC complex values that share one index (the fields
+// psi/con, the convolutions a/b, or all four), so one pass of index arithmetic
+// and twiddle loads serves every field, and global memory moves 16 B (fp32)
+// or 32 B (fp64) per pair of fields.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -152,30 +151,47 @@ constexpr bool plan_ok(int n, int E, int ns = 1) {
     return ns == n ? true : (pick_radix(n / ns, E) > 0 && plan_ok(n, E, ns * pick_radix(n / ns, E)));
 }
 
-// Elements per thread used for a transform of length n.
-constexpr int elems_for(int n) { return (n % 3 == 0) ? 12 : (n >= 8 ? 8 : n); }
+// Elements per thread for a transform of length n: `pow2` for powers of two
+// (radix-8/4/2 passes), 12 for 3*2^k (radix-4/3/2 passes).
+constexpr int elems_for(int n, int pow2) { return (n % 3 == 0) ? 12 : (n >= pow2 ? pow2 : n); }
 
-// One Stockham pass, in place in LDS (read all, barrier, write all, barrier).
-// `lds` points at element 0 of this thread's transform; consecutive elements
-// are STRIDE Vec apart.  tw[t] = exp(-2 pi i t / NLEN).  Barriers are taken
-// unconditionally; `active` only guards the loads and stores, so threads that
-// own no element of this transform never diverge around a barrier.
-template <typename R, int NC, int NLEN, int E, int DIR, int STRIDE, int NS, int RAD>
-__device__ __forceinline__ void fft_pass(Vec<R, NC>* lds, int t, bool active, const Cx<R>* __restrict__ tw) {
-    constexpr int G = NLEN / E;
-    constexpr int NB = E / RAD;
-    constexpr int STEP = NLEN / RAD;
-    Vec<R, NC> x[E];
-    if (active) {
+// ---------------------------------------------------------------------------
+// Register-resident Stockham transform.
+//
+// Thread t of a transform of length N owns elements t + G*q (q < E, G = N/E)
+// in x[q] on entry and again on exit (natural order, unnormalised).  The first
+// radix pass takes its inputs straight from x and the last pass writes its
+// outputs straight back to x; only the passes in between exchange data
+// through LDS.  The LDS buffer of one transform holds padded(N) elements with
+// one pad element after every 8 (pidx), which makes the radix-8 Stockham
+// scatter (stride-8 writes) conflict-free.  Every thread of the workgroup
+// must call (the exchanges contain barriers); each exchange starts with a
+// barrier, so the caller needs none before the call, and the call ends
+// without LDS reads in flight only after the caller's next barrier.
+// ---------------------------------------------------------------------------
+constexpr int padded(int n) { return n + n / 8 + 1; }
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
+
+template <typename R, int NC, int N, int E, int DIR, int NS, bool FIRST>
+__device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], Vec<R, NC>* lds, int t,
+                                               const Cx<R>* __restrict__ tw) {
+    if constexpr (NS < N) {
+        constexpr int RAD = pick_radix(N / NS, E);
+        static_assert(RAD > 0, "unsupported FFT length / elements-per-thread");
+        constexpr int G = N / E, NB = E / RAD, STEP = N / RAD;
+        constexpr bool LAST = (NS * RAD == N);
+        Vec<R, NC> v[E];
+        if constexpr (FIRST) {
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
-            const int j = t + G * b;
+            for (int b = 0; b < NB; b++)
 #pragma unroll
-            for (int r = 0; r < RAD; r++) x[b * RAD + r] = lds[(j + r * STEP) * STRIDE];
+                for (int r = 0; r < RAD; r++) v[b * RAD + r] = x[b + r * NB];
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; b++)
+#pragma unroll
+                for (int r = 0; r < RAD; r++) v[b * RAD + r] = lds[pidx(t + G * b + r * STEP)];
         }
-    }
-    __syncthreads();
-    if (active) {
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             const int j = t + G * b;
@@ -183,40 +199,38 @@ __device__ __forceinline__ void fft_pass(Vec<R, NC>* lds, int t, bool active, co
             if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < RAD; r++) {
-                    Cx<R> w = tw[r * k * (NLEN / (NS * RAD))];
+                    Cx<R> w = tw[r * k * (N / (NS * RAD))];
                     if constexpr (DIR > 0) w.y = -w.y;
-                    x[b * RAD + r] = vmulw(x[b * RAD + r], w);
+                    v[b * RAD + r] = vmulw(v[b * RAD + r], w);
                 }
             }
-            bfly<RAD, DIR>(&x[b * RAD]);
-            const int idx = (j / NS) * NS * RAD + k;
+            bfly<RAD, DIR>(&v[b * RAD]);
+        }
+        if constexpr (LAST) {
 #pragma unroll
-            for (int r = 0; r < RAD; r++) lds[(idx + r * NS) * STRIDE] = x[b * RAD + r];
+            for (int b = 0; b < NB; b++)
+#pragma unroll
+                for (int r = 0; r < RAD; r++) x[b + r * NB] = v[b * RAD + r];
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int j = t + G * b;
+                const int idx = (j / NS) * NS * RAD + j % NS;
+#pragma unroll
+                for (int r = 0; r < RAD; r++) lds[pidx(idx + r * NS)] = v[b * RAD + r];
+            }
+            __syncthreads();
+            fft_reg_passes<R, NC, N, E, DIR, NS * RAD, false>(x, lds, t, tw);
         }
     }
-    __syncthreads();
 }
 
-template <typename R, int NC, int NLEN, int E, int DIR, int STRIDE, int NS>
-__device__ __forceinline__ void fft_passes(Vec<R, NC>* lds, int t, bool active, const Cx<R>* __restrict__ tw) {
-    if constexpr (NS < NLEN) {
-        constexpr int RAD = pick_radix(NLEN / NS, E);
-        static_assert(RAD > 0, "unsupported FFT length / elements-per-thread");
-        fft_pass<R, NC, NLEN, E, DIR, STRIDE, NS, RAD>(lds, t, active, tw);
-        fft_passes<R, NC, NLEN, E, DIR, STRIDE, NS * RAD>(lds, t, active, tw);
-    }
-}
-
-// Unnormalised length-NLEN transform of the data in LDS.  Every thread of the
-// workgroup must call it (it contains barriers).  The caller must barrier
-// before the call; the call ends with a barrier, so its results can be read
-// right away.
-template <typename R, int NC, int NLEN, int DIR, int STRIDE>
-__device__ __forceinline__ void lds_fft(Vec<R, NC>* lds, int t, bool active, const Cx<R>* __restrict__ tw) {
-    constexpr int E = elems_for(NLEN);
-    static_assert(NLEN % E == 0, "length must be a multiple of elements per thread");
-    static_assert(plan_ok(NLEN, E), "no radix plan");
-    fft_passes<R, NC, NLEN, E, DIR, STRIDE, 1>(lds, t, active, tw);
+template <typename R, int NC, int N, int E, int DIR>
+__device__ __forceinline__ void fft_reg(Vec<R, NC> (&x)[E], Vec<R, NC>* lds, int t, const Cx<R>* __restrict__ tw) {
+    static_assert(N % E == 0, "length must be a multiple of elements per thread");
+    static_assert(plan_ok(N, E), "no radix plan");
+    fft_reg_passes<R, NC, N, E, DIR, 1, true>(x, lds, t, tw);
 }
 
 }  // namespace pfc

@@ -1,7 +1,8 @@
 // pfc_kernels.h - host-side launchers for the spectral step kernels.
 //
 // Device data layout (one batch of B instances of one grid L x M, N = 1):
-//   S   [B][L][K]  row-spectral state: r2c along m of (dpsi, dcon), K = M/2+1,
+//   S   [B][L][K]  row-spectral state: r2c along m of (dpsi, dcon), K = M/2
+//                  (the real Nyquist value X[M/2] is stored as Im X[0]),
 //                  one element = {psi_hat_m, con_hat_m} (two complex values);
 //                  the k=0 mode of both fields is held apart, as FP64 means.
 //   AB  [B][L][K]  row-spectral C2A*psi_hat and C2B*psi_hat of the same state

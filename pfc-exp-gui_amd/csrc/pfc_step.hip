@@ -166,91 +166,120 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // ---------------------------------------------------------------------------
 // Launch shapes
 // ---------------------------------------------------------------------------
-template <typename R, int H>
-struct RowCfg {
-    static constexpr int E = elems_for(H);
-    static constexpr int G = H / E;
-    static constexpr int cap = sizeof(R) == 8 ? 8 : 16;
-    static constexpr int want = (sizeof(R) == 8 ? 128 : 256) / G;
-    static constexpr int RPW = want < 1 ? 1 : (want > cap ? cap : want);
-    static constexpr int NT = RPW * G;
-    static constexpr int K = H + 1;
-};
-
-template <typename R, int LL>
-struct ColCfg {
-    static constexpr int E = elems_for(LL);
-    static constexpr int G = LL / E;
-    static constexpr int T = sizeof(R) == 8 ? (LL <= 128 ? 8 : (LL <= 512 ? 4 : 2))
-                                            : (LL <= 128 ? 16 : (LL <= 512 ? 8 : 4));
-    static constexpr int NT = T * G;
-};
-
-// ---------------------------------------------------------------------------
-// Row-transform helpers (one row of length M = 2H held as K = H+1 elements)
-// ---------------------------------------------------------------------------
-
-// c2r: lds[0..H] holds X[k]; on return z[q] = M*(x[2j] + i x[2j+1]) per field,
-// j = t + G*q.  Uses z = IFFT_H(Z), Z[k] = (X[k] + X*[H-k]) + i (X[k] - X*[H-k]) e^{+2 pi i k/M}.
-template <typename R, int H>
-__device__ __forceinline__ void row_c2r(Vec<R, 2>* lds, int t, const Cx<R>* __restrict__ twH,
-                                        const Cx<R>* __restrict__ twM, Vec<R, 2> (&z)[elems_for(H)]) {
-    constexpr int E = elems_for(H), G = H / E;
-    Vec<R, 2> zz[E];
-#pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int k = t + G * q;
-        const Vec<R, 2> xk = lds[k], xc = lds[H - k];
-        const Cx<R> w = twM[k];
-#pragma unroll
-        for (int f = 0; f < 2; f++) {
-            const R ax = xk.c[f].x + xc.c[f].x, ay = xk.c[f].y - xc.c[f].y;
-            const R dx = xk.c[f].x - xc.c[f].x, dy = xk.c[f].y + xc.c[f].y;
-            // B = D * conj(w)
-            const R bx = dx * w.x + dy * w.y, by = dy * w.x - dx * w.y;
-            zz[q].c[f] = Cx<R>{ax - by, ay + bx};
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < E; q++) lds[t + G * q] = zz[q];
-    __syncthreads();
-    lds_fft<R, 2, H, +1, 1>(lds, t, true, twH);
-#pragma unroll
-    for (int q = 0; q < E; q++) z[q] = lds[t + G * q];
+// Launch shapes (elements per thread E, rows per workgroup RPW, columns per
+// workgroup T) per precision and length.  The values are the fastest of the
+// shapes timed on MI355X for the sweep grids (16 instances of 128^2..512^2,
+// FP32) and the single-instance FP64 path; the PFC_* macros override them
+// for timing sweeps.
+#ifndef PFC_ROW_THREADS
+#define PFC_ROW_THREADS 256
+#endif
+#ifndef PFC_COL_T_DIV
+#define PFC_COL_T_DIV 1
+#endif
+#ifndef PFC_ROW_E
+#define PFC_ROW_E 4
+#endif
+constexpr int col_e(bool f64, int LL) {
+    return LL % 3 == 0 ? 12 : (f64 ? (LL >= 1024 ? 8 : 4) : (LL == 512 ? 8 : 4));
+}
+constexpr int col_t(bool f64, int LL) {
+    return f64 ? (LL <= 256 ? 4 : 2) : (LL <= 128 ? 8 : (LL <= 768 ? 4 : 2));
 }
 
-// r2c of w[q] (= x[2j] + i x[2j+1] per field, j = t + G*q); writes K = H+1
-// output elements to dst.  Leaves lds busy; ends without a trailing barrier.
-template <typename R, int H, int NC>
-__device__ __forceinline__ void row_r2c_store(Vec<R, NC>* lds, int t, const Vec<R, NC> (&w)[elems_for(H)],
+// Row pass: RPW rows of length M = 2H per workgroup, G = H/E threads per row.
+// LDS: one padded exchange buffer of 4-field elements per row.
+template <typename R, int H>
+struct RowCfg {
+    static constexpr int E = elems_for(H, PFC_ROW_E);
+    static constexpr int G = H / E;
+    static constexpr int cap = sizeof(R) == 8 ? 4 : 8;
+    static constexpr int want = PFC_ROW_THREADS / G;
+    static constexpr int RPW = want < 1 ? 1 : (want > cap ? cap : want);
+    static constexpr int NT = RPW * G;
+    static constexpr int K = H;  // row spectrum with X[H] packed into Im X[0]
+    static constexpr int PB = padded(H);                       // elements per row buffer
+    static constexpr size_t LDS = (size_t)RPW * PB * sizeof(Vec<R, 4>);
+};
+
+// Column pass: T columns of length LL per workgroup, G = LL/E threads per
+// column; consecutive threads take consecutive columns (coalesced T*16 or
+// T*32 byte segments per row).
+template <typename R, int LL>
+struct ColCfg {
+    static constexpr bool F64 = sizeof(R) == 8;
+    static constexpr int E = col_e(F64, LL);
+    static constexpr int G = LL / E;
+    static constexpr int T0 = col_t(F64, LL);
+    static constexpr int T = T0 / PFC_COL_T_DIV > 0 ? T0 / PFC_COL_T_DIV : 1;
+    static constexpr int NT = T * G;
+    static constexpr int PB = padded(LL);
+    // T column buffers; at least two (the packed column's noise exchange uses the second).
+    static constexpr size_t LDS = (size_t)(T < 2 ? 2 : T) * PB * sizeof(Vec<R, 4>);
+};
+
+// ---------------------------------------------------------------------------
+// Real-to-complex helpers for one row of length M = 2H.
+// ---------------------------------------------------------------------------
+
+// c2r split: Z[k] = (X[k] + X*[H-k]) + i (X[k] - X*[H-k]) e^{+2 pi i k/M};
+// IFFT_H(Z) = M (x[2j] + i x[2j+1]).
+template <typename R>
+__device__ __forceinline__ Cx<R> c2r_split(Cx<R> xk, Cx<R> xc, Cx<R> w) {
+    const R ax = xk.x + xc.x, ay = xk.y - xc.y;
+    const R dx = xk.x - xc.x, dy = xk.y + xc.y;
+    const R bx = dx * w.x + dy * w.y, by = dy * w.x - dx * w.y;
+    return Cx<R>{ax - by, ay + bx};
+}
+
+// r2c merge: with W = FFT_H(x[2j] + i x[2j+1]),
+// X[k] = (W[k] + W*[H-k])/2 + e^{-2 pi i k/M} (W[k] - W*[H-k])/(2i).
+template <typename R>
+__device__ __forceinline__ Cx<R> r2c_merge(Cx<R> wk, Cx<R> wc, Cx<R> tw) {
+    const R half = R(0.5);
+    const R ex = half * (wk.x + wc.x), ey = half * (wk.y - wc.y);
+    const R dx = wk.x - wc.x, dy = wk.y + wc.y;
+    const R ox = half * dy, oy = -half * dx;
+    return Cx<R>{ex + (tw.x * ox - tw.y * oy), ey + (tw.x * oy + tw.y * ox)};
+}
+
+// Packed row spectra: a real row of length M = 2H has real X[0] and X[H];
+// they are stored as one element X[0] + i X[H], so a row holds H elements.
+// r2c of w[q] (= x[2j] + i x[2j+1] per field, j = t + G*q), stored packed
+// to dst[0..H-1].  `lds` is this row's padded buffer viewed as Vec<R, NC>.
+template <typename R, int H, int E, int NC>
+__device__ __forceinline__ void row_r2c_store(Vec<R, NC> (&w)[E], Vec<R, NC>* lds, int t,
                                               const Cx<R>* __restrict__ twH, const Cx<R>* __restrict__ twM,
                                               Vec<R, NC>* __restrict__ dst) {
-    constexpr int E = elems_for(H), G = H / E;
+    constexpr int G = H / E;
+    fft_reg<R, NC, H, E, -1>(w, lds, t, twH);
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < E; q++) lds[t + G * q] = w[q];
+    for (int q = 0; q < E; q++) lds[pidx(t + G * q)] = w[q];
     __syncthreads();
-    lds_fft<R, NC, H, -1, 1>(lds, t, true, twH);
-    const R half = R(0.5);
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int k = t + G * q;
-        const Vec<R, NC> wk = lds[k], wc = lds[(H - k) % H];
-        const Cx<R> tw = twM[k];
         Vec<R, NC> X;
-        Vec<R, NC> XH;
+        if (k == 0) {
+            // X[0] = Re W0 + Im W0, X[H] = Re W0 - Im W0, both real.
 #pragma unroll
-        for (int f = 0; f < NC; f++) {
-            const R ex = half * (wk.c[f].x + wc.c[f].x), ey = half * (wk.c[f].y - wc.c[f].y);
-            const R dx = wk.c[f].x - wc.c[f].x, dy = wk.c[f].y + wc.c[f].y;
-            const R ox = half * dy, oy = -half * dx;  // D / (2i)
-            X.c[f] = Cx<R>{ex + (tw.x * ox - tw.y * oy), ey + (tw.x * oy + tw.y * ox)};
-            XH.c[f] = Cx<R>{ex - ox, R(0)};
+            for (int f = 0; f < NC; f++) X.c[f] = Cx<R>{w[q].c[f].x + w[q].c[f].y, w[q].c[f].x - w[q].c[f].y};
+        } else {
+            const Vec<R, NC> wc = lds[pidx(H - k)];
+            const Cx<R> tw = twM[k];
+#pragma unroll
+            for (int f = 0; f < NC; f++) X.c[f] = r2c_merge(w[q].c[f], wc.c[f], tw);
         }
         dst[k] = X;
-        if (k == 0) dst[H] = XH;
     }
+}
+
+// c2r split input for packed index k: (X[k], X[H-k]) with X[0], X[H] unpacked.
+template <typename R>
+__device__ __forceinline__ Cx<R> c2r_split_packed(Cx<R> xk, Cx<R> xc, int k, Cx<R> w) {
+    if (k == 0) return c2r_split(Cx<R>{xk.x, R(0)}, Cx<R>{xk.y, R(0)}, w);
+    return c2r_split(xk, xc, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -259,27 +288,31 @@ __device__ __forceinline__ void row_r2c_store(Vec<R, NC>* lds, int t, const Vec<
 template <typename R, int H>
 __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, PhysArgs p, int flags) {
     using C = RowCfg<R, H>;
-    constexpr int E = C::E, G = C::G, RPW = C::RPW, K = C::K;
-    __shared__ Vec<R, 2> sm[RPW][K];
+    constexpr int E = C::E, G = C::G, K = C::K;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     __shared__ double red[(C::NT + 63) / 64 + 1];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
-    const int inst = blockIdx.y, row = blockIdx.x * RPW + rl;
+    const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
     const size_t rbase = ((size_t)inst * b.L + row) * K;
-    const Vec<R, 2>* S = (const Vec<R, 2>*)b.S;
-    const Vec<R, 2>* AB = (const Vec<R, 2>*)b.AB;
+    const Vec<R, 2>* S = (const Vec<R, 2>*)b.S + rbase;
+    const Vec<R, 2>* AB = (const Vec<R, 2>*)b.AB + rbase;
     const Cx<R>* twH = (const Cx<R>*)b.twH;
     const Cx<R>* twM = (const Cx<R>*)b.twM;
-    Vec<R, 2>* lds = sm[rl];
 
-    for (int k = t; k < K; k += G) lds[k] = S[rbase + k];
-    __syncthreads();
-    Vec<R, 2> zs[E];
-    row_c2r<R, H>(lds, t, twH, twM, zs);
-    __syncthreads();
-    for (int k = t; k < K; k += G) lds[k] = AB[rbase + k];
-    __syncthreads();
-    Vec<R, 2> zab[E];
-    row_c2r<R, H>(lds, t, twH, twM, zab);
+    // One 4-field c2r: psi, con (from S) and a, b (from AB).
+    Vec<R, 4> z[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int k = t + G * q;
+        const int kc = (H - k) % H;
+        const Vec<R, 2> s0 = S[k], s1 = S[kc], a0 = AB[k], a1 = AB[kc];
+        const Cx<R> w = twM[k];
+        z[q].c[0] = c2r_split_packed(s0.c[0], s1.c[0], k, w);
+        z[q].c[1] = c2r_split_packed(s0.c[1], s1.c[1], k, w);
+        z[q].c[2] = c2r_split_packed(a0.c[0], a1.c[0], k, w);
+        z[q].c[3] = c2r_split_packed(a0.c[1], a1.c[1], k, w);
+    }
+    fft_reg<R, 4, H, E, +1>(z, (Vec<R, 4>*)smem + rl * C::PB, t, twH);
 
     const double* mu = b.means + 4 * inst;
     const R mph = (R)mu[0], mco = (R)mu[1], ma = (R)mu[2], mb = (R)mu[3];
@@ -289,23 +322,25 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
     const bool want_e = (flags & ROW_ENERGY) != 0;
 #pragma unroll
     for (int q = 0; q < E; q++) {
-        R ph[2] = {mph + zs[q].c[0].x * invM, mph + zs[q].c[0].y * invM};
-        R co[2] = {mco + zs[q].c[1].x * invM, mco + zs[q].c[1].y * invM};
-        R aa[2] = {ma + zab[q].c[0].x * invM, ma + zab[q].c[0].y * invM};
-        R bb[2] = {mb + zab[q].c[1].x * invM, mb + zab[q].c[1].y * invM};
+        const R ph[2] = {mph + z[q].c[0].x * invM, mph + z[q].c[0].y * invM};
+        const R co[2] = {mco + z[q].c[1].x * invM, mco + z[q].c[1].y * invM};
+        const R aa[2] = {ma + z[q].c[2].x * invM, ma + z[q].c[2].y * invM};
+        const R bb[2] = {mb + z[q].c[3].x * invM, mb + z[q].c[3].y * invM};
         R inn[2], inc[2], in3[2];
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-            nonlinear(p, ph[s], co[s], aa[s], bb[s], inn[s], inc[s], in3[s]);
+        for (int s2 = 0; s2 < 2; s2++) {
+            nonlinear(p, ph[s2], co[s2], aa[s2], bb[s2], inn[s2], inc[s2], in3[s2]);
             if (want_e) {
-                esum += energy_point<R>(p, ph[s], co[s], in3[s]);
-                dsum += (double)ph[s];
+                esum += energy_point<R>(p, ph[s2], co[s2], in3[s2]);
+                // density: the FP64 mean is added exactly in k_energy_reduce
+                dsum += (double)((s2 == 0 ? z[q].c[0].x : z[q].c[0].y) * invM);
             }
         }
         w[q].c[0] = Cx<R>{inn[0], inn[1]};
         w[q].c[1] = Cx<R>{inc[0], inc[1]};
     }
-    if (flags & ROW_WRITE_IN) row_r2c_store<R, H, 2>(lds, t, w, twH, twM, (Vec<R, 2>*)b.IN + rbase);
+    if (flags & ROW_WRITE_IN)
+        row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)smem + rl * C::PB, t, twH, twM, (Vec<R, 2>*)b.IN + rbase);
     if (want_e) {
         const double se = block_sum<C::NT>(esum, red);
         const double sd = block_sum<C::NT>(dsum, red);
@@ -320,13 +355,13 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
 // Real planes (dpsi, dcon) -> S.
 template <typename R, int H>
 __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_in(DevBufs b, const R* __restrict__ inphi,
-                                                             const R* __restrict__ incon) {
+                                                               const R* __restrict__ incon) {
     using C = RowCfg<R, H>;
-    constexpr int E = C::E, G = C::G, RPW = C::RPW, K = C::K;
+    constexpr int E = C::E, G = C::G, K = C::K;
     constexpr int M = 2 * H;
-    __shared__ Vec<R, 2> sm[RPW][K];
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
-    const int inst = blockIdx.y, row = blockIdx.x * RPW + rl;
+    const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
     const size_t rbase = ((size_t)inst * b.L + row) * K;
     const size_t pbase = ((size_t)inst * b.L + row) * M;
     Vec<R, 2> w[E];
@@ -336,19 +371,20 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_in(DevBufs b, const 
         w[q].c[0] = Cx<R>{inphi[pbase + 2 * j], inphi[pbase + 2 * j + 1]};
         w[q].c[1] = Cx<R>{incon[pbase + 2 * j], incon[pbase + 2 * j + 1]};
     }
-    row_r2c_store<R, H, 2>(sm[rl], t, w, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM, (Vec<R, 2>*)b.S + rbase);
+    row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
+                           (Vec<R, 2>*)b.S + rbase);
 }
 
-// One real plane per instance -> its row spectrum (the noise field).
+// One real plane per instance -> its packed row spectrum (the noise field).
 template <typename R, int H>
 __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_r2c1(DevBufs b, const R* __restrict__ in,
                                                                  Vec<R, 1>* __restrict__ out) {
     using C = RowCfg<R, H>;
-    constexpr int E = C::E, G = C::G, RPW = C::RPW, K = C::K;
+    constexpr int E = C::E, G = C::G, K = C::K;
     constexpr int M = 2 * H;
-    __shared__ Vec<R, 1> sm[RPW][K];
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
-    const int inst = blockIdx.y, row = blockIdx.x * RPW + rl;
+    const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
     const size_t rbase = ((size_t)inst * b.L + row) * K;
     const size_t pbase = ((size_t)inst * b.L + row) * M;
     Vec<R, 1> w[E];
@@ -357,27 +393,35 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_r2c1(DevBufs b, cons
         const int j = t + G * q;
         w[q].c[0] = Cx<R>{in[pbase + 2 * j], in[pbase + 2 * j + 1]};
     }
-    row_r2c_store<R, H, 1>(sm[rl], t, w, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM, out + rbase);
+    row_r2c_store<R, H, E, 1>(w, (Vec<R, 1>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
+                           out + rbase);
 }
 
 // S -> real planes (psi, con) including the means.
 template <typename R, int H>
 __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __restrict__ outphi,
-                                                              R* __restrict__ outcon) {
+                                                                R* __restrict__ outcon) {
     using C = RowCfg<R, H>;
-    constexpr int E = C::E, G = C::G, RPW = C::RPW, K = C::K;
+    constexpr int E = C::E, G = C::G, K = C::K;
     constexpr int M = 2 * H;
-    __shared__ Vec<R, 2> sm[RPW][K];
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
-    const int inst = blockIdx.y, row = blockIdx.x * RPW + rl;
+    const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
     const size_t rbase = ((size_t)inst * b.L + row) * K;
     const size_t pbase = ((size_t)inst * b.L + row) * M;
-    const Vec<R, 2>* S = (const Vec<R, 2>*)b.S;
-    Vec<R, 2>* lds = sm[rl];
-    for (int k = t; k < K; k += G) lds[k] = S[rbase + k];
-    __syncthreads();
+    const Vec<R, 2>* S = (const Vec<R, 2>*)b.S + rbase;
+    const Cx<R>* twM = (const Cx<R>*)b.twM;
     Vec<R, 2> z[E];
-    row_c2r<R, H>(lds, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM, z);
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int k = t + G * q;
+        const int kc = (H - k) % H;
+        const Vec<R, 2> s0 = S[k], s1 = S[kc];
+        const Cx<R> w = twM[k];
+        z[q].c[0] = c2r_split_packed(s0.c[0], s1.c[0], k, w);
+        z[q].c[1] = c2r_split_packed(s0.c[1], s1.c[1], k, w);
+    }
+    fft_reg<R, 2, H, E, +1>(z, (Vec<R, 2>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH);
     const double* mu = b.means + 4 * inst;
     const R mph = (R)mu[0], mco = (R)mu[1];
     const R invM = R(1) / R(M);
@@ -394,139 +438,173 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __
 // ---------------------------------------------------------------------------
 // Column pass
 // ---------------------------------------------------------------------------
+
+// Split of a packed column value (k_m = 0 and k_m = M/2 carried as
+// Z = X0 + i XH, X0 and XH Hermitian along l): from Z(k) and Z(-k).
+template <typename R>
+__device__ __forceinline__ void hsplit(Cx<R> z, Cx<R> zm, Cx<R>& x0, Cx<R>& xh) {
+    const R h = R(0.5);
+    x0 = Cx<R>{h * (z.x + zm.x), h * (z.y - zm.y)};
+    xh = Cx<R>{h * (z.y + zm.y), h * (zm.x - z.x)};
+}
+template <typename R>
+__device__ __forceinline__ Cx<R> hjoin(Cx<R> x0, Cx<R> xh) {
+    return Cx<R>{x0.x - xh.y, x0.y + xh.x};
+}
+
+// Per-mode work of the column pass on one (k_l = i, k_m = kmi) mode:
+// update (mode STEP), DC removal, energy term, C2 products.
+template <typename R>
+__device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, int mode, bool want_e, int LL, int i,
+                                         int kmi, Cx<R>& ph, Cx<R>& co, Cx<R> on, Cx<R> oc, Cx<R> nz, Cx<R>& a,
+                                         Cx<R>& bb, double& g) {
+    const float kmv = b.km[kmi];
+    const float ks = ksq(b.kl[i], kmv);
+    if (mode == COL_STEP) {
+        Vec<R, 2> s, o;
+        s.c[0] = ph; s.c[1] = co;
+        o.c[0] = on; o.c[1] = oc;
+        Cx<R> nt{R(0), R(0)};
+        if (b.NZ) nt = noise_term(p, ks, (kmi == 0 || kmi == b.M / 2) && (i == 0 || i == LL / 2), nz);
+        update_mode(p, ks, s, o, nt);
+        ph = s.c[0]; co = s.c[1];
+    }
+    if (i == 0 && kmi == 0) {
+        ph = Cx<R>{R(0), R(0)};
+        co = Cx<R>{R(0), R(0)};
+    }
+    if (want_e) {
+        // (kl'^2 + 2 km'^2) |con_hat|^2: own-direction Nyquist removed (Re of
+        // the reference's i k con_hat transform); 0 < km < M/2 counted twice.
+        const int Mh = b.M / 2;
+        const double kmd = (kmi == Mh) ? 0.0 : (double)kmv;
+        const double mult = (kmi == 0 || kmi == Mh) ? 1.0 : 2.0;
+        const double kld = (i == LL / 2) ? 0.0 : (double)b.kl[i];
+        g += mult * (kld * kld + 2.0 * kmd * kmd) * ((double)co.x * co.x + (double)co.y * co.y);
+    }
+    const int il = i <= LL / 2 ? i : LL - i;
+    const size_t ci = 2 * ((size_t)il * (b.M / 2 + 1) + kmi);
+    const R ca = (R)b.c2[ci], cb = (R)b.c2[ci + 1];
+    a = Cx<R>{ph.x * ca, ph.y * ca};
+    bb = Cx<R>{ph.x * cb, ph.y * cb};
+}
+
 template <typename R, int LL>
 __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     using C = ColCfg<R, LL>;
     constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT;
-    __shared__ Vec<R, 2> sm[LL * T];
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     __shared__ double red[(NT + 63) / 64 + 1];
     const int tid = threadIdx.x, col = tid % T, t = tid / T;
     const int inst = blockIdx.y;
     const int K = b.K;
-    const int kc0 = blockIdx.x * T, kc = kc0 + col;
-    const bool valid = kc < K;
-    const size_t ibase = (size_t)inst * LL * K;
+    const int kc = blockIdx.x * T + col;  // packed column index (kc = 0 holds k_m = 0 and M/2)
+    const size_t cbase = (size_t)inst * LL * K + kc;  // element (i, kc) at cbase + i*K
     Vec<R, 2>* S = (Vec<R, 2>*)b.S;
     const Cx<R>* twL = (const Cx<R>*)b.twL;
+    Vec<R, 4>* lds = (Vec<R, 4>*)smem + col * C::PB;
+    const bool packed_tile = blockIdx.x == 0;   // uniform per workgroup
+    const bool packed = packed_tile && col == 0;
+    const bool step = mode == COL_STEP;
 
-    for (int e = tid; e < LL * T; e += NT) {
-        const int i = e / T, c = e % T, kk = kc0 + c;
-        sm[e] = (kk < K) ? S[ibase + (size_t)i * K + kk] : vzero<R, 2>();
+    // Forward transform of {psi, con, inn, inc} along l in one pass.
+    Vec<R, 4> x[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const size_t e = cbase + (size_t)(t + G * q) * K;
+        const Vec<R, 2> sv = S[e];
+        x[q].c[0] = sv.c[0];
+        x[q].c[1] = sv.c[1];
+        if (step) {
+            const Vec<R, 2> iv = ((const Vec<R, 2>*)b.IN)[e];
+            x[q].c[2] = iv.c[0];
+            x[q].c[3] = iv.c[1];
+        } else {
+            x[q].c[2] = x[q].c[3] = Cx<R>{R(0), R(0)};
+        }
     }
-    __syncthreads();
-    lds_fft<R, 2, LL, -1, T>(sm + col, t, true, twL);
-    Vec<R, 2> sh[E];
+    fft_reg<R, 4, LL, E, -1>(x, lds, t, twL);
+    Cx<R> nz[E];
 #pragma unroll
-    for (int q = 0; q < E; q++) sh[q] = sm[(t + G * q) * T + col];
-
-    if (mode == COL_STEP) {
-        const Vec<R, 2>* IN = (const Vec<R, 2>*)b.IN;
+    for (int q = 0; q < E; q++) nz[q] = Cx<R>{R(0), R(0)};
+    if (step && b.NZ) {
+        Vec<R, 1> n1[E];
+#pragma unroll
+        for (int q = 0; q < E; q++) n1[q] = ((const Vec<R, 1>*)b.NZ)[cbase + (size_t)(t + G * q) * K];
+        fft_reg<R, 1, LL, E, -1>(n1, (Vec<R, 1>*)smem + col * C::PB, t, twL);
+#pragma unroll
+        for (int q = 0; q < E; q++) nz[q] = n1[q].c[0];
+    }
+    // The packed column needs each mode's mirror -k: exchange it through LDS.
+    Cx<R>* nzx = (Cx<R>*)((Vec<R, 4>*)smem + C::PB);
+    if (packed_tile) {
         __syncthreads();
-        for (int e = tid; e < LL * T; e += NT) {
-            const int i = e / T, c = e % T, kk = kc0 + c;
-            sm[e] = (kk < K) ? IN[ibase + (size_t)i * K + kk] : vzero<R, 2>();
-        }
-        __syncthreads();
-        lds_fft<R, 2, LL, -1, T>(sm + col, t, true, twL);
-        Vec<R, 2> oh[E];
-#pragma unroll
-        for (int q = 0; q < E; q++) oh[q] = sm[(t + G * q) * T + col];
-        Cx<R> nz[E];
-#pragma unroll
-        for (int q = 0; q < E; q++) nz[q] = Cx<R>{R(0), R(0)};
-        if (b.NZ) {
-            // Noise spectrum along l (MAINEQUATIONC2_NOI's noisf).
-            const Vec<R, 1>* NZ = (const Vec<R, 1>*)b.NZ;
-            Vec<R, 1>* sm1 = (Vec<R, 1>*)sm;
-            __syncthreads();
-            for (int e = tid; e < LL * T; e += NT) {
-                const int i = e / T, c = e % T, kk = kc0 + c;
-                sm1[e] = (kk < K) ? NZ[ibase + (size_t)i * K + kk] : vzero<R, 1>();
-            }
-            __syncthreads();
-            lds_fft<R, 1, LL, -1, T>(sm1 + col, t, true, twL);
-#pragma unroll
-            for (int q = 0; q < E; q++) nz[q] = sm1[(t + G * q) * T + col].c[0];
-        }
-        if (valid) {
-            const float kmv = b.km[kc];
-            const bool mself = (kc == 0 || kc == b.M / 2);
+        if (packed) {
 #pragma unroll
             for (int q = 0; q < E; q++) {
-                const int i = t + G * q;
-                const float ks = ksq(b.kl[i], kmv);
-                Cx<R> nt{R(0), R(0)};
-                if (b.NZ) nt = noise_term(p, ks, mself && (i == 0 || i == LL / 2), nz[q]);
-                update_mode(p, ks, sh[q], oh[q], nt);
+                lds[pidx(t + G * q)] = x[q];
+                if (step && b.NZ) nzx[pidx(t + G * q)] = nz[q];
             }
         }
-        if (b.tcount && blockIdx.x == 0 && tid == 0) b.tcount[inst] += 1;
+        __syncthreads();
     }
-    if (kc == 0) {
+
+    const bool want_e = (flags & COL_ENERGY) != 0;
+    double g = 0.0;
+    Vec<R, 4> y[E];  // {psi', con', C2A psi', C2B psi'}
 #pragma unroll
-        for (int q = 0; q < E; q++)
-            if (t + G * q == 0) sh[q] = vzero<R, 2>();
-    }
-    if (flags & COL_ENERGY) {
-        // sum_k (kl'^2 + 2 km'^2) |con_hat|^2 over this tile, with the own-direction
-        // Nyquist removed (Re of the reference's i k con_hat transform) and the
-        // half-spectrum columns 0 < kc < M/2 counted twice.
-        double g = 0.0;
-        if (valid) {
-            const int Mh = b.M / 2;
-            const double kmv = (kc == Mh) ? 0.0 : (double)b.km[kc];
-            const double mult = (kc == 0 || kc == Mh) ? 1.0 : 2.0;
-#pragma unroll
-            for (int q = 0; q < E; q++) {
-                const int i = t + G * q;
-                const double klv = (i == LL / 2) ? 0.0 : (double)b.kl[i];
-                const double cx = (double)sh[q].c[1].x, cy = (double)sh[q].c[1].y;
-                g += mult * (klv * klv + 2.0 * kmv * kmv) * (cx * cx + cy * cy);
-            }
+    for (int q = 0; q < E; q++) {
+        const int i = t + G * q;
+        Cx<R> ph = x[q].c[0], co = x[q].c[1], a, bb;
+        if (!packed) {
+            col_mode(b, p, mode, want_e, LL, i, kc, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
+        } else {
+            const int im = (LL - i) % LL;
+            const Vec<R, 4> xm = lds[pidx(im)];
+            const Cx<R> nzm = (step && b.NZ) ? nzx[pidx(im)] : Cx<R>{R(0), R(0)};
+            Cx<R> ph0, phh, co0, coh, on0, onh, oc0, och, nz0, nzh, a0, ah, b0, bh;
+            hsplit(x[q].c[0], xm.c[0], ph0, phh);
+            hsplit(x[q].c[1], xm.c[1], co0, coh);
+            hsplit(x[q].c[2], xm.c[2], on0, onh);
+            hsplit(x[q].c[3], xm.c[3], oc0, och);
+            hsplit(nz[q], nzm, nz0, nzh);
+            col_mode(b, p, mode, want_e, LL, i, 0, ph0, co0, on0, oc0, nz0, a0, b0, g);
+            col_mode(b, p, mode, want_e, LL, i, b.M / 2, phh, coh, onh, och, nzh, ah, bh, g);
+            ph = hjoin(ph0, phh);
+            co = hjoin(co0, coh);
+            a = hjoin(a0, ah);
+            bb = hjoin(b0, bh);
         }
+        y[q].c[0] = ph;
+        y[q].c[1] = co;
+        y[q].c[2] = a;
+        y[q].c[3] = bb;
+    }
+    if (step && b.tcount && blockIdx.x == 0 && tid == 0) b.tcount[inst] += 1;
+    if (want_e) {
         const double gs = block_sum<NT>(g, red);
         if (tid == 0) b.ecol[(size_t)inst * gridDim.x + blockIdx.x] = gs;
     }
     if (mode == COL_ENERGY_ONLY) return;
 
+    // Inverse transform of the new state and of C2A/C2B times the new psi.
+    fft_reg<R, 4, LL, E, +1>(y, lds, t, twL);
     const R invL = R(1) / R(LL);
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < E; q++) sm[(t + G * q) * T + col] = sh[q];
-    __syncthreads();
-    lds_fft<R, 2, LL, +1, T>(sm + col, t, true, twL);
-    for (int e = tid; e < LL * T; e += NT) {
-        const int i = e / T, c = e % T, kk = kc0 + c;
-        if (kk < K) S[ibase + (size_t)i * K + kk] = vscale(sm[e], invL);
-    }
-    __syncthreads();
-    {
-        const float* c2 = b.c2;
-#pragma unroll
-        for (int q = 0; q < E; q++) {
-            const int i = t + G * q;
-            const int il = i <= LL / 2 ? i : LL - i;
-            Vec<R, 2> v;
-            if (valid) {
-                const float ca = c2[2 * ((size_t)il * K + kc)], cb = c2[2 * ((size_t)il * K + kc) + 1];
-                v.c[0] = Cx<R>{sh[q].c[0].x * (R)ca, sh[q].c[0].y * (R)ca};
-                v.c[1] = Cx<R>{sh[q].c[0].x * (R)cb, sh[q].c[0].y * (R)cb};
-            } else {
-                v = vzero<R, 2>();
-            }
-            sm[i * T + col] = v;
-        }
-    }
-    __syncthreads();
-    lds_fft<R, 2, LL, +1, T>(sm + col, t, true, twL);
     Vec<R, 2>* AB = (Vec<R, 2>*)b.AB;
-    for (int e = tid; e < LL * T; e += NT) {
-        const int i = e / T, c = e % T, kk = kc0 + c;
-        if (kk < K) AB[ibase + (size_t)i * K + kk] = vscale(sm[e], invL);
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const size_t e = cbase + (size_t)(t + G * q) * K;
+        Vec<R, 2> sv, av;
+        sv.c[0] = Cx<R>{y[q].c[0].x * invL, y[q].c[0].y * invL};
+        sv.c[1] = Cx<R>{y[q].c[1].x * invL, y[q].c[1].y * invL};
+        av.c[0] = Cx<R>{y[q].c[2].x * invL, y[q].c[2].y * invL};
+        av.c[1] = Cx<R>{y[q].c[3].x * invL, y[q].c[3].y * invL};
+        S[e] = sv;
+        AB[e] = av;
     }
 }
 
-// Per-instance energy: e_out = {(sum e + alpha/P sum_k w|c|^2)/L/M/N, sum psi/L/M/N}.
+// Per-instance energy: e_out = {(sum e + alpha/P sum_k w|c|^2)/L/M, mean psi + sum dpsi/L/M}.
 __global__ void k_energy_reduce(DevBufs b, int nrow, int ncol, double alpha, double* e_out) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= b.B) return;
@@ -538,8 +616,8 @@ __global__ void k_energy_reduce(DevBufs b, int nrow, int ncol, double alpha, dou
     for (int i = 0; i < ncol; i++) sg += b.ecol[(size_t)inst * ncol + i];
     const double P = (double)b.L * (double)b.M;
     const double tot = se + alpha * sg / P;
-    e_out[2 * inst] = tot / b.L / b.M / 1;
-    e_out[2 * inst + 1] = sd / b.L / b.M / 1;
+    e_out[2 * inst] = tot / b.L / b.M;
+    e_out[2 * inst + 1] = sd / b.L / b.M + b.means[4 * inst];
 }
 
 // ---------------------------------------------------------------------------
@@ -574,7 +652,7 @@ static int col_blocks_t(int L, int M) {
 #undef CB
         default: return -1;
     }
-    const int K = M / 2 + 1;
+    const int K = M / 2;
     return (K + T - 1) / T;
 }
 
