@@ -1,11 +1,20 @@
 """Benchmark of the PFC spectral stepper on one MI355X.
 
-Workload (the headline metric "Mpoint-steps/s batched FP32 vs plain FP64"):
+Workload (the headline metric "Mpoint-steps/s batched FP32 vs plain FP64
+@ stated tol; p50/p99 sweep-job latency"):
   * batched Misfit sweep: 64 instances = u0 {0.02,0.04,0.06,0.08} x con0
     {0.1,0.2,0.3,0.4} x 4 seeds, grids 16 each of 128^2, 256^2, 384^2, 512^2,
     FP32 (FP64 k=0 modes and reductions), hipGraph-captured steps; one
     "step" advances all 64 instances by one time step.  -> `value`
   * plain FP64 path: one Misfit 512^2 instance, eager launches.  -> fp64_plain
+  * accuracy: Misfit 512^2 run 10^4 steps in FP32 and in plain FP64 from the
+    same initial fields; field L-inf / L2, mean drift and Phimax differences
+    every 1000 steps.  -> accuracy
+  * request loop: a Poisson stream of mixed Misfit/CVD jobs (grids 128^2,
+    256^2, 512^2, 1024^2, 128x128x64; 500..20000 steps) through the
+    continuous-batching server in FP32, final files written per job; the first
+    jobs of the same stream also run one at a time on the plain FP64 path.
+    -> serving (p50/p99 job latency, aggregate Mpoint-steps/s)
 Initial fields are the reference initialisers (synthetic, seeded).
 
 The dominant kernel (column pass of the 512^2 class) is timed with HIP
@@ -14,13 +23,19 @@ launch over that duration give roofline.achieved.  The CPU baseline times the
 oracle (the FP64 CPU restatement, single thread) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --pmc-drive   (a few sweep steps and one 256 MiB copy
+                                      kernel, for hardware-counter runs)
 """
 import argparse
 import importlib.util
 import json
 import os
+import shutil
 import sys
+import tempfile
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "Mpoint-steps/s batched FP32 vs plain FP64 @ stated tol; p50/p99 sweep-job latency"
@@ -82,6 +97,199 @@ def cpu_baseline(budget_s=15.0):
             "sample": f"oracle Misfit 256x256, {steps} steps (faithful: 14 c2c FFTs/step), {el:.1f} s, 1 thread"}
 
 
+def accuracy_study(hip, steps, every, L=512):
+    """FP32 path against the plain FP64 path (the drop-in, itself pinned to the
+    oracle) on Misfit L^2 from the same initial fields."""
+    res = {"workload": f"Misfit {L}x{L}, FP32 vs plain FP64 (reference narrowing), same initial fields",
+           "steps": steps, "every": every, "trace": []}
+    sims = {}
+    for prec in (hip.FP64, hip.FP32):
+        p = hip.default_params(model=hip.MISFIT, L=L, M=L, precision=prec)
+        phi, con = hip.init_fields(p)
+        s = hip.Sim(p)
+        s.set_fields(phi, con)
+        s.set_graph(every // 10 if every % 10 == 0 else 0)
+        sims[prec] = s
+    m0 = {k: v.energy()[0][1] for k, v in sims.items()}
+    t = 0
+    while t < steps:
+        for s in sims.values():
+            s.step(every)
+        t += every
+        a = sims[hip.FP64].get_fields()
+        b = sims[hip.FP32].get_fields()
+        ea, eb = sims[hip.FP64].energy()[0], sims[hip.FP32].energy()[0]
+        row = {"step": t}
+        for name, x, y in (("phi", a[0], b[0]), ("con", a[1], b[1])):
+            d = y - x
+            row[f"linf_{name}"] = float(np.abs(d).max())
+            row[f"l2_{name}"] = float(np.sqrt(np.mean(d * d)))
+        row["mean_drift_fp32"] = float(eb[1] - m0[hip.FP32])
+        row["mean_drift_fp64"] = float(ea[1] - m0[hip.FP64])
+        row["energy_rel_diff"] = float(abs(eb[0] - ea[0]) / max(abs(ea[0]), 1e-30))
+        xa = sims[hip.FP64].phimax_misfit()
+        xb = sims[hip.FP32].phimax_misfit()
+        row.update(_atom_match(xa, xb, L * 0.125))
+        res["trace"].append(row)
+    last = res["trace"][-1]
+    res["final"] = {k: last[k] for k in ("linf_phi", "linf_con", "l2_phi", "l2_con")}
+    res["tolerance_linf"] = 1e-4
+    res["within_tolerance"] = bool(max(last["linf_phi"], last["linf_con"]) <= 1e-4)
+    for s in sims.values():
+        s.close()
+    return res
+
+
+def _atom_match(a, b, box):
+    """Phimax atoms matched by nearest neighbour (periodic box): count
+    difference, largest matched position and phi differences."""
+    xa, ya, pa = a[0], a[1], a[3]
+    xb, yb, pb = b[0], b[1], b[3]
+    out = {"atoms_fp64": int(len(xa)), "atoms_fp32": int(len(xb))}
+    if len(xa) == 0 or len(xb) == 0:
+        return out
+    best = np.full(len(xa), np.inf)
+    idx = np.zeros(len(xa), dtype=np.int64)
+    for i0 in range(0, len(xb), 2048):
+        dx = xa[:, None] - xb[None, i0:i0 + 2048]
+        dy = ya[:, None] - yb[None, i0:i0 + 2048]
+        dx -= box * np.round(dx / box)
+        dy -= box * np.round(dy / box)
+        d = dx * dx + dy * dy
+        j = d.argmin(axis=1)
+        dj = d[np.arange(len(xa)), j]
+        upd = dj < best
+        best[upd] = dj[upd]
+        idx[upd] = j[upd] + i0
+    matched = np.sqrt(best) < 0.5  # half a lattice spacing
+    out["atoms_matched"] = int(matched.sum())
+    out["max_pos_err"] = float(np.sqrt(best[matched]).max()) if matched.any() else None
+    out["max_phi_err"] = float(np.abs(pa[matched] - pb[idx[matched]]).max()) if matched.any() else None
+    return out
+
+
+SERVE_GRIDS = [(128, 128, 1), (256, 256, 1), (512, 512, 1), (1024, 1024, 1), (128, 128, 64)]
+
+
+def serve_jobs(hip, rng, n, prec, step_scale=1.0):
+    jobs = []
+    for _ in range(n):
+        model = hip.MISFIT if rng.random() < 0.5 else hip.CVD
+        L, M, N = SERVE_GRIDS[rng.integers(len(SERVE_GRIDS))]
+        steps = max(1, int(rng.integers(500, 20001) * step_scale))
+        kw = dict(u0=float(rng.uniform(0.01, 0.1)), con0=float(rng.uniform(0.1, 0.4)),
+                  seed=int(rng.integers(0, 2 ** 32)))
+        if model == hip.MISFIT:
+            kw.update(grainx=max(1, L // 16), grainy=max(1, M // 16))
+        p = hip.default_params(model=model, L=L, M=M, N=N, precision=prec, **kw)
+        jobs.append((p, steps))
+    return jobs
+
+
+def _pct(x, q):
+    return float(np.percentile(np.asarray(x), q)) if len(x) else None
+
+
+def serve_leg(hip, seconds, seed=20240611):
+    """Poisson arrivals through the continuous-batching server (FP32), then the
+    first jobs of the same stream one at a time on the plain FP64 path."""
+    tmp = tempfile.mkdtemp(prefix="pfc_serve_")
+    rng = np.random.default_rng(seed)
+    try:
+        # Capacity of the server on this mix: a closed batch of short jobs.
+        srv = hip.Server()
+        cal = serve_jobs(hip, np.random.default_rng(seed + 1), 20, hip.FP32, step_scale=0.1)
+        t0 = time.perf_counter()
+        for p, st in cal:
+            srv.submit(p, st)
+        srv.drain(300)
+        cap = sum(p.L * p.M * p.N * st for p, st in cal) / (time.perf_counter() - t0)
+        srv.close()
+        # Poisson stream at ~0.8 of that capacity.
+        mean_work = np.mean([L * M * N for L, M, N in SERVE_GRIDS]) * 10250.0
+        lam = 0.8 * cap / mean_work
+        arrivals = []
+        t = 0.0
+        while True:
+            t += rng.exponential(1.0 / lam)
+            if t > seconds:
+                break
+            arrivals.append(t)
+        jobs = serve_jobs(hip, rng, len(arrivals), hip.FP32)
+        srv = hip.Server()
+        ids = []
+        start = time.perf_counter()
+        for k, (ta, (p, st)) in enumerate(zip(arrivals, jobs)):
+            while time.perf_counter() - start < ta:
+                time.sleep(min(0.002, max(0.0, ta - (time.perf_counter() - start))))
+            ids.append(srv.submit(p, st, outdir=os.path.join(tmp, f"job{k}")))
+        srv.drain(600)
+        lat, work, t_end = [], 0.0, 0.0
+        failed = 0
+        for jid, (p, st) in zip(ids, jobs):
+            s = srv.poll(jid)
+            if s.state != hip.JOB_DONE:
+                failed += 1
+                continue
+            lat.append(s.t_end - s.t_submit)
+            work += p.L * p.M * p.N * st
+            t_end = max(t_end, s.t_end)
+        t_first = srv.poll(ids[0]).t_submit if ids else 0.0
+        info = srv.stats()
+        srv.close()
+        fp32 = {"jobs": len(ids), "failed": failed, "p50_s": _pct(lat, 50), "p99_s": _pct(lat, 99),
+                "mpoint_steps_per_s": work / max(t_end - t_first, 1e-9) / 1e6,
+                "contexts_at_end": info.contexts}
+        # Plain FP64 path: the first jobs of the stream, one at a time (GUI queue).
+        k64 = min(12, len(jobs))
+        lat64, work64 = [], 0.0
+        start = time.perf_counter()
+        busy_until = 0.0
+        for ta, (p, st) in zip(arrivals[:k64], jobs[:k64]):
+            q = hip.default_params(model=p.model, L=p.L, M=p.M, N=p.N, precision=hip.FP64, u0=p.u0, con0=p.con0,
+                                   seed=p.seed, grainx=p.grainx, grainy=p.grainy)
+            now = time.perf_counter() - start
+            if now < ta:
+                time.sleep(ta - now)
+            phi, con = hip.init_fields(q)
+            s = hip.Sim(q)
+            s.set_fields(phi, con)
+            s.step(st)
+            s.get_fields()
+            s.close()
+            done = time.perf_counter() - start
+            lat64.append(done - ta)
+            work64 += q.L * q.M * q.N * st
+            busy_until = done
+        fp64 = {"jobs": k64, "p50_s": _pct(lat64, 50), "p99_s": _pct(lat64, 99),
+                "mpoint_steps_per_s": work64 / max(busy_until - (arrivals[0] if arrivals else 0.0), 1e-9) / 1e6,
+                "note": "first jobs of the same stream, same arrival times, one job at a time, "
+                        "FP64 one-instance contexts without graphs"}
+        return {"workload": f"Poisson stream for {seconds:.0f} s at 0.8 x measured FP32 capacity "
+                            f"({cap / 1e6:.0f} Mpoint-steps/s); Misfit/CVD 1:1, grids 128^2/256^2/512^2/1024^2/"
+                            "128x128x64, steps U[500,20000], u0 U[0.01,0.1], con0 U[0.1,0.4]; latency = "
+                            "arrival to final files written", "arrival_rate_per_s": lam, "seed": seed,
+                "fp32_batched": fp32, "fp64_plain": fp64}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def pmc_drive(hip, steps):
+    """A few sweep steps and one copy kernel of known size (256 MiB read and
+    written) for hardware-counter runs: the copy calibrates the counters."""
+    import torch
+    sims = build_sweep(hip, 0)
+    for s in sims.values():
+        s.step(steps)
+        s.sync()
+    n = 64 * 1024 * 1024  # floats
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_drive": True, "steps": steps, "copy_bytes_each_way": 4 * n}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +298,9 @@ def main():
     ap.add_argument("--graph-chunk", type=int, default=25)
     ap.add_argument("--fp64-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=10000)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--pmc-drive", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -97,6 +308,9 @@ def main():
         sys.exit(2)
     hip = _load("pfc_hip", os.path.join(ROOT, "pfc-exp-gui_amd", "pfc_hip.py"))
     hip.load()
+    if args.pmc_drive:
+        pmc_drive(hip, 3)
+        return
 
     # ---- batched FP32 sweep ------------------------------------------------
     sims = build_sweep(hip, args.graph_chunk)
@@ -171,6 +385,20 @@ def main():
             "bytes_per_launch": col_bytes,
         },
     }
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_k_col_float_512_sweep16.json")
+    if os.path.exists(traffic_file):
+        tr = json.load(open(traffic_file))
+        out["roofline"]["traffic"] = tr.get("bytes_per_dispatch")
+        out["roofline"]["traffic_source"] = "profiles/traffic_k_col_float_512_sweep16.json (rocprofv3 --pmc)"
+    for s in sims.values():
+        s.close()
+    s64.close()
+    if args.accuracy_steps > 0:
+        out["accuracy"] = accuracy_study(hip, args.accuracy_steps, max(1, args.accuracy_steps // 10))
+        print(f"accuracy done: {out['accuracy']['final']}", file=sys.stderr, flush=True)
+    if args.serve_seconds > 0:
+        out["serving"] = serve_leg(hip, args.serve_seconds)
+        print(f"serving done: {out['serving']['fp32_batched']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -13,6 +13,7 @@
 #ifndef PFC_HIP_H
 #define PFC_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -38,7 +39,8 @@ extern "C" {
  * mobilities, lattice wave numbers) are the reference's file-static values. */
 typedef struct pfc_params {
     int model;      /* PFC_MODEL_MISFIT or PFC_MODEL_CVD */
-    int L, M, N;    /* grid; the reference fixes it at compile time (pfc_common.h:138-146) */
+    int L, M, N;    /* grid; the reference fixes it at compile time (pfc_common.h:138-146).
+                       L, M in {32,64,128,256,384,512,768,1024}; N in {1,16,32,64,128} */
     int precision;  /* PFC_PREC_FP64 or PFC_PREC_FP32 */
     int narrow;     /* 1: reference float narrowing (Rhat, inc, energy terms) */
     float u0, con0, sig, dt, dx, axx;
@@ -64,6 +66,13 @@ int pfc_destroy(pfc_ctx* ctx);
 
 /* Load real fields of instance `inst` (host doubles, L*M*N each). */
 int pfc_set_fields(pfc_ctx* ctx, int inst, const double* phi, const double* con);
+
+/* Step mask: an inactive instance keeps its state while pfc_step and
+ * pfc_step_energy advance the others (all instances start active).  Serves
+ * the continuous-batching request loop, where slots of one batch hold jobs
+ * admitted at different times; the reference runs one job per process
+ * (MainWindow.cpp queue, misfit.cpp:311-590), so there is no counterpart. */
+int pfc_set_active(pfc_ctx* ctx, int inst, int active);
 
 /* Seed of instance `inst` for the Misfit noise stream (Gauss,
  * misfit.cpp:951-956); defaults to the creation parameters' seed.  Loading
@@ -111,15 +120,84 @@ int pfc_strain(int L, int M, int N, float dx, float benchel, int n, const double
 int pfc_set_graph(pfc_ctx* ctx, int chunk);
 
 /* Kernel timing with HIP events on the context's stream: kernel 0 = row pass,
- * 1 = column pass.  While enabled, every launch of that kernel is bracketed by
- * an event pair; pfc_kernel_time reports the mean duration since enabling. */
+ * 1 = column pass, 2 = depth passes (N > 1).  While enabled, every launch of
+ * that kernel is bracketed by an event pair; pfc_kernel_time reports the mean
+ * duration since enabling. */
 int pfc_time_kernel(pfc_ctx* ctx, int kernel, int enable);
 int pfc_kernel_time(pfc_ctx* ctx, double* mean_ms, int* count);
 
 /* Algorithmic HBM bytes one launch of kernel `kernel` moves (all instances). */
 int pfc_kernel_bytes(pfc_ctx* ctx, int kernel, double* bytes);
 
+/* Free and total device memory of the current GPU (hipMemGetInfo). */
+int pfc_device_memory(size_t* free_bytes, size_t* total_bytes);
+
 const char* pfc_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Request loop with continuous batching.  The reference runs one job per
+ * pfc-exp-cli process, queued one at a time by the GUI (MainWindow.cpp:
+ * 2920-2960, misfit.cpp:311-590); these entry points have no counterpart
+ * there.  A job is one run (what one CLI invocation computes); jobs of one
+ * class (model, precision, grid, sig/dt/dx/axx, narrowing) share a batch
+ * context, are admitted into free slots as they arrive while the others keep
+ * stepping, and leave when done.  Results are bit-identical to running the
+ * job alone in a one-instance context of the same precision.
+ * ------------------------------------------------------------------------ */
+#define PFC_JOB_QUEUED 0
+#define PFC_JOB_RUNNING 1
+#define PFC_JOB_DONE 2
+#define PFC_JOB_CANCELLED 3
+#define PFC_JOB_FAILED 4
+
+typedef struct pfc_server pfc_server;
+
+typedef struct pfc_server_opts {
+    double hbm_budget; /* bytes of device memory for batch contexts; <= 0: 80% of free memory */
+    int max_slots;     /* instances per batch context (default 16) */
+    int graph_chunk;   /* steps per captured HIP graph (default 25; 0: eager launches) */
+    int quantum;       /* most steps between two admission rounds (default 500) */
+} pfc_server_opts;
+
+typedef struct pfc_job {
+    pfc_params p;       /* run parameters; p.seed seeds the initial fields and the noise */
+    int steps;          /* time steps (> 0) */
+    int mod;            /* checkpoint interval for outdir output (0: final state only) */
+    int keep_fields;    /* 1: keep the final fields for pfc_server_result */
+    const char* outdir; /* NULL: no files; else the CLI's output files are written there
+                           (run_config.txt, energy.txt, phi/con VTI + PVTI, Phimax_<kd>.txt,
+                           checkpoint_timestamps.txt) by a writer thread */
+} pfc_job;
+
+typedef struct pfc_job_status {
+    int state;              /* PFC_JOB_* */
+    int steps_done;
+    double t_submit, t_start, t_end; /* seconds on the server clock; t_end: files written */
+    double energy, density;          /* ENERGYC2 of the final state (DONE) */
+} pfc_job_status;
+
+typedef struct pfc_server_info {
+    double now;             /* server clock, seconds */
+    long long jobs_done;
+    int jobs_unfinished;    /* queued, running or awaiting file writes */
+    double point_steps;     /* L*M*N*steps summed over finished jobs */
+    int contexts;
+    double bytes_in_use, budget;
+} pfc_server_info;
+
+int pfc_server_create(const pfc_server_opts* opts, pfc_server** out); /* opts may be NULL */
+int pfc_server_submit(pfc_server* s, const pfc_job* job, long long* id);
+int pfc_server_poll(pfc_server* s, long long id, pfc_job_status* st);
+/* Final fields (L*M*N doubles each) of a DONE job submitted with keep_fields. */
+int pfc_server_result(pfc_server* s, long long id, double* phi, double* con);
+/* Failure text of a FAILED job. */
+int pfc_server_error(pfc_server* s, long long id, char* buf, int cap);
+int pfc_server_cancel(pfc_server* s, long long id);
+/* Wait until no job is queued, running or awaiting writes (timeout_s < 0: forever). */
+int pfc_server_drain(pfc_server* s, double timeout_s);
+int pfc_server_stats(pfc_server* s, pfc_server_info* st);
+/* Cancels what is unfinished, stops the threads, frees the contexts. */
+int pfc_server_destroy(pfc_server* s);
 
 #ifdef __cplusplus
 }

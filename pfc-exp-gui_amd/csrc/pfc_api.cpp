@@ -37,14 +37,15 @@ static int fail(int code, const char* fmt, ...) {
 struct pfc_ctx {
     pfc_params p;
     Model m;
-    int B = 1, prec = 0, L = 0, M = 0, K = 0;
+    int B = 1, prec = 0, L = 0, M = 0, N = 1, K = 0;
     size_t rsize = 8, vsize = 32;  // bytes per real / per Vec element
     hipStream_t stream = nullptr;
     void *S = nullptr, *AB = nullptr, *IN = nullptr;
     double* means_d = nullptr;
     std::vector<double> means_h;
-    void *twH = nullptr, *twM = nullptr, *twL = nullptr;
-    float *kl_d = nullptr, *km_d = nullptr, *c2_d = nullptr;
+    void *twH = nullptr, *twM = nullptr, *twL = nullptr, *twN = nullptr;
+    float *kl_d = nullptr, *km_d = nullptr, *kn_d = nullptr, *c2_d = nullptr;
+    int* active_d = nullptr;  // [B] step mask
     double *erow = nullptr, *ecol[2] = {nullptr, nullptr}, *eout = nullptr;
     int nrow = 0, ncol = 0;
     void *stage_a = nullptr, *stage_b = nullptr;
@@ -68,6 +69,29 @@ struct pfc_ctx {
 };
 
 extern "C" const char* pfc_last_error(void) { return g_err.c_str(); }
+
+namespace pfc {
+int api_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+
+// Device bytes pfc_create allocates for `batch` instances (tables aside).
+size_t ctx_bytes(const pfc_params& p, int batch) {
+    const size_t rs = p.precision == PFC_PREC_FP64 ? 8 : 4;
+    const size_t P = (size_t)p.L * p.M * p.N;
+    const size_t nvec = (size_t)batch * p.L * p.N * (p.M / 2), nreal = (size_t)batch * P;
+    size_t b = 3 * nvec * 4 * rs + 2 * nreal * rs;
+    if (p.model == PFC_MODEL_MISFIT && p.axx > 0.0f)
+        b += nreal * rs + nvec * 2 * rs + noise_scratch_bytes((long)P, batch);
+    const size_t tables = ((size_t)(p.L / 2 + 1) * (p.M / 2 + 1) * (p.N / 2 + 1) * 2 + p.L + p.M + p.N) * 4 +
+                          (size_t)(p.L + p.M + p.N + p.M / 2) * 2 * rs;
+    return b + tables + (size_t)batch * 256 + (1u << 20);
+}
+}  // namespace pfc
+
+extern "C" int pfc_device_memory(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(PFC_E_ARG, "bad arguments");
+    HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+    return PFC_OK;
+}
 
 extern "C" int pfc_params_default(pfc_params* p, int model) {
     if (!p || (model != PFC_MODEL_MISFIT && model != PFC_MODEL_CVD)) return fail(PFC_E_ARG, "bad model");
@@ -103,9 +127,9 @@ static void free_ctx(pfc_ctx* c) {
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
     for (auto e : c->ev) hipEventDestroy(e);
-    void* ptrs[] = {c->S, c->AB, c->IN, c->means_d, c->twH, c->twM, c->twL, c->kl_d, c->km_d, c->c2_d,
-                    c->erow, c->ecol[0], c->ecol[1], c->eout, c->stage_a, c->stage_b, c->tcount,
-                    c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr};
+    void* ptrs[] = {c->S, c->AB, c->IN, c->means_d, c->twH, c->twM, c->twL, c->twN, c->kl_d, c->km_d, c->kn_d,
+                    c->c2_d, c->active_d, c->erow, c->ecol[0], c->ecol[1], c->eout, c->stage_a, c->stage_b,
+                    c->tcount, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -117,39 +141,45 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     *out = nullptr;
     if (p->model != PFC_MODEL_MISFIT && p->model != PFC_MODEL_CVD) return fail(PFC_E_ARG, "bad model %d", p->model);
     if (p->precision != PFC_PREC_FP64 && p->precision != PFC_PREC_FP32) return fail(PFC_E_ARG, "bad precision");
-    if (p->N != 1) return fail(PFC_E_UNSUPPORTED, "only 2-D grids (N = 1) are supported, got N = %d", p->N);
     if (!size_supported(p->L) || !size_supported(p->M))
         return fail(PFC_E_UNSUPPORTED, "grid %dx%d not supported (sizes: 32,64,128,256,384,512,768,1024)", p->L, p->M);
+    if (p->N != 1 && !depth_supported(p->N))
+        return fail(PFC_E_UNSUPPORTED, "N = %d not supported (N: 1,16,32,64,128)", p->N);
     if (p->dt <= 0.0f || p->dx <= 0.0f) return fail(PFC_E_ARG, "dt and dx must be > 0");
     Model m = make_model(*p);
     const int prec = p->precision == PFC_PREC_FP64 ? PREC_FP64 : PREC_FP32;
-    const int nrow = row_blocks(prec, p->M, p->L), ncol = col_blocks(prec, p->L, p->M);
-    if (nrow <= 0 || ncol <= 0) return fail(PFC_E_UNSUPPORTED, "no launch shape for %dx%d", p->L, p->M);
+    const int nrow = row_blocks(prec, p->M, p->L * p->N), ncol = col_blocks(prec, p->L, p->M, p->N);
+    if (nrow <= 0 || ncol <= 0) return fail(PFC_E_UNSUPPORTED, "no launch shape for %dx%dx%d", p->L, p->M, p->N);
 
     pfc_ctx* c = new pfc_ctx();
     c->p = *p;
     c->m = m;
     c->B = batch;
     c->prec = prec;
-    c->L = p->L; c->M = p->M; c->K = p->M / 2;  // packed: X[M/2] rides in Im X[0]
+    c->L = p->L; c->M = p->M; c->N = p->N; c->K = p->M / 2;  // packed: X[M/2] rides in Im X[0]
     c->rsize = prec == PREC_FP64 ? 8 : 4;
     c->vsize = 4 * c->rsize;
     c->nrow = nrow; c->ncol = ncol;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    const size_t nvec = (size_t)batch * c->L * c->K;
-    const size_t nreal = (size_t)batch * c->L * c->M;
+    const size_t nvec = (size_t)batch * c->L * c->N * c->K;
+    const size_t nreal = (size_t)batch * c->L * c->M * c->N;
     if (e == hipSuccess) e = hipMalloc(&c->S, nvec * c->vsize);
     if (e == hipSuccess) e = hipMalloc(&c->AB, nvec * c->vsize);
     if (e == hipSuccess) e = hipMalloc(&c->IN, nvec * c->vsize);
     if (e == hipSuccess) e = hipMalloc(&c->stage_a, nreal * c->rsize);
     if (e == hipSuccess) e = hipMalloc(&c->stage_b, nreal * c->rsize);
     if (e == hipSuccess) e = hipMalloc(&c->means_d, sizeof(double) * 4 * batch);
-    if (e == hipSuccess) e = hipMalloc(&c->erow, sizeof(double) * 2 * nrow * batch);
+    if (e == hipSuccess) e = hipMalloc(&c->erow, sizeof(double) * nrow * batch);
     if (e == hipSuccess) e = hipMalloc(&c->ecol[0], sizeof(double) * ncol * batch);
     if (e == hipSuccess) e = hipMalloc(&c->ecol[1], sizeof(double) * ncol * batch);
     if (e == hipSuccess) e = hipMalloc(&c->eout, sizeof(double) * 2 * batch);
     if (e == hipSuccess) e = hipMalloc(&c->tcount, sizeof(long long) * batch);
     if (e == hipSuccess) e = hipMemsetAsync(c->tcount, 0, sizeof(long long) * batch, c->stream);
+    if (e == hipSuccess) e = hipMalloc(&c->active_d, sizeof(int) * batch);
+    if (e == hipSuccess) {
+        std::vector<int> on(batch, 1);
+        e = hipMemcpy(c->active_d, on.data(), sizeof(int) * batch, hipMemcpyHostToDevice);
+    }
     c->noise = (p->model == PFC_MODEL_MISFIT && m.b0 != 0.0f);
     if (c->noise) {
         std::vector<unsigned> sd(batch, p->seed);
@@ -157,7 +187,7 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
         if (e == hipSuccess) e = hipMemcpy(c->seeds_d, sd.data(), sizeof(unsigned) * batch, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc(&c->nplanes, nreal * c->rsize);
         if (e == hipSuccess) e = hipMalloc(&c->nz, nvec * c->vsize / 2);
-        if (e == hipSuccess) e = hipMalloc(&c->nscratch, noise_scratch_bytes((long)c->L * c->M, batch));
+        if (e == hipSuccess) e = hipMalloc(&c->nscratch, noise_scratch_bytes((long)c->L * c->M * c->N, batch));
         if (e == hipSuccess) e = hipMalloc(&c->nerr, sizeof(int));
         if (e == hipSuccess) e = hipMemsetAsync(c->nerr, 0, sizeof(int), c->stream);
     }
@@ -172,19 +202,23 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
         c->twH = upload_twiddles<double>(c->M / 2, &e);
         if (e == hipSuccess) c->twM = upload_twiddles<double>(c->M, &e);
         if (e == hipSuccess) c->twL = upload_twiddles<double>(c->L, &e);
+        if (e == hipSuccess && c->N > 1) c->twN = upload_twiddles<double>(c->N, &e);
     } else {
         c->twH = upload_twiddles<float>(c->M / 2, &e);
         if (e == hipSuccess) c->twM = upload_twiddles<float>(c->M, &e);
         if (e == hipSuccess) c->twL = upload_twiddles<float>(c->L, &e);
+        if (e == hipSuccess && c->N > 1) c->twN = upload_twiddles<float>(c->N, &e);
     }
     std::vector<float> kl, km, kn, c2;
     k_tables(m, kl, km, kn);
     c2_tables_folded(m, c2, &c->c2a0, &c->c2b0);
     if (e == hipSuccess) e = hipMalloc(&c->kl_d, sizeof(float) * kl.size());
     if (e == hipSuccess) e = hipMalloc(&c->km_d, sizeof(float) * km.size());
+    if (e == hipSuccess) e = hipMalloc(&c->kn_d, sizeof(float) * kn.size());
     if (e == hipSuccess) e = hipMalloc(&c->c2_d, sizeof(float) * c2.size());
     if (e == hipSuccess) e = hipMemcpy(c->kl_d, kl.data(), sizeof(float) * kl.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->km_d, km.data(), sizeof(float) * km.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->kn_d, kn.data(), sizeof(float) * kn.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->c2_d, c2.data(), sizeof(float) * c2.size(), hipMemcpyHostToDevice);
     c->means_h.assign(4 * (size_t)batch, 0.0);
     if (e == hipSuccess) e = hipMemcpy(c->means_d, c->means_h.data(), sizeof(double) * 4 * batch, hipMemcpyHostToDevice);
@@ -202,12 +236,13 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     DevBufs& b = c->bufs;
     b.S = c->S; b.AB = c->AB; b.IN = c->IN;
     b.means = c->means_d;
-    b.twH = c->twH; b.twM = c->twM; b.twL = c->twL;
-    b.kl = c->kl_d; b.km = c->km_d; b.c2 = c->c2_d;
+    b.twH = c->twH; b.twM = c->twM; b.twL = c->twL; b.twN = c->twN;
+    b.kl = c->kl_d; b.km = c->km_d; b.kn = c->kn_d; b.c2 = c->c2_d;
     b.erow = c->erow; b.ecol = c->ecol[0];
     b.NZ = c->noise ? c->nz : nullptr;
     b.tcount = c->tcount;
-    b.L = c->L; b.M = c->M; b.K = c->K; b.B = batch;
+    b.active = c->active_d;
+    b.L = c->L; b.M = c->M; b.N = c->N; b.K = c->K; b.B = batch;
     *out = c;
     return PFC_OK;
 }
@@ -222,22 +257,23 @@ extern "C" int pfc_destroy(pfc_ctx* ctx) {
 // Buffers of one instance only.
 static DevBufs inst_bufs(const pfc_ctx* c, int inst) {
     DevBufs b = c->bufs;
-    const size_t off = (size_t)inst * c->L * c->K * c->vsize;
+    const size_t off = (size_t)inst * c->L * c->N * c->K * c->vsize;
     b.S = (char*)c->S + off;
     b.AB = (char*)c->AB + off;
     b.IN = (char*)c->IN + off;
     b.means = c->means_d + 4 * inst;
-    b.erow = c->erow + 2 * (size_t)c->nrow * inst;
+    b.erow = c->erow + (size_t)c->nrow * inst;
     b.ecol = c->ecol[0] + (size_t)c->ncol * inst;
     b.NZ = c->noise ? (char*)c->nz + off / 2 : nullptr;
     b.tcount = c->tcount + inst;
+    b.active = nullptr;  // single-instance set-up and read-back run regardless of the step mask
     b.B = 1;
     return b;
 }
 
 extern "C" int pfc_set_fields(pfc_ctx* c, int inst, const double* phi, const double* con) {
     if (!c || !phi || !con || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
-    const size_t P = (size_t)c->L * c->M;
+    const size_t P = (size_t)c->L * c->M * c->N;
     double sp = 0.0, sc = 0.0;
     for (size_t i = 0; i < P; i++) { sp += phi[i]; sc += con[i]; }
     // The reference rounds every spectral coefficient, the k=0 (mass) mode
@@ -263,7 +299,9 @@ extern "C" int pfc_set_fields(pfc_ctx* c, int inst, const double* phi, const dou
     HIP_TRY(hipMemcpyAsync(c->tcount + inst, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
     DevBufs b = inst_bufs(c, inst);
     HIP_TRY(launch_row_in(c->prec, b, (char*)c->stage_a + poff, (char*)c->stage_b + poff, c->stream));
+    if (c->N > 1) HIP_TRY(launch_depth(c->prec, b, -1, b.S, b.S, nullptr, nullptr, false, c->stream));
     HIP_TRY(launch_col(c->prec, b, c->phys, COL_INIT, 0, c->stream));
+    if (c->N > 1) HIP_TRY(launch_depth(c->prec, b, +1, b.S, b.S, b.AB, b.AB, false, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return PFC_OK;
 }
@@ -277,24 +315,51 @@ static hipEvent_t next_event(pfc_ctx* c) {
     return c->ev[c->ev_used++];
 }
 
-// One step on all instances: row pass then column pass.
+// Event pair around one launch of the timed kernel class.
+struct TimedLaunch {
+    pfc_ctx* c;
+    hipEvent_t e1 = nullptr;
+    TimedLaunch(pfc_ctx* cc, int kid) : c(cc) {
+        if (c->time_kid != kid) return;
+        hipEvent_t e0 = next_event(c);
+        e1 = next_event(c);
+        if (e0) hipEventRecord(e0, c->stream);
+    }
+    ~TimedLaunch() {
+        if (e1) hipEventRecord(e1, c->stream);
+    }
+};
+
+// One step on all active instances: row pass, then column pass; on 3-D
+// grids the column pass sits between the two depth passes.
 static hipError_t enqueue_step(pfc_ctx* c, int row_flags, int col_flags, int ecol_slot) {
     DevBufs b = c->bufs;
     b.ecol = c->ecol[ecol_slot];
-    const bool t0 = c->time_kid == KID_ROW, t1 = c->time_kid == KID_COL;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipSuccess;
     if (c->noise) {
-        hipError_t en = launch_noise(c->prec, b, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr, c->stream);
-        if (en != hipSuccess) return en;
+        e = launch_noise(c->prec, b, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr, c->stream);
+        if (e != hipSuccess) return e;
     }
-    if (t0) { e0 = next_event(c); e1 = next_event(c); if (e0) hipEventRecord(e0, c->stream); }
-    hipError_t e = launch_row_step(c->prec, b, c->phys, row_flags, c->stream);
-    if (t0 && e1) hipEventRecord(e1, c->stream);
+    {
+        TimedLaunch tl(c, KID_ROW);
+        e = launch_row_step(c->prec, b, c->phys, row_flags, c->stream);
+    }
     if (e != hipSuccess) return e;
-    if (t1) { e0 = next_event(c); e1 = next_event(c); if (e0) hipEventRecord(e0, c->stream); }
-    e = launch_col(c->prec, b, c->phys, COL_STEP, col_flags, c->stream);
-    if (t1 && e1) hipEventRecord(e1, c->stream);
-    return e;
+    if (c->N > 1) {
+        {
+            TimedLaunch tl(c, KID_DEPTH);
+            e = launch_depth(c->prec, b, -1, b.S, b.S, b.IN, b.IN, false, c->stream);
+        }
+        if (e == hipSuccess && c->noise) e = launch_depth(c->prec, b, -1, b.NZ, (void*)b.NZ, nullptr, nullptr, true, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    {
+        TimedLaunch tl(c, KID_COL);
+        e = launch_col(c->prec, b, c->phys, COL_STEP, col_flags, c->stream);
+    }
+    if (e != hipSuccess || c->N == 1) return e;
+    TimedLaunch tl(c, KID_DEPTH);
+    return launch_depth(c->prec, b, +1, b.S, b.S, b.AB, b.AB, false, c->stream);
 }
 
 extern "C" int pfc_set_graph(pfc_ctx* c, int chunk) {
@@ -342,6 +407,14 @@ extern "C" int pfc_sync(pfc_ctx* c) {
     return PFC_OK;
 }
 
+extern "C" int pfc_set_active(pfc_ctx* c, int inst, int active) {
+    if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
+    const int v = active ? 1 : 0;
+    HIP_TRY(hipMemcpyAsync(c->active_d + inst, &v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PFC_OK;
+}
+
 extern "C" int pfc_set_seed(pfc_ctx* c, int inst, unsigned int seed) {
     if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
     if (!c->noise) return PFC_OK;
@@ -358,7 +431,7 @@ static void widen(const void* src, double* dst, size_t n) {
 
 // S of instance `inst` -> real planes in stage_a/stage_b (device).
 static int fields_to_stage(pfc_ctx* c, int inst) {
-    const size_t P = (size_t)c->L * c->M;
+    const size_t P = (size_t)c->L * c->M * c->N;
     const size_t poff = (size_t)inst * P * c->rsize;
     DevBufs b = inst_bufs(c, inst);
     HIP_TRY(launch_row_out(c->prec, b, (char*)c->stage_a + poff, (char*)c->stage_b + poff, c->stream));
@@ -369,7 +442,7 @@ extern "C" int pfc_get_fields(pfc_ctx* c, int inst, double* phi, double* con) {
     if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
     int rc = fields_to_stage(c, inst);
     if (rc) return rc;
-    const size_t P = (size_t)c->L * c->M;
+    const size_t P = (size_t)c->L * c->M * c->N;
     const size_t poff = (size_t)inst * P * c->rsize;
     std::vector<char> h(P * c->rsize);
     if (phi) {
@@ -387,7 +460,7 @@ extern "C" int pfc_get_fields(pfc_ctx* c, int inst, double* phi, double* con) {
 
 extern "C" int pfc_get_fields_f32(pfc_ctx* c, int inst, float* phi, float* con) {
     if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
-    const size_t P = (size_t)c->L * c->M;
+    const size_t P = (size_t)c->L * c->M * c->N;
     std::vector<double> a(phi ? P : 0), b(con ? P : 0);
     int rc = pfc_get_fields(c, inst, phi ? a.data() : nullptr, con ? b.data() : nullptr);
     if (rc) return rc;
@@ -400,7 +473,13 @@ extern "C" int pfc_energy(pfc_ctx* c, double* e_out) {
     if (!c || !e_out) return fail(PFC_E_ARG, "bad arguments");
     DevBufs b = c->bufs;
     b.ecol = c->ecol[0];
+    b.active = nullptr;  // every instance's state has an energy
     HIP_TRY(launch_row_step(c->prec, b, c->phys, ROW_ENERGY, c->stream));
+    if (c->N > 1) {
+        // (m, n)-spectral copy of S in IN (free between steps); S is left untouched.
+        HIP_TRY(launch_depth(c->prec, b, -1, b.S, b.IN, nullptr, nullptr, false, c->stream));
+        b.S = b.IN;
+    }
     HIP_TRY(launch_col(c->prec, b, c->phys, COL_ENERGY_ONLY, COL_ENERGY, c->stream));
     HIP_TRY(launch_energy_reduce(b, c->nrow, c->ncol, (double)c->m.alpha, c->eout, c->stream));
     HIP_TRY(hipMemcpyAsync(e_out, c->eout, sizeof(double) * 2 * c->B, hipMemcpyDeviceToHost, c->stream));
@@ -441,7 +520,7 @@ extern "C" int pfc_step_energy(pfc_ctx* c, int nsteps, double* trace) {
 }
 
 extern "C" int pfc_time_kernel(pfc_ctx* c, int kernel, int enable) {
-    if (!c || (kernel != KID_ROW && kernel != KID_COL)) return fail(PFC_E_ARG, "bad arguments");
+    if (!c || (kernel != KID_ROW && kernel != KID_COL && kernel != KID_DEPTH)) return fail(PFC_E_ARG, "bad arguments");
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->ev_used = 0;
     c->time_kid = enable ? kernel : -1;
@@ -466,9 +545,10 @@ extern "C" int pfc_kernel_time(pfc_ctx* c, double* mean_ms, int* count) {
 
 extern "C" int pfc_kernel_bytes(pfc_ctx* c, int kernel, double* bytes) {
     if (!c || !bytes) return fail(PFC_E_ARG, "bad arguments");
-    const double arr = (double)c->B * c->L * c->K * (double)c->vsize;
+    const double arr = (double)c->B * c->L * c->N * c->K * (double)c->vsize;
     if (kernel == KID_ROW) *bytes = 3.0 * arr;       // read S, AB; write IN
     else if (kernel == KID_COL) *bytes = 4.0 * arr;  // read S, IN; write S, AB
+    else if (kernel == KID_DEPTH && c->N > 1) *bytes = 4.0 * arr;  // two field pairs in and out
     else return fail(PFC_E_ARG, "bad kernel id");
     return PFC_OK;
 }
@@ -481,7 +561,7 @@ extern "C" int pfc_phimax_misfit(pfc_ctx* c, int inst, double* x, double* y, dou
     if (!c || inst < 0 || inst >= c->B || !n || cap < 0) return fail(PFC_E_ARG, "bad arguments");
     int rc = fields_to_stage(c, inst);
     if (rc) return rc;
-    const size_t P = (size_t)c->L * c->M;
+    const size_t P = (size_t)c->L * c->M * c->N;
     const size_t poff = (size_t)inst * P * c->rsize;
     PostResult r;
     hipError_t e = post_phimax_misfit(c->prec, c->L, c->M, c->m.dx, c->m.con0, (char*)c->stage_a + poff,
@@ -497,10 +577,10 @@ extern "C" int pfc_phimax_cvd(pfc_ctx* c, int inst, float* pc_x, float* pc_y, fl
     if (!c || inst < 0 || inst >= c->B || !n || cap < 0) return fail(PFC_E_ARG, "bad arguments");
     int rc = fields_to_stage(c, inst);
     if (rc) return rc;
-    const size_t P = (size_t)c->L * c->M;
+    const size_t P = (size_t)c->L * c->M * c->N;
     const size_t poff = (size_t)inst * P * c->rsize;
     PostResult r;
-    hipError_t e = post_phimax_cvd(c->prec, c->L, c->M, c->m.dx, c->m.u0, (char*)c->stage_a + poff, pc_x, pc_y, pc_z,
+    hipError_t e = post_phimax_cvd(c->prec, c->L, c->M, c->N, c->m.dx, c->m.u0, (char*)c->stage_a + poff, pc_x, pc_y, pc_z,
                                    pc_phi, cap, &r, c->stream);
     if (e != hipSuccess) return fail(PFC_E_HIP, "phimax_cvd: %s", hipGetErrorString(e));
     *n = r.n;

@@ -1,14 +1,20 @@
 // pfc_kernels.h - host-side launchers for the spectral step kernels.
 //
-// Device data layout (one batch of B instances of one grid L x M, N = 1):
-//   S   [B][L][K]  row-spectral state: r2c along m of (dpsi, dcon), K = M/2
-//                  (the real Nyquist value X[M/2] is stored as Im X[0]),
-//                  one element = {psi_hat_m, con_hat_m} (two complex values);
-//                  the k=0 mode of both fields is held apart, as FP64 means.
-//   AB  [B][L][K]  row-spectral C2A*psi_hat and C2B*psi_hat of the same state
-//                  (the reference's c2aden/c2bden, misfit.cpp:801-833).
-//   IN  [B][L][K]  row-spectral nonlinear terms {inn, inc} (misfit.cpp:835-874).
-// One time step = row pass (S, AB -> IN) + column pass (S, IN -> S, AB).
+// Device data layout (one batch of B instances of one grid L x M x N):
+//   S   [B][L][N][K]  row-spectral state: r2c along m of (dpsi, dcon), K = M/2
+//                     (the real Nyquist value X[M/2] is stored as Im X[0]),
+//                     one element = {psi_hat_m, con_hat_m} (two complex values);
+//                     the k=0 mode of both fields is held apart, as FP64 means.
+//   AB  [B][L][N][K]  row-spectral C2A*psi_hat and C2B*psi_hat of the same state
+//                     (the reference's c2aden/c2bden, misfit.cpp:801-833).
+//   IN  [B][L][N][K]  row-spectral nonlinear terms {inn, inc} (misfit.cpp:835-874).
+// Real-space planes (inputs, outputs, noise) use the reference order
+// n + N*(m + M*l).  A 2-D step (N = 1) is row pass (S, AB -> IN) + column
+// pass (S, IN -> S, AB).  A 3-D step adds a depth pass (FFT along n) on each
+// side of the column pass: depth+ (S, IN in place), column, depth- (S, AB in
+// place); between the depth passes the arrays hold (m, n)-spectral values
+// with the packed k_m = 0 / M/2 column split along n (slot j < N/2: k_m = 0,
+// k_n = j; slot N - j: k_m = M/2, k_n = j; slots 0 and N/2 stay packed).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
@@ -35,14 +41,17 @@ struct DevBufs {
     const void* twH;      // exp(-2 pi i t/H), H = M/2
     const void* twM;      // exp(-2 pi i t/M)
     const void* twL;      // exp(-2 pi i t/L)
+    const void* twN;      // exp(-2 pi i t/N) (N > 1)
     const float* kl;      // [L]  reference krefl
     const float* km;      // [M]  reference krefm
-    const float* c2;      // [(L/2+1)][K][2] folded {C2A, C2B}
-    double* erow;         // [B][row_blocks][2]  {sum e_point, sum psi}
+    const float* kn;      // [N]  reference krefn
+    const float* c2;      // [(L/2+1)][(M/2+1)][(N/2+1)][2] folded {C2A, C2B}
+    double* erow;         // [B][row_blocks]  sum of pointwise energy terms
     double* ecol;         // [B][col_blocks]     sum w(k)|con_hat|^2
     const void* NZ;       // [B][L][K] complex: row-spectral noise (nullptr: no noise)
     long long* tcount;    // [B] step counter per instance (advanced by the column pass)
-    int L, M, K, B;
+    const int* active;    // [B] 0 = instance idle (step kernels skip it); nullptr = all
+    int L, M, N, K, B;
 };
 
 enum { ROW_WRITE_IN = 1, ROW_ENERGY = 2 };
@@ -50,20 +59,27 @@ enum { COL_INIT = 0, COL_STEP = 1, COL_ENERGY_ONLY = 2 };
 enum { COL_ENERGY = 1 };
 
 bool size_supported(int n);
-int row_blocks(int prec, int M, int L);  // row-pass workgroups per instance
-int col_blocks(int prec, int L, int M);  // column-pass workgroups per instance
+bool depth_supported(int n);                    // N > 1 lengths of the depth pass
+int row_blocks(int prec, int M, int rows);      // row-pass workgroups per instance (rows = L*N)
+int col_blocks(int prec, int L, int M, int N);  // column-pass workgroups per instance
 
 hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s);
 hipError_t launch_col(int prec, const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s);
 hipError_t launch_row_in(int prec, const DevBufs& b, const void* inphi, const void* incon, hipStream_t s);
 hipError_t launch_row_out(int prec, const DevBufs& b, void* outphi, void* outcon, hipStream_t s);
-// One real plane per instance [B][L][M] -> row spectrum [B][L][K] (single field).
+// Depth pass (N > 1): FFT along n of one or two field pairs, dir -1 forward
+// (then the k_m = 0 / M/2 column is split along n), +1 inverse (joined first,
+// result scaled by 1/N).  in1/out1 may be null (one pair).  With one_field
+// set, in0/out0 hold single complex values (the noise spectrum) instead.
+hipError_t launch_depth(int prec, const DevBufs& b, int dir, const void* in0, void* out0, const void* in1,
+                        void* out1, bool one_field, hipStream_t s);
+// One real plane per instance [B][L][M][N] -> row spectrum [B][L][N][K] (single field).
 hipError_t launch_row_r2c1(int prec, const DevBufs& b, const void* in, void* out, hipStream_t s);
 // e_out[inst*2 + {0,1}] = {mean energy, mean density}; alpha_over_P = alpha/P.
 hipError_t launch_energy_reduce(const DevBufs& b, int nrow, int ncol, double alpha, double* e_out,
                                 hipStream_t s);
 
-// Misfit noise (misfit.cpp:951-981): real noise planes [B][L][M] for the
+// Misfit noise (misfit.cpp:951-981): real noise planes [B][L][M][N] for the
 // current step of each instance, then NZ = r2c along m of those planes.
 // seeds: [B] run seeds; scratch sized by noise_scratch_bytes(P, B).
 size_t noise_scratch_bytes(long P, int B);
@@ -71,6 +87,6 @@ hipError_t launch_noise(int prec, const DevBufs& b, const unsigned* seeds, void*
                         int* err_flag, hipStream_t s);
 
 // Named kernels (for timing the dominant one).
-enum { KID_ROW = 0, KID_COL = 1 };
+enum { KID_ROW = 0, KID_COL = 1, KID_DEPTH = 2 };
 
 }  // namespace pfc

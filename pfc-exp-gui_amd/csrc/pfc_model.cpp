@@ -115,16 +115,17 @@ void c2_tables_folded(const Model& m, std::vector<float>& out, float* c2a0, floa
     k_tables(m, kl, km, kn);
     const float kc1 = crossover(m.sig, m.sigMa1, m.sigMa2, m.gaa1, m.gaa2, m.ka1, m.ka2);
     const float kc2 = crossover(m.sig, m.sigMb1, m.sigMb2, m.gab1, m.gab2, m.kb1, m.kb2);
-    const int Lh = m.L / 2 + 1, K = m.M / 2 + 1;
-    out.assign(static_cast<size_t>(Lh) * K * 2, 0.0f);
+    const int Lh = m.L / 2 + 1, K = m.M / 2 + 1, Nh = m.N / 2 + 1;
+    out.assign(static_cast<size_t>(Lh) * K * Nh * 2, 0.0f);
     for (int il = 0; il < Lh; il++)
-        for (int kc = 0; kc < K; kc++) {
-            const float ks = kl[il] * kl[il] + km[kc] * km[kc] + kn[0] * kn[0];
-            const float sk = std::sqrt(ks);
-            const size_t i = (static_cast<size_t>(il) * K + kc) * 2;
-            out[i] = peak(sk, kc1, m.sig, m.sigMa1, m.sigMa2, m.gaa1, m.gaa2, m.ka1, m.ka2);
-            out[i + 1] = peak(sk, kc2, m.sig, m.sigMb1, m.sigMb2, m.gab1, m.gab2, m.kb1, m.kb2);
-        }
+        for (int kc = 0; kc < K; kc++)
+            for (int jn = 0; jn < Nh; jn++) {
+                const float ks = kl[il] * kl[il] + km[kc] * km[kc] + kn[jn] * kn[jn];
+                const float sk = std::sqrt(ks);
+                const size_t i = ((static_cast<size_t>(il) * K + kc) * Nh + jn) * 2;
+                out[i] = peak(sk, kc1, m.sig, m.sigMa1, m.sigMa2, m.gaa1, m.gaa2, m.ka1, m.ka2);
+                out[i + 1] = peak(sk, kc2, m.sig, m.sigMb1, m.sigMb2, m.gab1, m.gab2, m.kb1, m.kb2);
+            }
     if (c2a0) *c2a0 = out[0];
     if (c2b0) *c2b0 = out[1];
 }

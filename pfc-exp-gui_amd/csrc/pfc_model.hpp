@@ -30,8 +30,9 @@ void default_params(pfc_params& p, int model);
 // k tables krefl/krefm/krefn (misfit.cpp:455-462).
 void k_tables(const Model& m, std::vector<float>& kl, std::vector<float>& km, std::vector<float>& kn);
 
-// C2A/C2B on the folded 2-D half spectrum: out[(il*K + kc)*2 + {0,1}],
-// il in [0, L/2], kc in [0, M/2] (misfit.cpp:700-750).  Also C2A(0), C2B(0).
+// C2A/C2B on the folded half spectrum: out[((il*K + kc)*Nh + jn)*2 + {0,1}],
+// il in [0, L/2], kc in [0, M/2], jn in [0, N/2], K = M/2+1, Nh = N/2+1
+// (misfit.cpp:700-750).  Also C2A(0), C2B(0).
 void c2_tables_folded(const Model& m, std::vector<float>& out, float* c2a0, float* c2b0);
 
 // Reference initialisers; phi/con: L*M*N doubles.

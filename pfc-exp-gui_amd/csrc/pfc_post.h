@@ -15,7 +15,7 @@ hipError_t post_phimax_misfit(int prec, int L, int M, float dx, float con0, cons
                               double* x, double* y, double* z, double* ph, float* co, int cap, PostResult* r,
                               hipStream_t s);
 // CVD atompositionCFGC2 maxima on device real field phi.
-hipError_t post_phimax_cvd(int prec, int L, int M, float dx, float u0, const void* phi, float* px, float* py,
+hipError_t post_phimax_cvd(int prec, int L, int M, int N, float dx, float u0, const void* phi, float* px, float* py,
                            float* pz, float* pphi, int cap, PostResult* r, hipStream_t s);
 // Elastic phimax_atmposi on a host float field.
 hipError_t post_phimax_elastic(int L, int M, int N, float dx, const float* phi, double* x, double* y, double* z,

@@ -115,28 +115,38 @@ struct MisfitOp {
 template <typename R>
 struct CvdOp {
     const R* phi;
-    int L, M;
+    int L, M, N;
     float dx, phic, fx, fy, fz;
     float* px; float* py; float* pz; float* pp;
 
-    __device__ float b(int i, int m) const { return (float)phi[(size_t)i * M + m]; }
+    // Flat index F of the reference's halo buffer (rows 0..L+1 of N*M floats,
+    // rows 0 and L+1 replicate the edge rows); F - 1 at n = 0 and F - N at
+    // m = 0 step into the previous m or l, as the reference's index arithmetic.
+    __device__ float bufv(long F) const {
+        const long NM = (long)N * M;
+        const long r = F / NM, rem = F - r * NM;
+        const long fr = r == 0 ? 0 : (r == L + 1 ? L - 1 : r - 1);
+        return (float)phi[fr * NM + rem];
+    }
     __device__ bool test(long p) const {
-        const int i = (int)(p / M), m = (int)(p % M);
-        const float c = b(i, m);
+        const long NM = (long)N * M;
+        const long F = p + NM;  // field row l sits in halo row l + 1
+        const float c = bufv(F);
         if (!(c >= phic)) return false;
-        const int up = i > 0 ? i - 1 : 0, dn = i < L - 1 ? i + 1 : L - 1;
-        if (!(c >= b(up, m) && c >= b(dn, m))) return false;
-        const float left = m > 0 ? b(i, m - 1) : b(up, M - 1);
-        const float right = m < M - 1 ? b(i, m + 1) : b(dn, 0);
-        return c >= left && c >= right;
+        if (!(c >= bufv(F - NM) && c >= bufv(F + NM))) return false;
+        if (!(c >= bufv(F - N) && c >= bufv(F + N))) return false;
+        return c >= bufv(F - 1) && c >= bufv(F + 1);
     }
     __device__ void emit(long p, size_t o) const {
 #pragma clang fp contract(off)
-        const int i = (int)(p / M), m = (int)(p % M);
+        const long NM = (long)N * M;
+        const int i = (int)(p / NM);
+        const long rem = p - (long)i * NM;
+        const int m = (int)(rem / N), n = (int)(rem % N);
         px[o] = ((float)(m + 1) * dx) / fy;
-        py[o] = ((float)1 * dx) / fz;
+        py[o] = ((float)(n + 1) * dx) / fz;
         pz[o] = ((float)(i + 1) * dx) / fx;
-        pp[o] = b(i, m);
+        pp[o] = bufv(p + NM);
     }
 };
 
@@ -386,9 +396,9 @@ static hipError_t field_max(const void* phi, int prec, long P, double* out, hipS
 }
 
 template <typename R>
-static hipError_t cvd_t(int L, int M, float dx, float u0, const void* phi, float* px, float* py, float* pz,
+static hipError_t cvd_t(int L, int M, int N, float dx, float u0, const void* phi, float* px, float* py, float* pz,
                         float* pp, int cap, PostResult* r, hipStream_t s) {
-    const long P = (long)L * M;
+    const long P = (long)L * M * N;
     double gmax = 0.0;
     hipError_t e = field_max(phi, sizeof(R) == 8 ? PREC_FP64 : PREC_FP32, P, &gmax, s);
     if (e != hipSuccess) return e;
@@ -400,8 +410,8 @@ static hipError_t cvd_t(int L, int M, float dx, float u0, const void* phi, float
     r->phimax = pmax;
     Scratch sc;
     CvdOp<R> op;
-    op.phi = (const R*)phi; op.L = L; op.M = M; op.dx = dx; op.phic = phic;
-    op.fx = L * dx; op.fy = M * dx; op.fz = 1 * dx;
+    op.phi = (const R*)phi; op.L = L; op.M = M; op.N = N; op.dx = dx; op.phic = phic;
+    op.fx = L * dx; op.fy = M * dx; op.fz = N * dx;
     op.px = sc.get<float>(P, e); op.py = sc.get<float>(P, e); op.pz = sc.get<float>(P, e); op.pp = sc.get<float>(P, e);
     if (e != hipSuccess) return e;
     int n = 0;
@@ -415,10 +425,10 @@ static hipError_t cvd_t(int L, int M, float dx, float u0, const void* phi, float
     return hipStreamSynchronize(s);
 }
 
-hipError_t post_phimax_cvd(int prec, int L, int M, float dx, float u0, const void* phi, float* px, float* py,
+hipError_t post_phimax_cvd(int prec, int L, int M, int N, float dx, float u0, const void* phi, float* px, float* py,
                            float* pz, float* pphi, int cap, PostResult* r, hipStream_t s) {
-    return prec == PREC_FP64 ? cvd_t<double>(L, M, dx, u0, phi, px, py, pz, pphi, cap, r, s)
-                             : cvd_t<float>(L, M, dx, u0, phi, px, py, pz, pphi, cap, r, s);
+    return prec == PREC_FP64 ? cvd_t<double>(L, M, N, dx, u0, phi, px, py, pz, pphi, cap, r, s)
+                             : cvd_t<float>(L, M, N, dx, u0, phi, px, py, pz, pphi, cap, r, s);
 }
 
 hipError_t post_phimax_elastic(int L, int M, int N, float dx, const float* phi, double* x, double* y, double* z,
@@ -629,7 +639,7 @@ size_t noise_scratch_bytes(long P, int B) {
 
 hipError_t launch_noise(int prec, const DevBufs& b, const unsigned* seeds, void* planes, void* nz, void* scratch,
                         int* err_flag, hipStream_t s) {
-    const long P = (long)b.L * b.M;
+    const long P = (long)b.L * b.M * b.N;
     const long NP = noise_pairs(P);
     const int nb = (int)((NP + PB - 1) / PB);
     int* counts = (int*)scratch;

@@ -141,9 +141,10 @@ __device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<flo
     s.c[1].y = (s.c[1].y - mkc * o.c[1].y) * d2;
 }
 
-__device__ __forceinline__ float ksq(float a, float b) {
+// |k|^2 in float, evaluated as the reference's pfc_ks (pfc_common.h:237-239).
+__device__ __forceinline__ float ksq(float a, float b, float c) {
 #pragma clang fp contract(off)
-    return a * a + b * b + 0.0f * 0.0f;
+    return a * a + b * b + c * c;
 }
 
 // Deterministic workgroup sum (fixed shuffle tree, then waves in order).
@@ -293,7 +294,8 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
     __shared__ double red[(C::NT + 63) / 64 + 1];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
     const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
-    const size_t rbase = ((size_t)inst * b.L + row) * K;
+    if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
+    const size_t rbase = ((size_t)inst * b.L * b.N + row) * K;
     const Vec<R, 2>* S = (const Vec<R, 2>*)b.S + rbase;
     const Vec<R, 2>* AB = (const Vec<R, 2>*)b.AB + rbase;
     const Cx<R>* twH = (const Cx<R>*)b.twH;
@@ -318,7 +320,7 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
     const R mph = (R)mu[0], mco = (R)mu[1], ma = (R)mu[2], mb = (R)mu[3];
     const R invM = R(1) / R(2 * H);
     Vec<R, 2> w[E];
-    double esum = 0.0, dsum = 0.0;
+    double esum = 0.0;
     const bool want_e = (flags & ROW_ENERGY) != 0;
 #pragma unroll
     for (int q = 0; q < E; q++) {
@@ -330,11 +332,7 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
 #pragma unroll
         for (int s2 = 0; s2 < 2; s2++) {
             nonlinear(p, ph[s2], co[s2], aa[s2], bb[s2], inn[s2], inc[s2], in3[s2]);
-            if (want_e) {
-                esum += energy_point<R>(p, ph[s2], co[s2], in3[s2]);
-                // density: the FP64 mean is added exactly in k_energy_reduce
-                dsum += (double)((s2 == 0 ? z[q].c[0].x : z[q].c[0].y) * invM);
-            }
+            if (want_e) esum += energy_point<R>(p, ph[s2], co[s2], in3[s2]);
         }
         w[q].c[0] = Cx<R>{inn[0], inn[1]};
         w[q].c[1] = Cx<R>{inc[0], inc[1]};
@@ -343,11 +341,10 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
         row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)smem + rl * C::PB, t, twH, twM, (Vec<R, 2>*)b.IN + rbase);
     if (want_e) {
         const double se = block_sum<C::NT>(esum, red);
-        const double sd = block_sum<C::NT>(dsum, red);
+
         if (tid == 0) {
             const size_t slot = (size_t)inst * gridDim.x + blockIdx.x;
-            b.erow[2 * slot] = se;
-            b.erow[2 * slot + 1] = sd;
+            b.erow[slot] = se;
         }
     }
 }
@@ -362,14 +359,15 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_in(DevBufs b, const 
     __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
     const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
-    const size_t rbase = ((size_t)inst * b.L + row) * K;
-    const size_t pbase = ((size_t)inst * b.L + row) * M;
+    const size_t rbase = ((size_t)inst * b.L * b.N + row) * K;
+    const size_t ps = b.N;  // real point (l, m, n) of row l*N + n at pbase + m*N
+    const size_t pbase = ((size_t)inst * b.L + row / b.N) * M * ps + row % b.N;
     Vec<R, 2> w[E];
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int j = t + G * q;
-        w[q].c[0] = Cx<R>{inphi[pbase + 2 * j], inphi[pbase + 2 * j + 1]};
-        w[q].c[1] = Cx<R>{incon[pbase + 2 * j], incon[pbase + 2 * j + 1]};
+        w[q].c[0] = Cx<R>{inphi[pbase + 2 * j * ps], inphi[pbase + (2 * j + 1) * ps]};
+        w[q].c[1] = Cx<R>{incon[pbase + 2 * j * ps], incon[pbase + (2 * j + 1) * ps]};
     }
     row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
                            (Vec<R, 2>*)b.S + rbase);
@@ -385,13 +383,15 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_r2c1(DevBufs b, cons
     __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
     const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
-    const size_t rbase = ((size_t)inst * b.L + row) * K;
-    const size_t pbase = ((size_t)inst * b.L + row) * M;
+    if (b.active && !b.active[inst]) return;
+    const size_t rbase = ((size_t)inst * b.L * b.N + row) * K;
+    const size_t ps = b.N;
+    const size_t pbase = ((size_t)inst * b.L + row / b.N) * M * ps + row % b.N;
     Vec<R, 1> w[E];
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int j = t + G * q;
-        w[q].c[0] = Cx<R>{in[pbase + 2 * j], in[pbase + 2 * j + 1]};
+        w[q].c[0] = Cx<R>{in[pbase + 2 * j * ps], in[pbase + (2 * j + 1) * ps]};
     }
     row_r2c_store<R, H, E, 1>(w, (Vec<R, 1>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
                            out + rbase);
@@ -407,8 +407,9 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __
     __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
     const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
-    const size_t rbase = ((size_t)inst * b.L + row) * K;
-    const size_t pbase = ((size_t)inst * b.L + row) * M;
+    const size_t rbase = ((size_t)inst * b.L * b.N + row) * K;
+    const size_t ps = b.N;
+    const size_t pbase = ((size_t)inst * b.L + row / b.N) * M * ps + row % b.N;
     const Vec<R, 2>* S = (const Vec<R, 2>*)b.S + rbase;
     const Cx<R>* twM = (const Cx<R>*)b.twM;
     Vec<R, 2> z[E];
@@ -428,10 +429,10 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int j = t + G * q;
-        outphi[pbase + 2 * j] = mph + z[q].c[0].x * invM;
-        outphi[pbase + 2 * j + 1] = mph + z[q].c[0].y * invM;
-        outcon[pbase + 2 * j] = mco + z[q].c[1].x * invM;
-        outcon[pbase + 2 * j + 1] = mco + z[q].c[1].y * invM;
+        outphi[pbase + 2 * j * ps] = mph + z[q].c[0].x * invM;
+        outphi[pbase + (2 * j + 1) * ps] = mph + z[q].c[0].y * invM;
+        outcon[pbase + 2 * j * ps] = mco + z[q].c[1].x * invM;
+        outcon[pbase + (2 * j + 1) * ps] = mco + z[q].c[1].y * invM;
     }
 }
 
@@ -452,38 +453,43 @@ __device__ __forceinline__ Cx<R> hjoin(Cx<R> x0, Cx<R> xh) {
     return Cx<R>{x0.x - xh.y, x0.y + xh.x};
 }
 
-// Per-mode work of the column pass on one (k_l = i, k_m = kmi) mode:
-// update (mode STEP), DC removal, energy term, C2 products.
+// Per-mode work of the column pass on one (k_l = i, k_m = kmi, k_n = kni)
+// mode: update (mode STEP), DC removal, energy term, C2 products.
 template <typename R>
 __device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, int mode, bool want_e, int LL, int i,
-                                         int kmi, Cx<R>& ph, Cx<R>& co, Cx<R> on, Cx<R> oc, Cx<R> nz, Cx<R>& a,
-                                         Cx<R>& bb, double& g) {
+                                         int kmi, int kni, Cx<R>& ph, Cx<R>& co, Cx<R> on, Cx<R> oc, Cx<R> nz,
+                                         Cx<R>& a, Cx<R>& bb, double& g) {
+    const int Mh = b.M / 2, Nh = b.N / 2;
     const float kmv = b.km[kmi];
-    const float ks = ksq(b.kl[i], kmv);
+    const float ks = ksq(b.kl[i], kmv, b.kn[kni]);
+    const bool mself = kmi == 0 || kmi == Mh;
     if (mode == COL_STEP) {
         Vec<R, 2> s, o;
         s.c[0] = ph; s.c[1] = co;
         o.c[0] = on; o.c[1] = oc;
         Cx<R> nt{R(0), R(0)};
-        if (b.NZ) nt = noise_term(p, ks, (kmi == 0 || kmi == b.M / 2) && (i == 0 || i == LL / 2), nz);
+        // self-conjugate selector of the reference: l and m only (misfit.cpp:991-1000)
+        if (b.NZ) nt = noise_term(p, ks, mself && (i == 0 || i == LL / 2), nz);
         update_mode(p, ks, s, o, nt);
         ph = s.c[0]; co = s.c[1];
     }
-    if (i == 0 && kmi == 0) {
+    if (i == 0 && kmi == 0 && kni == 0) {
         ph = Cx<R>{R(0), R(0)};
         co = Cx<R>{R(0), R(0)};
     }
     if (want_e) {
         // (kl'^2 + 2 km'^2) |con_hat|^2: own-direction Nyquist removed (Re of
-        // the reference's i k con_hat transform); 0 < km < M/2 counted twice.
-        const int Mh = b.M / 2;
+        // the reference's i k con_hat transform).  A mode whose conjugate is
+        // not stored (0 < km < M/2, or km in {0, M/2} with 0 < kn < N/2)
+        // counts twice.
         const double kmd = (kmi == Mh) ? 0.0 : (double)kmv;
-        const double mult = (kmi == 0 || kmi == Mh) ? 1.0 : 2.0;
+        const double mult = (mself && (kni == 0 || kni == Nh)) ? 1.0 : 2.0;
         const double kld = (i == LL / 2) ? 0.0 : (double)b.kl[i];
         g += mult * (kld * kld + 2.0 * kmd * kmd) * ((double)co.x * co.x + (double)co.y * co.y);
     }
     const int il = i <= LL / 2 ? i : LL - i;
-    const size_t ci = 2 * ((size_t)il * (b.M / 2 + 1) + kmi);
+    const int jn = kni <= Nh ? kni : b.N - kni;
+    const size_t ci = 2 * (((size_t)il * (Mh + 1) + kmi) * (Nh + 1) + jn);
     const R ca = (R)b.c2[ci], cb = (R)b.c2[ci + 1];
     a = Cx<R>{ph.x * ca, ph.y * ca};
     bb = Cx<R>{ph.x * cb, ph.y * cb};
@@ -497,21 +503,35 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     __shared__ double red[(NT + 63) / 64 + 1];
     const int tid = threadIdx.x, col = tid % T, t = tid / T;
     const int inst = blockIdx.y;
+    if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
     const int K = b.K;
-    const int kc = blockIdx.x * T + col;  // packed column index (kc = 0 holds k_m = 0 and M/2)
-    const size_t cbase = (size_t)inst * LL * K + kc;  // element (i, kc) at cbase + i*K
+    const size_t CS = (size_t)b.N * K;       // stride between consecutive l
+    const int tile0 = blockIdx.x * T;        // first column of the tile; T divides K
+    const int j = tile0 / K;                 // n slot of the whole tile
+    const int kc = tile0 - j * K + col;      // packed m column (kc = 0 holds k_m = 0 and M/2)
+    const size_t cbase = (size_t)inst * LL * CS + (size_t)j * K + kc;  // element i at cbase + i*CS
     Vec<R, 2>* S = (Vec<R, 2>*)b.S;
     const Cx<R>* twL = (const Cx<R>*)b.twL;
     Vec<R, 4>* lds = (Vec<R, 4>*)smem + col * C::PB;
-    const bool packed_tile = blockIdx.x == 0;   // uniform per workgroup
+    const int Nh = b.N / 2;
+    // Columns kc = 0 of slots j = 0 and N/2 carry two Hermitian-in-l sequences
+    // (k_m = 0 and M/2); uniform per workgroup.
+    const bool packed_tile = tile0 - j * K == 0 && (j == 0 || j == Nh);
     const bool packed = packed_tile && col == 0;
+    // Mode of a plain column; the kc = 0 column of the other slots holds
+    // k_m = 0 (j < N/2) or k_m = M/2 with k_n = N - j (j > N/2).
+    int kmi = kc, kni = j;
+    if (kc == 0 && !packed_tile) {
+        kmi = j < Nh ? 0 : b.M / 2;
+        kni = j < Nh ? j : b.N - j;
+    }
     const bool step = mode == COL_STEP;
 
     // Forward transform of {psi, con, inn, inc} along l in one pass.
     Vec<R, 4> x[E];
 #pragma unroll
     for (int q = 0; q < E; q++) {
-        const size_t e = cbase + (size_t)(t + G * q) * K;
+        const size_t e = cbase + (size_t)(t + G * q) * CS;
         const Vec<R, 2> sv = S[e];
         x[q].c[0] = sv.c[0];
         x[q].c[1] = sv.c[1];
@@ -530,7 +550,7 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     if (step && b.NZ) {
         Vec<R, 1> n1[E];
 #pragma unroll
-        for (int q = 0; q < E; q++) n1[q] = ((const Vec<R, 1>*)b.NZ)[cbase + (size_t)(t + G * q) * K];
+        for (int q = 0; q < E; q++) n1[q] = ((const Vec<R, 1>*)b.NZ)[cbase + (size_t)(t + G * q) * CS];
         fft_reg<R, 1, LL, E, -1>(n1, (Vec<R, 1>*)smem + col * C::PB, t, twL);
 #pragma unroll
         for (int q = 0; q < E; q++) nz[q] = n1[q].c[0];
@@ -557,7 +577,7 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
         const int i = t + G * q;
         Cx<R> ph = x[q].c[0], co = x[q].c[1], a, bb;
         if (!packed) {
-            col_mode(b, p, mode, want_e, LL, i, kc, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
+            col_mode(b, p, mode, want_e, LL, i, kmi, kni, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
         } else {
             const int im = (LL - i) % LL;
             const Vec<R, 4> xm = lds[pidx(im)];
@@ -568,8 +588,8 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
             hsplit(x[q].c[2], xm.c[2], on0, onh);
             hsplit(x[q].c[3], xm.c[3], oc0, och);
             hsplit(nz[q], nzm, nz0, nzh);
-            col_mode(b, p, mode, want_e, LL, i, 0, ph0, co0, on0, oc0, nz0, a0, b0, g);
-            col_mode(b, p, mode, want_e, LL, i, b.M / 2, phh, coh, onh, och, nzh, ah, bh, g);
+            col_mode(b, p, mode, want_e, LL, i, 0, j, ph0, co0, on0, oc0, nz0, a0, b0, g);
+            col_mode(b, p, mode, want_e, LL, i, b.M / 2, j, phh, coh, onh, och, nzh, ah, bh, g);
             ph = hjoin(ph0, phh);
             co = hjoin(co0, coh);
             a = hjoin(a0, ah);
@@ -593,7 +613,7 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     Vec<R, 2>* AB = (Vec<R, 2>*)b.AB;
 #pragma unroll
     for (int q = 0; q < E; q++) {
-        const size_t e = cbase + (size_t)(t + G * q) * K;
+        const size_t e = cbase + (size_t)(t + G * q) * CS;
         Vec<R, 2> sv, av;
         sv.c[0] = Cx<R>{y[q].c[0].x * invL, y[q].c[0].y * invL};
         sv.c[1] = Cx<R>{y[q].c[1].x * invL, y[q].c[1].y * invL};
@@ -604,20 +624,163 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     }
 }
 
-// Per-instance energy: e_out = {(sum e + alpha/P sum_k w|c|^2)/L/M, mean psi + sum dpsi/L/M}.
+// ---------------------------------------------------------------------------
+// Depth pass (3-D grids): FFT along n of the row spectra
+// ---------------------------------------------------------------------------
+// T packed m columns of one l row per workgroup, G = N/E threads per column.
+template <typename R, int NN>
+struct DepCfg {
+    static constexpr int E = NN >= 64 ? 8 : 4;
+    static constexpr int G = NN / E;
+    static constexpr int T = 16;  // 16 consecutive m columns: 256 or 512 byte segments per n
+    static constexpr int NT = T * G;
+    static constexpr int PB = padded(NN);
+    static constexpr size_t LDS = (size_t)T * PB * sizeof(Vec<R, 4>);
+};
+
+// NC = 1: one complex field (in0/out0 are Cx<R>); NC = 2: one field pair
+// (Vec<R, 2>); NC = 4: two field pairs (in1 null reads zeros, out1 null skips).
+// Along n the kc = 0 column holds Z = X0 + i XH with X0, XH real; the forward
+// pass replaces slot j in (0, N/2) by X0_hat(j) and slot N - j by XH_hat(j)
+// (slots 0 and N/2 keep the packed pair), the inverse pass undoes it.
+template <typename R, int NN, int NC, int DIR>
+__global__ void __launch_bounds__((DepCfg<R, NN>::NT)) k_depth(DevBufs b, const void* in0, void* out0,
+                                                               const void* in1, void* out1) {
+    using C = DepCfg<R, NN>;
+    constexpr int E = C::E, G = C::G, T = C::T;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    const int tid = threadIdx.x, col = tid % T, t = tid / T;
+    const int inst = blockIdx.y;
+    if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
+    const int K = b.K;
+    const int tpr = K / T;  // tiles per l row; T divides K (checked at launch)
+    const int l = blockIdx.x / tpr;
+    const int kc0 = (blockIdx.x - l * tpr) * T;
+    const size_t base = ((size_t)inst * b.L + l) * NN * K + kc0 + col;  // slot n at base + n*K
+    Vec<R, NC>* lds = (Vec<R, NC>*)smem + col * C::PB;
+    const bool split_col = kc0 == 0 && col == 0;
+
+    Vec<R, NC> x[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const size_t e = base + (size_t)(t + G * q) * K;
+        if constexpr (NC == 1) {
+            x[q].c[0] = ((const Cx<R>*)in0)[e];
+        } else {
+            const Vec<R, 2> v = ((const Vec<R, 2>*)in0)[e];
+            x[q].c[0] = v.c[0];
+            x[q].c[1] = v.c[1];
+            if constexpr (NC == 4) {
+                Vec<R, 2> w;
+                if (in1) w = ((const Vec<R, 2>*)in1)[e];
+                else w.c[0] = w.c[1] = Cx<R>{R(0), R(0)};
+                x[q].c[2] = w.c[0];
+                x[q].c[3] = w.c[1];
+            }
+        }
+    }
+    if constexpr (DIR > 0) {
+        // Join: slot j < N/2 becomes X0_hat(j) + i XH_hat(j), slot N - j
+        // conj(X0_hat(j)) + i conj(XH_hat(j)); the mirror slot comes via LDS.
+        if (kc0 == 0) {  // uniform per workgroup
+            if (split_col) {
+#pragma unroll
+                for (int q = 0; q < E; q++) lds[pidx(t + G * q)] = x[q];
+            }
+            __syncthreads();
+            if (split_col) {
+#pragma unroll
+                for (int q = 0; q < E; q++) {
+                    const int jj = t + G * q;
+                    if (jj == 0 || jj == NN / 2) continue;
+                    const Vec<R, NC> xm = lds[pidx(NN - jj)];
+#pragma unroll
+                    for (int f = 0; f < NC; f++) {
+                        if (jj < NN / 2) {
+                            x[q].c[f] = hjoin(x[q].c[f], xm.c[f]);
+                        } else {
+                            const Cx<R> x0c{xm.c[f].x, -xm.c[f].y}, xhc{x[q].c[f].x, -x[q].c[f].y};
+                            x[q].c[f] = hjoin(x0c, xhc);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    fft_reg<R, NC, NN, E, DIR>(x, lds, t, (const Cx<R>*)b.twN);
+    if constexpr (DIR < 0) {
+        if (kc0 == 0) {  // uniform per workgroup
+            __syncthreads();
+            if (split_col) {
+#pragma unroll
+                for (int q = 0; q < E; q++) lds[pidx(t + G * q)] = x[q];
+            }
+            __syncthreads();
+            if (split_col) {
+#pragma unroll
+                for (int q = 0; q < E; q++) {
+                    const int jj = t + G * q;
+                    if (jj == 0 || jj == NN / 2) continue;
+                    const Vec<R, NC> xm = lds[pidx(NN - jj)];
+#pragma unroll
+                    for (int f = 0; f < NC; f++) {
+                        Cx<R> x0, xh;
+                        if (jj < NN / 2) {
+                            hsplit(x[q].c[f], xm.c[f], x0, xh);
+                            x[q].c[f] = x0;
+                        } else {
+                            hsplit(xm.c[f], x[q].c[f], x0, xh);
+                            x[q].c[f] = xh;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    const R sc = DIR > 0 ? R(1) / R(NN) : R(1);
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const size_t e = base + (size_t)(t + G * q) * K;
+        if constexpr (DIR > 0) {
+#pragma unroll
+            for (int f = 0; f < NC; f++) x[q].c[f] = Cx<R>{x[q].c[f].x * sc, x[q].c[f].y * sc};
+        }
+        if constexpr (NC == 1) {
+            ((Cx<R>*)out0)[e] = x[q].c[0];
+        } else {
+            Vec<R, 2> v;
+            v.c[0] = x[q].c[0];
+            v.c[1] = x[q].c[1];
+            ((Vec<R, 2>*)out0)[e] = v;
+            if constexpr (NC == 4) {
+                if (out1) {
+                    Vec<R, 2> w;
+                    w.c[0] = x[q].c[2];
+                    w.c[1] = x[q].c[3];
+                    ((Vec<R, 2>*)out1)[e] = w;
+                }
+            }
+        }
+    }
+}
+
+// Per-instance energy: e_out = {(sum e + alpha/P sum_k w|c|^2)/P, mean psi}, P = L*M*N.
 __global__ void k_energy_reduce(DevBufs b, int nrow, int ncol, double alpha, double* e_out) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= b.B) return;
-    double se = 0.0, sd = 0.0, sg = 0.0;
+    double se = 0.0, sg = 0.0;
     for (int i = 0; i < nrow; i++) {
-        se += b.erow[2 * ((size_t)inst * nrow + i)];
-        sd += b.erow[2 * ((size_t)inst * nrow + i) + 1];
+        se += b.erow[(size_t)inst * nrow + i];
     }
     for (int i = 0; i < ncol; i++) sg += b.ecol[(size_t)inst * ncol + i];
-    const double P = (double)b.L * (double)b.M;
+    const double P = (double)b.L * (double)b.M * (double)b.N;
     const double tot = se + alpha * sg / P;
-    e_out[2 * inst] = tot / b.L / b.M;
-    e_out[2 * inst + 1] = sd / b.L / b.M + b.means[4 * inst];
+    e_out[2 * inst] = tot / b.L / b.M / b.N;
+    // Density: the k=0 mode, held in FP64 and left unchanged by every step
+    // (k = 0 makes the update an identity, misfit.cpp:1002-1010); the sum of
+    // the reconstructed deviations is zero up to transform round-off.
+    e_out[2 * inst + 1] = b.means[4 * inst];
 }
 
 // ---------------------------------------------------------------------------
@@ -626,6 +789,13 @@ __global__ void k_energy_reduce(DevBufs b, int nrow, int ncol, double alpha, dou
 bool size_supported(int n) {
     switch (n) {
         case 32: case 64: case 128: case 256: case 384: case 512: case 768: case 1024: return true;
+        default: return false;
+    }
+}
+
+bool depth_supported(int n) {
+    switch (n) {
+        case 16: case 32: case 64: case 128: return true;
         default: return false;
     }
 }
@@ -644,7 +814,7 @@ static int row_blocks_t(int M, int L) {
 }
 
 template <typename R>
-static int col_blocks_t(int L, int M) {
+static int col_blocks_t(int L, int M, int N) {
     int T = 1;
     switch (L) {
 #define CB(LL) case LL: T = ColCfg<R, LL>::T; break;
@@ -653,19 +823,20 @@ static int col_blocks_t(int L, int M) {
         default: return -1;
     }
     const int K = M / 2;
-    return (K + T - 1) / T;
+    if (K % T) return -1;  // k_col handles whole tiles of T columns only
+    return N * (K / T);
 }
 
-int row_blocks(int prec, int M, int L) {
-    return prec == PREC_FP64 ? row_blocks_t<double>(M, L) : row_blocks_t<float>(M, L);
+int row_blocks(int prec, int M, int rows) {
+    return prec == PREC_FP64 ? row_blocks_t<double>(M, rows) : row_blocks_t<float>(M, rows);
 }
-int col_blocks(int prec, int L, int M) {
-    return prec == PREC_FP64 ? col_blocks_t<double>(L, M) : col_blocks_t<float>(L, M);
+int col_blocks(int prec, int L, int M, int N) {
+    return prec == PREC_FP64 ? col_blocks_t<double>(L, M, N) : col_blocks_t<float>(L, M, N);
 }
 
 template <typename R>
 static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s) {
-    const int nb = row_blocks_t<R>(b.M, b.L);
+    const int nb = row_blocks_t<R>(b.M, b.L * b.N);
     if (nb <= 0) return hipErrorInvalidValue;
     dim3 grid(nb, b.B);
     switch (b.M / 2) {
@@ -679,7 +850,7 @@ static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hip
 
 template <typename R>
 static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s) {
-    const int nb = col_blocks_t<R>(b.L, b.M);
+    const int nb = col_blocks_t<R>(b.L, b.M, b.N);
     if (nb <= 0) return hipErrorInvalidValue;
     dim3 grid(nb, b.B);
     switch (b.L) {
@@ -693,7 +864,7 @@ static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags
 
 template <typename R>
 static hipError_t row_in_t(const DevBufs& b, const void* inphi, const void* incon, hipStream_t s) {
-    const int nb = row_blocks_t<R>(b.M, b.L);
+    const int nb = row_blocks_t<R>(b.M, b.L * b.N);
     if (nb <= 0) return hipErrorInvalidValue;
     dim3 grid(nb, b.B);
     switch (b.M / 2) {
@@ -707,7 +878,7 @@ static hipError_t row_in_t(const DevBufs& b, const void* inphi, const void* inco
 
 template <typename R>
 static hipError_t row_out_t(const DevBufs& b, void* outphi, void* outcon, hipStream_t s) {
-    const int nb = row_blocks_t<R>(b.M, b.L);
+    const int nb = row_blocks_t<R>(b.M, b.L * b.N);
     if (nb <= 0) return hipErrorInvalidValue;
     dim3 grid(nb, b.B);
     switch (b.M / 2) {
@@ -721,7 +892,7 @@ static hipError_t row_out_t(const DevBufs& b, void* outphi, void* outcon, hipStr
 
 template <typename R>
 static hipError_t row_r2c1_t(const DevBufs& b, const void* in, void* out, hipStream_t s) {
-    const int nb = row_blocks_t<R>(b.M, b.L);
+    const int nb = row_blocks_t<R>(b.M, b.L * b.N);
     if (nb <= 0) return hipErrorInvalidValue;
     dim3 grid(nb, b.B);
     switch (b.M / 2) {
@@ -731,6 +902,39 @@ static hipError_t row_r2c1_t(const DevBufs& b, const void* in, void* out, hipStr
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <typename R, int NC, int DIR>
+static hipError_t depth_t(const DevBufs& b, const void* in0, void* out0, const void* in1, void* out1, hipStream_t s) {
+    constexpr int T = 16;
+    if (b.K % T) return hipErrorInvalidValue;
+    dim3 grid(b.L * (b.K / T), b.B);
+    switch (b.N) {
+#define DP(NN) case NN: static_assert(DepCfg<R, NN>::T == T, "tile width"); \
+        hipLaunchKernelGGL((k_depth<R, NN, NC, DIR>), grid, dim3(DepCfg<R, NN>::NT), 0, s, b, in0, out0, in1, out1); break;
+        DP(16) DP(32) DP(64) DP(128)
+#undef DP
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t depth_r(const DevBufs& b, int dir, const void* in0, void* out0, const void* in1, void* out1,
+                          bool one_field, hipStream_t s) {
+    if (dir < 0) {
+        if (one_field) return depth_t<R, 1, -1>(b, in0, out0, nullptr, nullptr, s);
+        if (!in1 && !out1) return depth_t<R, 2, -1>(b, in0, out0, nullptr, nullptr, s);
+        return depth_t<R, 4, -1>(b, in0, out0, in1, out1, s);
+    }
+    if (one_field || !in1 || !out1) return hipErrorInvalidValue;
+    return depth_t<R, 4, +1>(b, in0, out0, in1, out1, s);
+}
+
+hipError_t launch_depth(int prec, const DevBufs& b, int dir, const void* in0, void* out0, const void* in1,
+                        void* out1, bool one_field, hipStream_t s) {
+    return prec == PREC_FP64 ? depth_r<double>(b, dir, in0, out0, in1, out1, one_field, s)
+                             : depth_r<float>(b, dir, in0, out0, in1, out1, one_field, s);
 }
 
 hipError_t launch_row_r2c1(int prec, const DevBufs& b, const void* in, void* out, hipStream_t s) {

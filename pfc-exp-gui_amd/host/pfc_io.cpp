@@ -119,8 +119,9 @@ bool write_strain(const std::string& path, int n, const double* x, const double*
 }
 
 bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float phimax, int n, const float* px,
-                     const float* py, const float* pz, const float* pphi) {
-    FILE* f = std::fopen("phimax.txt", "a");
+                     const float* py, const float* pz, const float* pphi, const std::string& dir) {
+    const std::string pre = dir.empty() ? std::string() : dir + "/";
+    FILE* f = std::fopen((pre + "phimax.txt").c_str(), "a");
     if (!f) return false;
     std::fprintf(f, "%d , %s , %s , %d\n", kd, g6(phic).c_str(), g6(phimax).c_str(), 0);
     std::fprintf(f, "phimax is done myid=0\n buf is done \n pc_xyz is initializing \n pc_xyz is done \n");
@@ -129,7 +130,7 @@ bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float ph
     const float frame_x = L * dx, frame_y = M * dx, frame_z = N * dx;
     char name[64];
     std::snprintf(name, sizeof name, "phiball_%d.cfg", kd);
-    f = std::fopen(name, "w");
+    f = std::fopen((pre + name).c_str(), "w");
     if (!f) return false;
     std::fprintf(f, "Number of particles = %d\n", n);
     std::fprintf(f, "A = %.4f Angstrom (basic length-scale)\n", 10.0);
@@ -142,7 +143,7 @@ bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float ph
     for (int i = 0; i < n; i++) std::fprintf(f, "%.4f %.4f %.4f\n", px[i], py[i], pz[i]);
     std::fclose(f);
     std::snprintf(name, sizeof name, "Phimax_%d.txt", kd);
-    f = std::fopen(name, "w");
+    f = std::fopen((pre + name).c_str(), "w");
     if (!f) return false;
     std::fprintf(f, "%d\nKommentar\n", n);
     for (int i = 0; i < n; i++)

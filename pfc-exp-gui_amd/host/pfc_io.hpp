@@ -26,8 +26,9 @@ bool write_strain(const std::string& path, int n, const double* x, const double*
                   const double* strain, const unsigned* nb);
 // CVD checkpoint outputs of atompositionCFGC2 (cvd.cpp:1086-1249):
 // phimax.txt (append), phiball_<kd>.cfg and Phimax_<kd>.txt.
+// Files go to `dir` (empty: the working directory).
 bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float phimax, int n, const float* px,
-                     const float* py, const float* pz, const float* pphi);
+                     const float* py, const float* pz, const float* pphi, const std::string& dir = "");
 
 // Default iostream formatting of a double (%g, 6 significant digits).
 std::string g6(double v);

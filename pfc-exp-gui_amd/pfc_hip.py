@@ -16,16 +16,21 @@ LIB_PATH = os.path.join(HERE, "libpfc_hip.so")
 
 MISFIT, CVD = 0, 1
 FP64, FP32 = 0, 1
-KID_ROW, KID_COL = 0, 1
+KID_ROW, KID_COL, KID_DEPTH = 0, 1, 2
 
 # Exported symbols (checked by the CPU test suite against include/pfc_hip.h).
 SYMBOLS = [
     "pfc_params_default", "pfc_init_fields", "pfc_create", "pfc_destroy", "pfc_set_fields", "pfc_set_seed",
+    "pfc_set_active",
     "pfc_step", "pfc_sync", "pfc_get_fields", "pfc_get_fields_f32", "pfc_energy",
     "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic",
     "pfc_strain", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
-    "pfc_last_error",
+    "pfc_device_memory", "pfc_last_error",
+    "pfc_server_create", "pfc_server_submit", "pfc_server_poll", "pfc_server_result", "pfc_server_error",
+    "pfc_server_cancel", "pfc_server_drain", "pfc_server_stats", "pfc_server_destroy",
 ]
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_FAILED = range(5)
 
 
 class Params(ctypes.Structure):
@@ -37,6 +42,28 @@ class Params(ctypes.Structure):
         ("grainx", ctypes.c_int), ("grainy", ctypes.c_int), ("grainz", ctypes.c_int),
         ("seed", ctypes.c_uint),
     ]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_double), ("max_slots", ctypes.c_int), ("graph_chunk", ctypes.c_int),
+                ("quantum", ctypes.c_int)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("p", Params), ("steps", ctypes.c_int), ("mod", ctypes.c_int), ("keep_fields", ctypes.c_int),
+                ("outdir", ctypes.c_char_p)]
+
+
+class JobStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("steps_done", ctypes.c_int), ("t_submit", ctypes.c_double),
+                ("t_start", ctypes.c_double), ("t_end", ctypes.c_double), ("energy", ctypes.c_double),
+                ("density", ctypes.c_double)]
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [("now", ctypes.c_double), ("jobs_done", ctypes.c_longlong), ("jobs_unfinished", ctypes.c_int),
+                ("point_steps", ctypes.c_double), ("contexts", ctypes.c_int), ("bytes_in_use", ctypes.c_double),
+                ("budget", ctypes.c_double)]
 
 
 class PfcError(RuntimeError):
@@ -66,6 +93,7 @@ def load():
         "pfc_set_fields": [vp, ctypes.c_int, dp, dp],
         "pfc_step": [vp, ctypes.c_int],
         "pfc_set_seed": [vp, ctypes.c_int, ctypes.c_uint],
+        "pfc_set_active": [vp, ctypes.c_int, ctypes.c_int],
         "pfc_sync": [vp],
         "pfc_get_fields": [vp, ctypes.c_int, dp, dp],
         "pfc_get_fields_f32": [vp, ctypes.c_int, fp, fp],
@@ -80,6 +108,16 @@ def load():
         "pfc_time_kernel": [vp, ctypes.c_int, ctypes.c_int],
         "pfc_kernel_time": [vp, dp, ip],
         "pfc_kernel_bytes": [vp, ctypes.c_int, dp],
+        "pfc_device_memory": [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
+        "pfc_server_create": [ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)],
+        "pfc_server_submit": [vp, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_longlong)],
+        "pfc_server_poll": [vp, ctypes.c_longlong, ctypes.POINTER(JobStatus)],
+        "pfc_server_result": [vp, ctypes.c_longlong, dp, dp],
+        "pfc_server_error": [vp, ctypes.c_longlong, ctypes.c_char_p, ctypes.c_int],
+        "pfc_server_cancel": [vp, ctypes.c_longlong],
+        "pfc_server_drain": [vp, ctypes.c_double],
+        "pfc_server_stats": [vp, ctypes.POINTER(ServerInfo)],
+        "pfc_server_destroy": [vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -154,6 +192,9 @@ class Sim:
 
     def set_seed(self, seed, inst=0):
         _check(self.lib.pfc_set_seed(self.h, inst, seed))
+
+    def set_active(self, inst, active=True):
+        _check(self.lib.pfc_set_active(self.h, inst, 1 if active else 0))
 
     def step(self, n=1):
         _check(self.lib.pfc_step(self.h, n))
@@ -237,3 +278,61 @@ def strain(L, M, N, dx, benchel, x, y, z):
     _check(lib.pfc_strain(L, M, N, ctypes.c_float(dx), ctypes.c_float(benchel), n, _dp(x), _dp(y), _dp(z), _dp(s),
                           nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))))
     return s, nb
+
+
+class Server:
+    """Request loop with continuous batching (pfc_server_* of the C ABI)."""
+
+    def __init__(self, hbm_budget=0.0, max_slots=16, graph_chunk=25, quantum=500):
+        self.lib = load()
+        o = ServerOpts(hbm_budget, max_slots, graph_chunk, quantum)
+        h = ctypes.c_void_p()
+        _check(self.lib.pfc_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, p, steps, mod=0, keep_fields=False, outdir=None):
+        j = Job(p, steps, mod, 1 if keep_fields else 0, outdir.encode() if outdir else None)
+        jid = ctypes.c_longlong()
+        _check(self.lib.pfc_server_submit(self.h, ctypes.byref(j), ctypes.byref(jid)))
+        self._keep[jid.value] = p.L * p.M * p.N
+        return jid.value
+
+    def poll(self, jid):
+        st = JobStatus()
+        _check(self.lib.pfc_server_poll(self.h, jid, ctypes.byref(st)))
+        return st
+
+    def result(self, jid):
+        P = self._keep[jid]
+        phi = np.zeros(P, np.float64)
+        con = np.zeros(P, np.float64)
+        _check(self.lib.pfc_server_result(self.h, jid, _dp(phi), _dp(con)))
+        return phi, con
+
+    def error(self, jid):
+        buf = ctypes.create_string_buffer(512)
+        _check(self.lib.pfc_server_error(self.h, jid, buf, 512))
+        return buf.value.decode()
+
+    def cancel(self, jid):
+        _check(self.lib.pfc_server_cancel(self.h, jid))
+
+    def drain(self, timeout=-1.0):
+        _check(self.lib.pfc_server_drain(self.h, timeout))
+
+    def stats(self):
+        st = ServerInfo()
+        _check(self.lib.pfc_server_stats(self.h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            self.lib.pfc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -98,3 +98,25 @@ def test_cli_elastic_ignores_unknown_and_missing_input(tmp_path):
     assert cfg[0] == "model elastic" and "benchel 1.5" in cfg
     assert (out / "checkpoint_timestamps.txt").read_text().startswith("# Checkpoint timestamps")
     assert "PFC_PROGRESS step=1 total=2" in r.stderr
+
+
+def test_server_admission_limits(hip):
+    """Request-loop host logic that ends before any device work: a job larger
+    than the HBM budget fails at admission, bad arguments are refused, and
+    destroying the server with nothing running returns promptly."""
+    srv = hip.Server(hbm_budget=1e6)
+    p = hip.default_params(model=hip.MISFIT, L=64, M=64, grainx=4, grainy=4)
+    jid = srv.submit(p, 10)
+    srv.drain(30)
+    st = srv.poll(jid)
+    assert st.state == hip.JOB_FAILED and "budget" in srv.error(jid)
+    assert st.t_end >= st.t_submit
+    with pytest.raises(hip.PfcError):
+        srv.submit(p, 0)
+    with pytest.raises(hip.PfcError):
+        srv.poll(jid + 100)
+    with pytest.raises(hip.PfcError):
+        srv.result(jid)
+    info = srv.stats()
+    assert info.jobs_done == 0 and info.jobs_unfinished == 0 and info.budget == 1e6
+    srv.close()
