@@ -18,13 +18,14 @@ Workload (the headline metric "Mpoint-steps/s batched FP32 vs plain FP64
 Initial fields are the reference initialisers (synthetic, seeded).
 
 The dominant kernel (column pass of the 512^2 class) is timed with HIP
-events on its own stream inside the timed region; its algorithmic bytes per
+events on its own stream inside the timed region (hipExtLaunchKernel
+start/stop events: the kernel's own start and end); its algorithmic bytes per
 launch over that duration give roofline.achieved.  The CPU baseline times the
 oracle (the FP64 CPU restatement, single thread) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-       python bench.py --pmc-drive   (a few sweep steps and one 256 MiB copy
-                                      kernel, for hardware-counter runs)
+       python bench.py --pmc-drive   (a few sweep steps and two 256 MiB copy
+                                      kernels, for hardware-counter runs)
 """
 import argparse
 import importlib.util
@@ -97,46 +98,75 @@ def cpu_baseline(budget_s=15.0):
             "sample": f"oracle Misfit 256x256, {steps} steps (faithful: 14 c2c FFTs/step), {el:.1f} s, 1 thread"}
 
 
-def accuracy_study(hip, steps, every, L=512):
-    """FP32 path against the plain FP64 path (the drop-in, itself pinned to the
-    oracle) on Misfit L^2 from the same initial fields."""
-    res = {"workload": f"Misfit {L}x{L}, FP32 vs plain FP64 (reference narrowing), same initial fields",
-           "steps": steps, "every": every, "trace": []}
+def _run_pair(hip, L, steps, every, perturb=0.0, seed=11):
+    """Runs FP64 (reference narrowing, the drop-in) and FP32 from the same
+    initial fields, plus FP64 from fields perturbed by +-`perturb` (the
+    trajectory's own sensitivity to round-off-sized changes); yields
+    (step, fp64, fp32, fp64 perturbed) states every `every` steps."""
     sims = {}
-    for prec in (hip.FP64, hip.FP32):
+    for tag, prec, eps in (("ref", hip.FP64, 0.0), ("fp32", hip.FP32, 0.0), ("pert", hip.FP64, perturb)):
         p = hip.default_params(model=hip.MISFIT, L=L, M=L, precision=prec)
         phi, con = hip.init_fields(p)
+        if eps:
+            phi = phi + eps * np.random.default_rng(seed).uniform(-1.0, 1.0, phi.size)
         s = hip.Sim(p)
         s.set_fields(phi, con)
-        s.set_graph(every // 10 if every % 10 == 0 else 0)
-        sims[prec] = s
-    m0 = {k: v.energy()[0][1] for k, v in sims.items()}
+        s.set_graph(100 if every % 100 == 0 else 0)
+        sims[tag] = s
     t = 0
     while t < steps:
         for s in sims.values():
             s.step(every)
         t += every
-        a = sims[hip.FP64].get_fields()
-        b = sims[hip.FP32].get_fields()
-        ea, eb = sims[hip.FP64].energy()[0], sims[hip.FP32].energy()[0]
-        row = {"step": t}
-        for name, x, y in (("phi", a[0], b[0]), ("con", a[1], b[1])):
-            d = y - x
-            row[f"linf_{name}"] = float(np.abs(d).max())
-            row[f"l2_{name}"] = float(np.sqrt(np.mean(d * d)))
-        row["mean_drift_fp32"] = float(eb[1] - m0[hip.FP32])
-        row["mean_drift_fp64"] = float(ea[1] - m0[hip.FP64])
-        row["energy_rel_diff"] = float(abs(eb[0] - ea[0]) / max(abs(ea[0]), 1e-30))
-        xa = sims[hip.FP64].phimax_misfit()
-        xb = sims[hip.FP32].phimax_misfit()
-        row.update(_atom_match(xa, xb, L * 0.125))
-        res["trace"].append(row)
-    last = res["trace"][-1]
-    res["final"] = {k: last[k] for k in ("linf_phi", "linf_con", "l2_phi", "l2_con")}
-    res["tolerance_linf"] = 1e-4
-    res["within_tolerance"] = bool(max(last["linf_phi"], last["linf_con"]) <= 1e-4)
+        yield t, sims
     for s in sims.values():
         s.close()
+
+
+def accuracy_study(hip, steps, every, L=512):
+    """FP32 path against the plain FP64 path (the drop-in, itself pinned to the
+    oracle) on Misfit L^2 from the same initial fields.  INITIAL_dis seeds
+    crystal patches only in rows m < 160 (misfit.cpp:616); the rest of the
+    grid starts uniform and crystallises from round-off, so the errors are
+    also given on the seeded band, and beside them the distance between two
+    FP64 runs whose initial fields differ by 1e-12."""
+    res = {"workload": f"Misfit {L}x{L}, 10^4 steps: FP32 vs plain FP64 (reference narrowing), same initial "
+                       "fields; 'pert' = plain FP64 from fields perturbed by 1e-12",
+           "steps": steps, "every": every, "trace": []}
+    band = min(L, 160)
+    m0 = None
+    for t, sims in _run_pair(hip, L, steps, every, perturb=1e-12):
+        a = sims["ref"].get_fields()
+        b = sims["fp32"].get_fields()
+        c = sims["pert"].get_fields()
+        ea, eb = sims["ref"].energy()[0], sims["fp32"].energy()[0]
+        if m0 is None:
+            m0 = (ea[1], eb[1])
+        row = {"step": t}
+        for name, k in (("phi", 0), ("con", 1)):
+            d = (b[k] - a[k]).reshape(L, L)
+            q = (c[k] - a[k]).reshape(L, L)
+            row[f"linf_{name}"] = float(np.abs(d).max())
+            row[f"l2_{name}"] = float(np.sqrt(np.mean(d * d)))
+            row[f"linf_{name}_seeded"] = float(np.abs(d[:, :band]).max())
+            row[f"linf_{name}_pert"] = float(np.abs(q).max())
+            row[f"linf_{name}_pert_seeded"] = float(np.abs(q[:, :band]).max())
+        row["mean_phi_fp32_minus_fp64"] = float(eb[1] - ea[1])
+        row["energy_rel_diff"] = float(abs(eb[0] - ea[0]) / max(abs(ea[0]), 1e-30))
+        row.update(_atom_match(sims["ref"].phimax_misfit(), sims["fp32"].phimax_misfit(), L * 0.125))
+        res["trace"].append(row)
+    last = res["trace"][-1]
+    res["final"] = {k: last[k] for k in ("linf_phi", "linf_con", "l2_phi", "l2_con", "linf_phi_seeded",
+                                         "linf_phi_pert", "linf_phi_pert_seeded")}
+    # A grid the initialiser seeds everywhere (m < 160 covers all rows).
+    small = []
+    for t, sims in _run_pair(hip, 128, steps, every, perturb=1e-12):
+        a, b, c = (sims[k].get_fields() for k in ("ref", "fp32", "pert"))
+        small.append({"step": t, "linf_phi": float(np.abs(b[0] - a[0]).max()),
+                      "linf_con": float(np.abs(b[1] - a[1]).max()),
+                      "linf_phi_pert": float(np.abs(c[0] - a[0]).max())})
+    res["fully_seeded_128"] = small
+    res["tolerance_linf"] = 1e-4
     return res
 
 
@@ -274,20 +304,23 @@ def serve_leg(hip, seconds, seed=20240611):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+COPY_PROBE_BYTES = 256 * 1024 * 1024
+
+
 def pmc_drive(hip, steps):
-    """A few sweep steps and one copy kernel of known size (256 MiB read and
-    written) for hardware-counter runs: the copy calibrates the counters."""
-    import torch
+    """A few sweep steps and plain copy kernels of known size (256 MiB read
+    and written; linear, and in the column pass's 64-byte pieces; each
+    launched twice) for hardware-counter runs: the copies calibrate the byte
+    counters."""
     sims = build_sweep(hip, 0)
     for s in sims.values():
         s.step(steps)
         s.sync()
-    n = 64 * 1024 * 1024  # floats
-    a = torch.ones(n, device="cuda", dtype=torch.float32)
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"pmc_drive": True, "steps": steps, "copy_bytes_each_way": 4 * n}))
+    ms = [hip.copy_probe(COPY_PROBE_BYTES, pat) for pat in (0, 1)]
+    print(json.dumps({"pmc_drive": True, "steps": steps, "copy_bytes_each_way": COPY_PROBE_BYTES,
+                      "copy_ms": {"linear": ms[0], "columns_64B": ms[1]},
+                      "copy_GBps": {"linear": 2 * COPY_PROBE_BYTES / ms[0] / 1e6,
+                                    "columns_64B": 2 * COPY_PROBE_BYTES / ms[1] / 1e6}}))
 
 
 def main():
@@ -334,22 +367,35 @@ def main():
     el = time.perf_counter() - t0
     col_ms, col_n = big.kernel_time()
     big.time_kernel(hip.KID_COL, False)
+    # The same kernel with the GPU to itself (the 512^2 class alone), for
+    # comparison with its duration beside the other three classes.
+    big.time_kernel(hip.KID_COL, True)
+    big.step(100)
+    iso_ms, iso_n = big.kernel_time()
+    big.time_kernel(hip.KID_COL, False)
     col_bytes = big.kernel_bytes(hip.KID_COL)
     value = pts * args.steps / el / 1e6
     achieved = col_bytes / (col_ms * 1e-3) / 1e9 if col_ms > 0 else 0.0
 
-    # ---- plain FP64 path (one instance, eager) -------------------------------
-    p = hip.default_params(model=hip.MISFIT, L=512, M=512, precision=hip.FP64)
-    phi, con = hip.init_fields(p)
-    s64 = hip.Sim(p)
-    s64.set_fields(phi, con)
-    s64.step(10)
-    s64.sync()
-    t1 = time.perf_counter()
-    s64.step(args.fp64_steps)
-    s64.sync()
-    el64 = time.perf_counter() - t1
-    fp64 = 512 * 512 * args.fp64_steps / el64 / 1e6
+    # ---- plain FP64 path (one instance at a time, eager) --------------------
+    # The same 64 sweep instances run one after another: per-step time of one
+    # instance of each grid, weighted by the sweep's 16 instances per grid.
+    per_grid = {}
+    for g in (128, 256, 384, 512):
+        p = hip.default_params(model=hip.MISFIT, L=g, M=g, precision=hip.FP64)
+        phi, con = hip.init_fields(p)
+        s64 = hip.Sim(p)
+        s64.set_fields(phi, con)
+        s64.step(10)
+        s64.sync()
+        t1 = time.perf_counter()
+        s64.step(args.fp64_steps)
+        s64.sync()
+        per_grid[g] = (time.perf_counter() - t1) / args.fp64_steps
+        s64.close()
+    el64 = 16 * sum(per_grid.values())  # seconds per sweep step
+    fp64 = pts / el64 / 1e6
+    fp64_512 = 512 * 512 / per_grid[512] / 1e6
 
     out = {
         "metric": METRIC,
@@ -368,9 +414,12 @@ def main():
             "workload": "Misfit sweep, 64 instances (4 u0 x 4 con0 x 4 seeds), 16 each of 128^2/256^2/384^2/512^2, "
                         "FP32 + FP64 k=0/reductions, hipGraph chunks of %d steps" % args.graph_chunk,
             "points_per_step": pts,
-            "fp64_plain_workload": "Misfit 512^2, one instance, FP64 (reference float narrowing), eager",
         },
-        "fp64_plain": {"value": round(fp64, 1), "unit": "Mpoint-steps/s", "ms_per_step": round(el64 / args.fp64_steps * 1e3, 4)},
+        "fp64_plain": {"value": round(fp64, 1), "unit": "Mpoint-steps/s",
+                       "workload": "the sweep's 64 instances one at a time, FP64 (reference narrowing), eager",
+                       "ms_per_sweep_step": round(el64 * 1e3, 4),
+                       "ms_per_step_by_grid": {str(g): round(t * 1e3, 4) for g, t in per_grid.items()}},
+        "fp64_plain_512": {"value": round(fp64_512, 1), "unit": "Mpoint-steps/s"},
         "fp32_batched_over_fp64_plain": round(value / fp64, 2),
         "roofline": {
             "bound": "hbm",
@@ -382,6 +431,9 @@ def main():
             "traffic": None,
             "launches_timed": col_n,
             "mean_launch_ms": round(col_ms, 5),
+            "timing": "hipExtLaunchKernel start/stop events on every launch of the kernel in the timed region",
+            "mean_launch_ms_alone": round(iso_ms, 5),
+            "launches_alone": iso_n,
             "bytes_per_launch": col_bytes,
         },
     }
@@ -392,7 +444,6 @@ def main():
         out["roofline"]["traffic_source"] = "profiles/traffic_k_col_float_512_sweep16.json (rocprofv3 --pmc)"
     for s in sims.values():
         s.close()
-    s64.close()
     if args.accuracy_steps > 0:
         out["accuracy"] = accuracy_study(hip, args.accuracy_steps, max(1, args.accuracy_steps // 10))
         print(f"accuracy done: {out['accuracy']['final']}", file=sys.stderr, flush=True)

@@ -121,13 +121,21 @@ int pfc_set_graph(pfc_ctx* ctx, int chunk);
 
 /* Kernel timing with HIP events on the context's stream: kernel 0 = row pass,
  * 1 = column pass, 2 = depth passes (N > 1).  While enabled, every launch of
- * that kernel is bracketed by an event pair; pfc_kernel_time reports the mean
- * duration since enabling. */
+ * that kernel carries a start/stop event pair that takes the kernel's own
+ * start and completion times (hipExtLaunchKernel) and steps launch without
+ * graphs; pfc_kernel_time reports the mean duration since enabling. */
 int pfc_time_kernel(pfc_ctx* ctx, int kernel, int enable);
 int pfc_kernel_time(pfc_ctx* ctx, double* mean_ms, int* count);
 
 /* Algorithmic HBM bytes one launch of kernel `kernel` moves (all instances). */
 int pfc_kernel_bytes(pfc_ctx* ctx, int kernel, double* bytes);
+
+/* Measurement aid, no counterpart in the reference: two launches of a plain
+ * device copy of `nbytes` (a multiple of 4096; reads and writes nbytes each),
+ * the second timed.  pattern 0: linear; 1: 64-byte pieces, column-wise over
+ * 4096-byte rows (the column pass's access granularity).  Calibrates hardware
+ * byte counters against kernels of known traffic. */
+int pfc_copy_probe(size_t nbytes, int pattern, double* ms);
 
 /* Free and total device memory of the current GPU (hipMemGetInfo). */
 int pfc_device_memory(size_t* free_bytes, size_t* total_bytes);

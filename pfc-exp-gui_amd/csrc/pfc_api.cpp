@@ -87,6 +87,32 @@ size_t ctx_bytes(const pfc_params& p, int batch) {
 }
 }  // namespace pfc
 
+extern "C" int pfc_copy_probe(size_t nbytes, int pattern, double* ms) {
+    if (nbytes < 4096 || !ms || pattern < 0 || pattern > 1) return fail(PFC_E_ARG, "bad arguments");
+    const size_t n = nbytes / 4096 * 256;  // whole rows of 256 words
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, n * 16);
+    if (e == hipSuccess) e = hipMalloc(&b, n * 16);
+    if (e == hipSuccess) e = hipMemset(a, 1, n * 16);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_copy16(a, b, n, pattern, nullptr);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = launch_copy16(a, b, n, pattern, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "copy probe: %s", hipGetErrorString(e));
+    *ms = t;
+    return PFC_OK;
+}
+
 extern "C" int pfc_device_memory(size_t* free_bytes, size_t* total_bytes) {
     if (!free_bytes || !total_bytes) return fail(PFC_E_ARG, "bad arguments");
     HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
@@ -315,18 +341,16 @@ static hipEvent_t next_event(pfc_ctx* c) {
     return c->ev[c->ev_used++];
 }
 
-// Event pair around one launch of the timed kernel class.
+// Start/stop events for one launch of the timed kernel class (none if
+// another class is timed).
 struct TimedLaunch {
-    pfc_ctx* c;
-    hipEvent_t e1 = nullptr;
-    TimedLaunch(pfc_ctx* cc, int kid) : c(cc) {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    TimedLaunch(pfc_ctx* c, int kid) {
         if (c->time_kid != kid) return;
-        hipEvent_t e0 = next_event(c);
-        e1 = next_event(c);
-        if (e0) hipEventRecord(e0, c->stream);
-    }
-    ~TimedLaunch() {
-        if (e1) hipEventRecord(e1, c->stream);
+        const size_t used = c->ev_used;
+        t0 = next_event(c);
+        t1 = t0 ? next_event(c) : nullptr;
+        if (!t1) { t0 = nullptr; c->ev_used = used; }
     }
 };
 
@@ -340,26 +364,20 @@ static hipError_t enqueue_step(pfc_ctx* c, int row_flags, int col_flags, int eco
         e = launch_noise(c->prec, b, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr, c->stream);
         if (e != hipSuccess) return e;
     }
-    {
-        TimedLaunch tl(c, KID_ROW);
-        e = launch_row_step(c->prec, b, c->phys, row_flags, c->stream);
-    }
+    TimedLaunch tr(c, KID_ROW);
+    e = launch_row_step(c->prec, b, c->phys, row_flags, c->stream, tr.t0, tr.t1);
     if (e != hipSuccess) return e;
     if (c->N > 1) {
-        {
-            TimedLaunch tl(c, KID_DEPTH);
-            e = launch_depth(c->prec, b, -1, b.S, b.S, b.IN, b.IN, false, c->stream);
-        }
+        TimedLaunch td(c, KID_DEPTH);
+        e = launch_depth(c->prec, b, -1, b.S, b.S, b.IN, b.IN, false, c->stream, td.t0, td.t1);
         if (e == hipSuccess && c->noise) e = launch_depth(c->prec, b, -1, b.NZ, (void*)b.NZ, nullptr, nullptr, true, c->stream);
         if (e != hipSuccess) return e;
     }
-    {
-        TimedLaunch tl(c, KID_COL);
-        e = launch_col(c->prec, b, c->phys, COL_STEP, col_flags, c->stream);
-    }
+    TimedLaunch tc(c, KID_COL);
+    e = launch_col(c->prec, b, c->phys, COL_STEP, col_flags, c->stream, tc.t0, tc.t1);
     if (e != hipSuccess || c->N == 1) return e;
-    TimedLaunch tl(c, KID_DEPTH);
-    return launch_depth(c->prec, b, +1, b.S, b.S, b.AB, b.AB, false, c->stream);
+    TimedLaunch ti(c, KID_DEPTH);
+    return launch_depth(c->prec, b, +1, b.S, b.S, b.AB, b.AB, false, c->stream, ti.t0, ti.t1);
 }
 
 extern "C" int pfc_set_graph(pfc_ctx* c, int chunk) {

@@ -63,8 +63,11 @@ bool depth_supported(int n);                    // N > 1 lengths of the depth pa
 int row_blocks(int prec, int M, int rows);      // row-pass workgroups per instance (rows = L*N)
 int col_blocks(int prec, int L, int M, int N);  // column-pass workgroups per instance
 
-hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s);
-hipError_t launch_col(int prec, const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s);
+// t0/t1 (optional): events that take the kernel's own start and end times.
+hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s,
+                           hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
+hipError_t launch_col(int prec, const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s,
+                      hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 hipError_t launch_row_in(int prec, const DevBufs& b, const void* inphi, const void* incon, hipStream_t s);
 hipError_t launch_row_out(int prec, const DevBufs& b, void* outphi, void* outcon, hipStream_t s);
 // Depth pass (N > 1): FFT along n of one or two field pairs, dir -1 forward
@@ -72,7 +75,7 @@ hipError_t launch_row_out(int prec, const DevBufs& b, void* outphi, void* outcon
 // result scaled by 1/N).  in1/out1 may be null (one pair).  With one_field
 // set, in0/out0 hold single complex values (the noise spectrum) instead.
 hipError_t launch_depth(int prec, const DevBufs& b, int dir, const void* in0, void* out0, const void* in1,
-                        void* out1, bool one_field, hipStream_t s);
+                        void* out1, bool one_field, hipStream_t s, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 // One real plane per instance [B][L][M][N] -> row spectrum [B][L][N][K] (single field).
 hipError_t launch_row_r2c1(int prec, const DevBufs& b, const void* in, void* out, hipStream_t s);
 // e_out[inst*2 + {0,1}] = {mean energy, mean density}; alpha_over_P = alpha/P.

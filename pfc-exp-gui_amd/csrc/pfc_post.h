@@ -24,4 +24,8 @@ hipError_t post_phimax_elastic(int L, int M, int N, float dx, const float* phi, 
 hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
                        const double* z, double* strain, unsigned* nb);
 
+// Plain device copy of n 16-byte words (counter calibration); pattern 0:
+// linear, 1: 64-byte pieces of a [n/256][256] array copied column-wise.
+hipError_t launch_copy16(const void* src, void* dst, size_t n, int pattern, hipStream_t s);
+
 }  // namespace pfc

@@ -657,4 +657,29 @@ hipError_t launch_noise(int prec, const DevBufs& b, const unsigned* seeds, void*
     return launch_row_r2c1(prec, b, planes, nz, s);
 }
 
+// Plain copy of n 16-byte words (reads n*16 bytes, writes n*16 bytes).
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+// The same words viewed as [n/256][256]: each workgroup copies 4 adjacent
+// columns down all rows, so every row contributes one 64-byte piece per
+// workgroup -- the access granularity of the column pass.
+__global__ void __launch_bounds__(256) k_copy16_cols(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     size_t rows) {
+    const size_t col = (size_t)blockIdx.x * 4 + threadIdx.x % 4;
+    for (size_t r = (size_t)blockIdx.y * 64 + threadIdx.x / 4; r < rows; r += (size_t)gridDim.y * 64)
+        dst[r * 256 + col] = src[r * 256 + col];
+}
+
+hipError_t launch_copy16(const void* src, void* dst, size_t n, int pattern, hipStream_t s) {
+    if (pattern == 0) {
+        hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+    } else {
+        if (n % 256) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_copy16_cols, dim3(64, 256), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n / 256);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace pfc

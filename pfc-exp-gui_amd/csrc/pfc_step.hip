@@ -16,8 +16,18 @@
 //
 // Both passes stream each array through HBM once; all transforms and the
 // pointwise algebra happen in LDS and VGPRs.
+#include <hip/hip_ext.h>
+
 #include "pfc_fft.hpp"
 #include "pfc_kernels.h"
+
+// Launch; with a start/stop event pair the events take the kernel's own
+// start and completion times (hipExtLaunchKernelGGL).
+#define PFC_LAUNCH(kern, grid, block, s, t0, t1, ...)                                  \
+    do {                                                                               \
+        if (t0) hipExtLaunchKernelGGL(kern, grid, block, 0, s, t0, t1, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(kern, grid, block, 0, s, __VA_ARGS__);                 \
+    } while (0)
 
 namespace pfc {
 
@@ -835,12 +845,13 @@ int col_blocks(int prec, int L, int M, int N) {
 }
 
 template <typename R>
-static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s) {
+static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s, hipEvent_t t0,
+                             hipEvent_t t1) {
     const int nb = row_blocks_t<R>(b.M, b.L * b.N);
     if (nb <= 0) return hipErrorInvalidValue;
     dim3 grid(nb, b.B);
     switch (b.M / 2) {
-#define RS(H) case H: hipLaunchKernelGGL((k_row_step<R, H>), grid, dim3(RowCfg<R, H>::NT), 0, s, b, p, flags); break;
+#define RS(H) case H: PFC_LAUNCH((k_row_step<R, H>), grid, dim3(RowCfg<R, H>::NT), s, t0, t1, b, p, flags); break;
         RS(16) RS(32) RS(64) RS(128) RS(192) RS(256) RS(384) RS(512)
 #undef RS
         default: return hipErrorInvalidValue;
@@ -849,12 +860,13 @@ static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hip
 }
 
 template <typename R>
-static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s) {
+static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s, hipEvent_t t0,
+                        hipEvent_t t1) {
     const int nb = col_blocks_t<R>(b.L, b.M, b.N);
     if (nb <= 0) return hipErrorInvalidValue;
     dim3 grid(nb, b.B);
     switch (b.L) {
-#define CS(LL) case LL: hipLaunchKernelGGL((k_col<R, LL>), grid, dim3(ColCfg<R, LL>::NT), 0, s, b, p, mode, flags); break;
+#define CS(LL) case LL: PFC_LAUNCH((k_col<R, LL>), grid, dim3(ColCfg<R, LL>::NT), s, t0, t1, b, p, mode, flags); break;
         CS(32) CS(64) CS(128) CS(256) CS(384) CS(512) CS(768) CS(1024)
 #undef CS
         default: return hipErrorInvalidValue;
@@ -905,13 +917,14 @@ static hipError_t row_r2c1_t(const DevBufs& b, const void* in, void* out, hipStr
 }
 
 template <typename R, int NC, int DIR>
-static hipError_t depth_t(const DevBufs& b, const void* in0, void* out0, const void* in1, void* out1, hipStream_t s) {
+static hipError_t depth_t(const DevBufs& b, const void* in0, void* out0, const void* in1, void* out1, hipStream_t s,
+                          hipEvent_t t0, hipEvent_t t1) {
     constexpr int T = 16;
     if (b.K % T) return hipErrorInvalidValue;
     dim3 grid(b.L * (b.K / T), b.B);
     switch (b.N) {
 #define DP(NN) case NN: static_assert(DepCfg<R, NN>::T == T, "tile width"); \
-        hipLaunchKernelGGL((k_depth<R, NN, NC, DIR>), grid, dim3(DepCfg<R, NN>::NT), 0, s, b, in0, out0, in1, out1); break;
+        PFC_LAUNCH((k_depth<R, NN, NC, DIR>), grid, dim3(DepCfg<R, NN>::NT), s, t0, t1, b, in0, out0, in1, out1); break;
         DP(16) DP(32) DP(64) DP(128)
 #undef DP
         default: return hipErrorInvalidValue;
@@ -921,31 +934,34 @@ static hipError_t depth_t(const DevBufs& b, const void* in0, void* out0, const v
 
 template <typename R>
 static hipError_t depth_r(const DevBufs& b, int dir, const void* in0, void* out0, const void* in1, void* out1,
-                          bool one_field, hipStream_t s) {
+                          bool one_field, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
     if (dir < 0) {
-        if (one_field) return depth_t<R, 1, -1>(b, in0, out0, nullptr, nullptr, s);
-        if (!in1 && !out1) return depth_t<R, 2, -1>(b, in0, out0, nullptr, nullptr, s);
-        return depth_t<R, 4, -1>(b, in0, out0, in1, out1, s);
+        if (one_field) return depth_t<R, 1, -1>(b, in0, out0, nullptr, nullptr, s, t0, t1);
+        if (!in1 && !out1) return depth_t<R, 2, -1>(b, in0, out0, nullptr, nullptr, s, t0, t1);
+        return depth_t<R, 4, -1>(b, in0, out0, in1, out1, s, t0, t1);
     }
     if (one_field || !in1 || !out1) return hipErrorInvalidValue;
-    return depth_t<R, 4, +1>(b, in0, out0, in1, out1, s);
+    return depth_t<R, 4, +1>(b, in0, out0, in1, out1, s, t0, t1);
 }
 
 hipError_t launch_depth(int prec, const DevBufs& b, int dir, const void* in0, void* out0, const void* in1,
-                        void* out1, bool one_field, hipStream_t s) {
-    return prec == PREC_FP64 ? depth_r<double>(b, dir, in0, out0, in1, out1, one_field, s)
-                             : depth_r<float>(b, dir, in0, out0, in1, out1, one_field, s);
+                        void* out1, bool one_field, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+    return prec == PREC_FP64 ? depth_r<double>(b, dir, in0, out0, in1, out1, one_field, s, t0, t1)
+                             : depth_r<float>(b, dir, in0, out0, in1, out1, one_field, s, t0, t1);
 }
 
 hipError_t launch_row_r2c1(int prec, const DevBufs& b, const void* in, void* out, hipStream_t s) {
     return prec == PREC_FP64 ? row_r2c1_t<double>(b, in, out, s) : row_r2c1_t<float>(b, in, out, s);
 }
 
-hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s) {
-    return prec == PREC_FP64 ? row_step_t<double>(b, p, flags, s) : row_step_t<float>(b, p, flags, s);
+hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s, hipEvent_t t0,
+                           hipEvent_t t1) {
+    return prec == PREC_FP64 ? row_step_t<double>(b, p, flags, s, t0, t1) : row_step_t<float>(b, p, flags, s, t0, t1);
 }
-hipError_t launch_col(int prec, const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s) {
-    return prec == PREC_FP64 ? col_t<double>(b, p, mode, flags, s) : col_t<float>(b, p, mode, flags, s);
+hipError_t launch_col(int prec, const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s,
+                      hipEvent_t t0, hipEvent_t t1) {
+    return prec == PREC_FP64 ? col_t<double>(b, p, mode, flags, s, t0, t1)
+                             : col_t<float>(b, p, mode, flags, s, t0, t1);
 }
 hipError_t launch_row_in(int prec, const DevBufs& b, const void* inphi, const void* incon, hipStream_t s) {
     return prec == PREC_FP64 ? row_in_t<double>(b, inphi, incon, s) : row_in_t<float>(b, inphi, incon, s);

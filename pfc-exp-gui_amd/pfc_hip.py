@@ -25,7 +25,7 @@ SYMBOLS = [
     "pfc_step", "pfc_sync", "pfc_get_fields", "pfc_get_fields_f32", "pfc_energy",
     "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic",
     "pfc_strain", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
-    "pfc_device_memory", "pfc_last_error",
+    "pfc_device_memory", "pfc_copy_probe", "pfc_last_error",
     "pfc_server_create", "pfc_server_submit", "pfc_server_poll", "pfc_server_result", "pfc_server_error",
     "pfc_server_cancel", "pfc_server_drain", "pfc_server_stats", "pfc_server_destroy",
 ]
@@ -109,6 +109,7 @@ def load():
         "pfc_kernel_time": [vp, dp, ip],
         "pfc_kernel_bytes": [vp, ctypes.c_int, dp],
         "pfc_device_memory": [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
+        "pfc_copy_probe": [ctypes.c_size_t, ctypes.c_int, dp],
         "pfc_server_create": [ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)],
         "pfc_server_submit": [vp, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_longlong)],
         "pfc_server_poll": [vp, ctypes.c_longlong, ctypes.POINTER(JobStatus)],
@@ -140,6 +141,14 @@ def _dp(a):
 
 def _fp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def copy_probe(nbytes, pattern=0):
+    """Milliseconds of one plain device copy of nbytes (counter calibration);
+    pattern 0 linear, 1 column-wise 64-byte pieces."""
+    ms = ctypes.c_double()
+    _check(load().pfc_copy_probe(nbytes, pattern, ctypes.byref(ms)))
+    return ms.value
 
 
 def default_params(model=MISFIT, L=256, M=256, N=1, precision=FP64, **kw):
