@@ -129,22 +129,78 @@ __device__ __forceinline__ void bfly8(Vec<R, NC>* v) {
     }
 }
 
+// cos and sin of 2 pi m / 12 (exact multiples of 30 degrees).
+__device__ __forceinline__ constexpr double cos12(int m) {
+    constexpr double t[12] = {1.0, 0.86602540378443864676, 0.5, 0.0, -0.5, -0.86602540378443864676,
+                              -1.0, -0.86602540378443864676, -0.5, 0.0, 0.5, 0.86602540378443864676};
+    return t[((m % 12) + 12) % 12];
+}
+__device__ __forceinline__ constexpr double sin12(int m) { return cos12(m - 3); }
+
+// Radix R1*R2 = 12 or 6 in registers (Cooley-Tukey inside the butterfly):
+// R2 radix-R1 transforms over n1 of x[R2*n1 + n2], the twiddles
+// exp(DIR 2 pi i n2 k1 / R), then R1 radix-R2 transforms over n2; the
+// output index is k1 + R1*k2.
+template <int R1, int R2, int DIR, typename R, int NC>
+__device__ __forceinline__ void bfly_split(Vec<R, NC>* v) {
+    constexpr int RR = R1 * R2;
+    Vec<R, NC> y[RR];  // y[n2*R1 + k1]
+#pragma unroll
+    for (int n2 = 0; n2 < R2; n2++) {
+        Vec<R, NC> a[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) a[n1] = v[R2 * n1 + n2];
+        if constexpr (R1 == 2) bfly2<DIR>(a);
+        else if constexpr (R1 == 3) bfly3<DIR>(a);
+        else bfly4<DIR>(a);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) {
+            const int m = (n2 * k1 * (12 / RR)) % 12;  // angle in units of 2 pi / 12
+            if (m == 0) {
+                y[n2 * R1 + k1] = a[k1];
+            } else if (m == 3 || m == 9) {
+                y[n2 * R1 + k1] = (m == 3) == (DIR > 0) ? vmuli<+1>(a[k1]) : vmuli<-1>(a[k1]);
+            } else {
+                y[n2 * R1 + k1] = vmulw(a[k1], Cx<R>{R(cos12(m)), R(DIR) * R(sin12(m))});
+            }
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; k1++) {
+        Vec<R, NC> b[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) b[n2] = y[n2 * R1 + k1];
+        if constexpr (R2 == 2) bfly2<DIR>(b);
+        else if constexpr (R2 == 3) bfly3<DIR>(b);
+        else bfly4<DIR>(b);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) v[k1 + R1 * k2] = b[k2];
+    }
+}
+
 template <int RAD, int DIR, typename R, int NC>
 __device__ __forceinline__ void bfly(Vec<R, NC>* v) {
     if constexpr (RAD == 2) bfly2<DIR>(v);
     else if constexpr (RAD == 3) bfly3<DIR>(v);
     else if constexpr (RAD == 4) bfly4<DIR>(v);
+    else if constexpr (RAD == 6) bfly_split<2, 3, DIR>(v);
     else if constexpr (RAD == 8) bfly8<DIR>(v);
+    else if constexpr (RAD == 12) bfly_split<4, 3, DIR>(v);
 }
 
 // Radix choice for the next pass: largest radix that divides both the
 // remaining length and the per-thread element count.
+#ifndef PFC_MAX_RADIX
+#define PFC_MAX_RADIX 12
+#endif
 constexpr int pick_radix(int rem, int E) {
-    return (rem % 8 == 0 && E % 8 == 0)   ? 8
-           : (rem % 4 == 0 && E % 4 == 0) ? 4
-           : (rem % 2 == 0 && E % 2 == 0) ? 2
-           : (rem % 3 == 0 && E % 3 == 0) ? 3
-                                          : -1;
+    return (PFC_MAX_RADIX >= 12 && rem % 12 == 0 && E % 12 == 0) ? 12
+           : (rem % 8 == 0 && E % 8 == 0)                         ? 8
+           : (PFC_MAX_RADIX >= 6 && rem % 6 == 0 && E % 6 == 0)   ? 6
+           : (rem % 4 == 0 && E % 4 == 0)                         ? 4
+           : (rem % 2 == 0 && E % 2 == 0)                         ? 2
+           : (rem % 3 == 0 && E % 3 == 0)                         ? 3
+                                                                  : -1;
 }
 
 constexpr bool plan_ok(int n, int E, int ns = 1) {
@@ -172,8 +228,55 @@ constexpr int elems_for(int n, int pow2) { return (n % 3 == 0) ? 12 : (n >= pow2
 constexpr int padded(int n) { return n + n / 8 + 1; }
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
 
-template <typename R, int NC, int N, int E, int DIR, int NS, bool FIRST>
-__device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], Vec<R, NC>* lds, int t,
+// Exchange buffer of one transform.  CH = 1: one Vec per slot (AoS).
+// CH > 1: the Vec is stored as CH chunks of 16 bytes in CH arrays PS slots
+// apart (structure of arrays), so each LDS access moves 16 bytes and lanes
+// that touch consecutive slots touch consecutive 16-byte bank groups.
+struct alignas(16) Chunk16 {
+    unsigned int w[4];
+};
+template <typename R, int NC, int CH, int PS>
+struct LdsBuf {
+    char* p;
+    static_assert(CH == 1 || sizeof(Vec<R, NC>) == 16 * CH, "chunks are 16 bytes");
+    __device__ __forceinline__ Vec<R, NC> ld(int i) const {
+        if constexpr (CH == 1) {
+            return ((const Vec<R, NC>*)p)[i];
+        } else {
+            Vec<R, NC> v;
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                const Chunk16 k = ((const Chunk16*)p)[c * PS + i];
+                __builtin_memcpy((char*)&v + 16 * c, &k, 16);
+            }
+            return v;
+        }
+    }
+    __device__ __forceinline__ void st(int i, const Vec<R, NC>& v) const {
+        if constexpr (CH == 1) {
+            ((Vec<R, NC>*)p)[i] = v;
+        } else {
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                Chunk16 k;
+                __builtin_memcpy(&k, (const char*)&v + 16 * c, 16);
+                ((Chunk16*)p)[c * PS + i] = k;
+            }
+        }
+    }
+};
+
+// Slots per chunk array for an SoA buffer of n padded slots: the first
+// PS >= padded(n) with (CH * PS) % 16 == MOD, so that buffers placed one
+// after another start MOD 16-byte groups apart.
+constexpr int soa_stride(int n, int CH, int MOD) {
+    int ps = padded(n);
+    for (int k = 0; k < 16 && (CH * ps) % 16 != MOD; k++) ps++;
+    return ps;
+}
+
+template <typename R, int NC, int N, int E, int DIR, int NS, bool FIRST, typename L>
+__device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], const L& lds, int t,
                                                const Cx<R>* __restrict__ tw) {
     if constexpr (NS < N) {
         constexpr int RAD = pick_radix(N / NS, E);
@@ -190,7 +293,7 @@ __device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], Vec<R, NC>* l
 #pragma unroll
             for (int b = 0; b < NB; b++)
 #pragma unroll
-                for (int r = 0; r < RAD; r++) v[b * RAD + r] = lds[pidx(t + G * b + r * STEP)];
+                for (int r = 0; r < RAD; r++) v[b * RAD + r] = lds.ld(pidx(t + G * b + r * STEP));
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
@@ -218,19 +321,25 @@ __device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], Vec<R, NC>* l
                 const int j = t + G * b;
                 const int idx = (j / NS) * NS * RAD + j % NS;
 #pragma unroll
-                for (int r = 0; r < RAD; r++) lds[pidx(idx + r * NS)] = v[b * RAD + r];
+                for (int r = 0; r < RAD; r++) lds.st(pidx(idx + r * NS), v[b * RAD + r]);
             }
             __syncthreads();
-            fft_reg_passes<R, NC, N, E, DIR, NS * RAD, false>(x, lds, t, tw);
+            fft_reg_passes<R, NC, N, E, DIR, NS * RAD, false, L>(x, lds, t, tw);
         }
     }
 }
 
-template <typename R, int NC, int N, int E, int DIR>
-__device__ __forceinline__ void fft_reg(Vec<R, NC> (&x)[E], Vec<R, NC>* lds, int t, const Cx<R>* __restrict__ tw) {
+template <typename R, int NC, int N, int E, int DIR, typename L>
+__device__ __forceinline__ void fft_reg(Vec<R, NC> (&x)[E], const L& lds, int t, const Cx<R>* __restrict__ tw) {
     static_assert(N % E == 0, "length must be a multiple of elements per thread");
     static_assert(plan_ok(N, E), "no radix plan");
-    fft_reg_passes<R, NC, N, E, DIR, 1, true>(x, lds, t, tw);
+    fft_reg_passes<R, NC, N, E, DIR, 1, true, L>(x, lds, t, tw);
+}
+
+// AoS exchange buffer (one Vec per slot).
+template <typename R, int NC, int N, int E, int DIR>
+__device__ __forceinline__ void fft_reg(Vec<R, NC> (&x)[E], Vec<R, NC>* lds, int t, const Cx<R>* __restrict__ tw) {
+    fft_reg<R, NC, N, E, DIR>(x, LdsBuf<R, NC, 1, 0>{(char*)lds}, t, tw);
 }
 
 }  // namespace pfc

@@ -31,6 +31,52 @@
 
 namespace pfc {
 
+// Buffer access to workgroup-uniform arrays: the descriptor (base, byte
+// range) lives in SGPRs, the per-thread byte offset in one VGPR and the
+// uniform part of the offset (per unrolled element) in an SGPR, so no 64-bit
+// address arithmetic is done per element.  Element sizes 8 or 16*k bytes.
+typedef __amdgpu_buffer_rsrc_t Buf;
+__device__ __forceinline__ Buf mkbuf(const void* base, size_t bytes) {
+    const uint32_t n = bytes > 0xffffffffull ? 0xffffffffu : (uint32_t)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)n, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(Buf r, uint32_t voff, uint32_t soff) {
+    T out;
+    if constexpr (sizeof(T) == 8) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const f2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        __builtin_memcpy(&out, &v, 8);
+    } else {
+        static_assert(sizeof(T) % 16 == 0, "element size");
+        typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int c = 0; c < (int)(sizeof(T) / 16); c++) {
+            const f4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * c, soff, 0);
+            __builtin_memcpy((char*)&out + 16 * c, &v, 16);
+        }
+    }
+    return out;
+}
+template <typename T>
+__device__ __forceinline__ void bstore(const T& val, Buf r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(T) == 8) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        f2 v;
+        __builtin_memcpy(&v, &val, 8);
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+    } else {
+        static_assert(sizeof(T) % 16 == 0, "element size");
+        typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int c = 0; c < (int)(sizeof(T) / 16); c++) {
+            f4 v;
+            __builtin_memcpy(&v, (const char*)&val + 16 * c, 16);
+            __builtin_amdgcn_raw_buffer_store_b128(v, r, voff + 16 * c, soff, 0);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Pointwise physics
 // ---------------------------------------------------------------------------
@@ -191,6 +237,14 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #ifndef PFC_ROW_E
 #define PFC_ROW_E 4
 #endif
+#ifndef PFC_COL_T_MUL
+#define PFC_COL_T_MUL 1
+#endif
+// Exchange buffers of 4-field transforms as 16-byte chunk arrays; successive
+// buffers start PFC_LDS_MOD 16-byte bank groups apart (0: plain AoS buffers).
+#ifndef PFC_LDS_MOD
+#define PFC_LDS_MOD 2
+#endif
 constexpr int col_e(bool f64, int LL) {
     return LL % 3 == 0 ? 12 : (f64 ? (LL >= 1024 ? 8 : 4) : (LL == 512 ? 8 : 4));
 }
@@ -210,7 +264,10 @@ struct RowCfg {
     static constexpr int NT = RPW * G;
     static constexpr int K = H;  // row spectrum with X[H] packed into Im X[0]
     static constexpr int PB = padded(H);                       // elements per row buffer
-    static constexpr size_t LDS = (size_t)RPW * PB * sizeof(Vec<R, 4>);
+    static constexpr int CH = PFC_LDS_MOD ? (int)(sizeof(Vec<R, 4>) / 16) : 1;
+    static constexpr int PS = PFC_LDS_MOD ? soa_stride(H, CH, PFC_LDS_MOD) : PB;
+    static constexpr size_t RB = (size_t)CH * PS * (sizeof(Vec<R, 4>) / CH);  // bytes per row buffer
+    static constexpr size_t LDS = (size_t)RPW * RB;
 };
 
 // Column pass: T columns of length LL per workgroup, G = LL/E threads per
@@ -222,11 +279,17 @@ struct ColCfg {
     static constexpr int E = col_e(F64, LL);
     static constexpr int G = LL / E;
     static constexpr int T0 = col_t(F64, LL);
-    static constexpr int T = T0 / PFC_COL_T_DIV > 0 ? T0 / PFC_COL_T_DIV : 1;
+    static constexpr int T1 = T0 / PFC_COL_T_DIV > 0 ? T0 / PFC_COL_T_DIV : 1;
+    // Wider tiles (PFC_COL_T_MUL) only while the exchange buffers fit in LDS.
+    static constexpr int T = (size_t)T1 * PFC_COL_T_MUL * padded(LL) * sizeof(Vec<R, 4>) <= 160 * 1024
+                                 ? T1 * PFC_COL_T_MUL : T1;
     static constexpr int NT = T * G;
     static constexpr int PB = padded(LL);
+    static constexpr int CH = PFC_LDS_MOD ? (int)(sizeof(Vec<R, 4>) / 16) : 1;
+    static constexpr int PS = PFC_LDS_MOD ? soa_stride(LL, CH, PFC_LDS_MOD) : PB;
+    static constexpr size_t CB = (size_t)CH * PS * (sizeof(Vec<R, 4>) / CH);  // bytes per column buffer
     // T column buffers; at least two (the packed column's noise exchange uses the second).
-    static constexpr size_t LDS = (size_t)(T < 2 ? 2 : T) * PB * sizeof(Vec<R, 4>);
+    static constexpr size_t LDS = (size_t)(T < 2 ? 2 : T) * CB;
 };
 
 // ---------------------------------------------------------------------------
@@ -261,7 +324,7 @@ __device__ __forceinline__ Cx<R> r2c_merge(Cx<R> wk, Cx<R> wc, Cx<R> tw) {
 template <typename R, int H, int E, int NC>
 __device__ __forceinline__ void row_r2c_store(Vec<R, NC> (&w)[E], Vec<R, NC>* lds, int t,
                                               const Cx<R>* __restrict__ twH, const Cx<R>* __restrict__ twM,
-                                              Vec<R, NC>* __restrict__ dst) {
+                                              Buf dst, uint32_t doff) {
     constexpr int G = H / E;
     fft_reg<R, NC, H, E, -1>(w, lds, t, twH);
     __syncthreads();
@@ -282,7 +345,7 @@ __device__ __forceinline__ void row_r2c_store(Vec<R, NC> (&w)[E], Vec<R, NC>* ld
 #pragma unroll
             for (int f = 0; f < NC; f++) X.c[f] = r2c_merge(w[q].c[f], wc.c[f], tw);
         }
-        dst[k] = X;
+        bstore(X, dst, (doff + k) * (uint32_t)sizeof(X), 0u);
     }
 }
 
@@ -303,11 +366,14 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
     __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     __shared__ double red[(C::NT + 63) / 64 + 1];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
-    const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
+    const int inst = blockIdx.y;
     if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
-    const size_t rbase = ((size_t)inst * b.L * b.N + row) * K;
-    const Vec<R, 2>* S = (const Vec<R, 2>*)b.S + rbase;
-    const Vec<R, 2>* AB = (const Vec<R, 2>*)b.AB + rbase;
+    // Rows of this workgroup from a uniform base; this thread's row at roff.
+    const size_t bbase = ((size_t)inst * b.L * b.N + (size_t)blockIdx.x * C::RPW) * K;
+    const uint32_t roff = (uint32_t)(rl * K);
+    constexpr size_t rows_bytes = (size_t)C::RPW * K * sizeof(Vec<R, 2>);
+    const Buf S = mkbuf((const Vec<R, 2>*)b.S + bbase, rows_bytes);
+    const Buf AB = mkbuf((const Vec<R, 2>*)b.AB + bbase, rows_bytes);
     const Cx<R>* twH = (const Cx<R>*)b.twH;
     const Cx<R>* twM = (const Cx<R>*)b.twM;
 
@@ -317,14 +383,16 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
     for (int q = 0; q < E; q++) {
         const int k = t + G * q;
         const int kc = (H - k) % H;
-        const Vec<R, 2> s0 = S[k], s1 = S[kc], a0 = AB[k], a1 = AB[kc];
+        constexpr uint32_t es = sizeof(Vec<R, 2>);
+        const Vec<R, 2> s0 = bload<Vec<R, 2>>(S, (roff + k) * es, 0u), s1 = bload<Vec<R, 2>>(S, (roff + kc) * es, 0u);
+        const Vec<R, 2> a0 = bload<Vec<R, 2>>(AB, (roff + k) * es, 0u), a1 = bload<Vec<R, 2>>(AB, (roff + kc) * es, 0u);
         const Cx<R> w = twM[k];
         z[q].c[0] = c2r_split_packed(s0.c[0], s1.c[0], k, w);
         z[q].c[1] = c2r_split_packed(s0.c[1], s1.c[1], k, w);
         z[q].c[2] = c2r_split_packed(a0.c[0], a1.c[0], k, w);
         z[q].c[3] = c2r_split_packed(a0.c[1], a1.c[1], k, w);
     }
-    fft_reg<R, 4, H, E, +1>(z, (Vec<R, 4>*)smem + rl * C::PB, t, twH);
+    fft_reg<R, 4, H, E, +1>(z, LdsBuf<R, 4, C::CH, C::PS>{smem + rl * C::RB}, t, twH);
 
     const double* mu = b.means + 4 * inst;
     const R mph = (R)mu[0], mco = (R)mu[1], ma = (R)mu[2], mb = (R)mu[3];
@@ -348,7 +416,8 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
         w[q].c[1] = Cx<R>{inc[0], inc[1]};
     }
     if (flags & ROW_WRITE_IN)
-        row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)smem + rl * C::PB, t, twH, twM, (Vec<R, 2>*)b.IN + rbase);
+        row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)(smem + rl * C::RB), t, twH, twM,
+                                  mkbuf((Vec<R, 2>*)b.IN + bbase, rows_bytes), roff);
     if (want_e) {
         const double se = block_sum<C::NT>(esum, red);
 
@@ -380,7 +449,7 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_in(DevBufs b, const 
         w[q].c[1] = Cx<R>{incon[pbase + 2 * j * ps], incon[pbase + (2 * j + 1) * ps]};
     }
     row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
-                           (Vec<R, 2>*)b.S + rbase);
+                           mkbuf((Vec<R, 2>*)b.S + rbase, K * sizeof(Vec<R, 2>)), 0u);
 }
 
 // One real plane per instance -> its packed row spectrum (the noise field).
@@ -404,7 +473,7 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_r2c1(DevBufs b, cons
         w[q].c[0] = Cx<R>{in[pbase + 2 * j * ps], in[pbase + (2 * j + 1) * ps]};
     }
     row_r2c_store<R, H, E, 1>(w, (Vec<R, 1>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
-                           out + rbase);
+                           mkbuf(out + rbase, K * sizeof(Vec<R, 1>)), 0u);
 }
 
 // S -> real planes (psi, con) including the means.
@@ -519,10 +588,14 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     const int tile0 = blockIdx.x * T;        // first column of the tile; T divides K
     const int j = tile0 / K;                 // n slot of the whole tile
     const int kc = tile0 - j * K + col;      // packed m column (kc = 0 holds k_m = 0 and M/2)
-    const size_t cbase = (size_t)inst * LL * CS + (size_t)j * K + kc;  // element i at cbase + i*CS
-    Vec<R, 2>* S = (Vec<R, 2>*)b.S;
+    // Element i of this column: uniform tile base + 32-bit offset col + i*CS.
+    const size_t tbase = (size_t)inst * LL * CS + tile0;
+    constexpr uint32_t es = sizeof(Vec<R, 2>);
+    const uint32_t toff = (uint32_t)(col + t * CS) * es;     // bytes
+    const size_t tbytes = ((size_t)LL * CS - tile0) * es;    // rest of the instance
+    const Buf S = mkbuf((const Vec<R, 2>*)b.S + tbase, tbytes);
     const Cx<R>* twL = (const Cx<R>*)b.twL;
-    Vec<R, 4>* lds = (Vec<R, 4>*)smem + col * C::PB;
+    const LdsBuf<R, 4, C::CH, C::PS> lds{smem + col * C::CB};
     const int Nh = b.N / 2;
     // Columns kc = 0 of slots j = 0 and N/2 carry two Hermitian-in-l sequences
     // (k_m = 0 and M/2); uniform per workgroup.
@@ -541,12 +614,12 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     Vec<R, 4> x[E];
 #pragma unroll
     for (int q = 0; q < E; q++) {
-        const size_t e = cbase + (size_t)(t + G * q) * CS;
-        const Vec<R, 2> sv = S[e];
+        const uint32_t qo = (uint32_t)(G * q * CS) * es;  // uniform
+        const Vec<R, 2> sv = bload<Vec<R, 2>>(S, toff, qo);
         x[q].c[0] = sv.c[0];
         x[q].c[1] = sv.c[1];
         if (step) {
-            const Vec<R, 2> iv = ((const Vec<R, 2>*)b.IN)[e];
+            const Vec<R, 2> iv = bload<Vec<R, 2>>(mkbuf((const Vec<R, 2>*)b.IN + tbase, tbytes), toff, qo);
             x[q].c[2] = iv.c[0];
             x[q].c[3] = iv.c[1];
         } else {
@@ -560,20 +633,25 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     if (step && b.NZ) {
         Vec<R, 1> n1[E];
 #pragma unroll
-        for (int q = 0; q < E; q++) n1[q] = ((const Vec<R, 1>*)b.NZ)[cbase + (size_t)(t + G * q) * CS];
-        fft_reg<R, 1, LL, E, -1>(n1, (Vec<R, 1>*)smem + col * C::PB, t, twL);
+        for (int q = 0; q < E; q++)
+            n1[q] = bload<Vec<R, 1>>(mkbuf((const Vec<R, 1>*)b.NZ + tbase, tbytes / 2), toff / 2,
+                                     (uint32_t)(G * q * CS) * (es / 2));
+        fft_reg<R, 1, LL, E, -1>(n1, (Vec<R, 1>*)(smem + col * C::CB), t, twL);
 #pragma unroll
         for (int q = 0; q < E; q++) nz[q] = n1[q].c[0];
     }
-    // The packed column needs each mode's mirror -k: exchange it through LDS.
-    Cx<R>* nzx = (Cx<R>*)((Vec<R, 4>*)smem + C::PB);
+    // The packed column needs each mode's mirror -k: exchange it through LDS
+    // (column 0's buffer, and column 1's for the noise).
+    Vec<R, 4>* mx4 = (Vec<R, 4>*)smem;
+    Cx<R>* nzx = (Cx<R>*)(mx4 + C::PB);
     if (packed_tile) {
         __syncthreads();
         if (packed) {
 #pragma unroll
             for (int q = 0; q < E; q++) {
-                lds[pidx(t + G * q)] = x[q];
-                if (step && b.NZ) nzx[pidx(t + G * q)] = nz[q];
+                const int i = pidx(t + G * q);
+                mx4[i] = x[q];
+                if (step && b.NZ) nzx[i] = nz[q];
             }
         }
         __syncthreads();
@@ -581,7 +659,7 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
 
     const bool want_e = (flags & COL_ENERGY) != 0;
     double g = 0.0;
-    Vec<R, 4> y[E];  // {psi', con', C2A psi', C2B psi'}
+    // x[q] becomes {psi', con', C2A psi', C2B psi'} in place.
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int i = t + G * q;
@@ -589,9 +667,9 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
         if (!packed) {
             col_mode(b, p, mode, want_e, LL, i, kmi, kni, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
         } else {
-            const int im = (LL - i) % LL;
-            const Vec<R, 4> xm = lds[pidx(im)];
-            const Cx<R> nzm = (step && b.NZ) ? nzx[pidx(im)] : Cx<R>{R(0), R(0)};
+            const int im = pidx((LL - i) % LL);
+            const Vec<R, 4> xm = mx4[im];
+            const Cx<R> nzm = (step && b.NZ) ? nzx[im] : Cx<R>{R(0), R(0)};
             Cx<R> ph0, phh, co0, coh, on0, onh, oc0, och, nz0, nzh, a0, ah, b0, bh;
             hsplit(x[q].c[0], xm.c[0], ph0, phh);
             hsplit(x[q].c[1], xm.c[1], co0, coh);
@@ -605,10 +683,10 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
             a = hjoin(a0, ah);
             bb = hjoin(b0, bh);
         }
-        y[q].c[0] = ph;
-        y[q].c[1] = co;
-        y[q].c[2] = a;
-        y[q].c[3] = bb;
+        x[q].c[0] = ph;
+        x[q].c[1] = co;
+        x[q].c[2] = a;
+        x[q].c[3] = bb;
     }
     if (step && b.tcount && blockIdx.x == 0 && tid == 0) b.tcount[inst] += 1;
     if (want_e) {
@@ -618,19 +696,19 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     if (mode == COL_ENERGY_ONLY) return;
 
     // Inverse transform of the new state and of C2A/C2B times the new psi.
-    fft_reg<R, 4, LL, E, +1>(y, lds, t, twL);
+    fft_reg<R, 4, LL, E, +1>(x, lds, t, twL);
     const R invL = R(1) / R(LL);
-    Vec<R, 2>* AB = (Vec<R, 2>*)b.AB;
+    const Buf AB = mkbuf((const Vec<R, 2>*)b.AB + tbase, tbytes);
 #pragma unroll
     for (int q = 0; q < E; q++) {
-        const size_t e = cbase + (size_t)(t + G * q) * CS;
+        const uint32_t qo = (uint32_t)(G * q * CS) * es;
         Vec<R, 2> sv, av;
-        sv.c[0] = Cx<R>{y[q].c[0].x * invL, y[q].c[0].y * invL};
-        sv.c[1] = Cx<R>{y[q].c[1].x * invL, y[q].c[1].y * invL};
-        av.c[0] = Cx<R>{y[q].c[2].x * invL, y[q].c[2].y * invL};
-        av.c[1] = Cx<R>{y[q].c[3].x * invL, y[q].c[3].y * invL};
-        S[e] = sv;
-        AB[e] = av;
+        sv.c[0] = Cx<R>{x[q].c[0].x * invL, x[q].c[0].y * invL};
+        sv.c[1] = Cx<R>{x[q].c[1].x * invL, x[q].c[1].y * invL};
+        av.c[0] = Cx<R>{x[q].c[2].x * invL, x[q].c[2].y * invL};
+        av.c[1] = Cx<R>{x[q].c[3].x * invL, x[q].c[3].y * invL};
+        bstore(sv, S, toff, qo);
+        bstore(av, AB, toff, qo);
     }
 }
 
@@ -666,23 +744,27 @@ __global__ void __launch_bounds__((DepCfg<R, NN>::NT)) k_depth(DevBufs b, const 
     const int tpr = K / T;  // tiles per l row; T divides K (checked at launch)
     const int l = blockIdx.x / tpr;
     const int kc0 = (blockIdx.x - l * tpr) * T;
-    const size_t base = ((size_t)inst * b.L + l) * NN * K + kc0 + col;  // slot n at base + n*K
+    const size_t base = ((size_t)inst * b.L + l) * NN * K + kc0;  // slot n of column col at base + col + n*K
+    const uint32_t toff = (uint32_t)(col + t * K);            // elements
+    const size_t nbytes_el = (size_t)NN * K - kc0;            // elements from base to the end of this l
     Vec<R, NC>* lds = (Vec<R, NC>*)smem + col * C::PB;
     const bool split_col = kc0 == 0 && col == 0;
 
     Vec<R, NC> x[E];
 #pragma unroll
     for (int q = 0; q < E; q++) {
-        const size_t e = base + (size_t)(t + G * q) * K;
+        const uint32_t e = (uint32_t)(G * q * K);  // uniform, elements
         if constexpr (NC == 1) {
-            x[q].c[0] = ((const Cx<R>*)in0)[e];
+            constexpr uint32_t es = sizeof(Cx<R>);
+            x[q].c[0] = bload<Cx<R>>(mkbuf((const Cx<R>*)in0 + base, nbytes_el * es), toff * es, e * es);
         } else {
-            const Vec<R, 2> v = ((const Vec<R, 2>*)in0)[e];
+            constexpr uint32_t es = sizeof(Vec<R, 2>);
+            const Vec<R, 2> v = bload<Vec<R, 2>>(mkbuf((const Vec<R, 2>*)in0 + base, nbytes_el * es), toff * es, e * es);
             x[q].c[0] = v.c[0];
             x[q].c[1] = v.c[1];
             if constexpr (NC == 4) {
                 Vec<R, 2> w;
-                if (in1) w = ((const Vec<R, 2>*)in1)[e];
+                if (in1) w = bload<Vec<R, 2>>(mkbuf((const Vec<R, 2>*)in1 + base, nbytes_el * es), toff * es, e * es);
                 else w.c[0] = w.c[1] = Cx<R>{R(0), R(0)};
                 x[q].c[2] = w.c[0];
                 x[q].c[3] = w.c[1];
@@ -751,24 +833,26 @@ __global__ void __launch_bounds__((DepCfg<R, NN>::NT)) k_depth(DevBufs b, const 
     const R sc = DIR > 0 ? R(1) / R(NN) : R(1);
 #pragma unroll
     for (int q = 0; q < E; q++) {
-        const size_t e = base + (size_t)(t + G * q) * K;
+        const uint32_t e = (uint32_t)(G * q * K);
         if constexpr (DIR > 0) {
 #pragma unroll
             for (int f = 0; f < NC; f++) x[q].c[f] = Cx<R>{x[q].c[f].x * sc, x[q].c[f].y * sc};
         }
         if constexpr (NC == 1) {
-            ((Cx<R>*)out0)[e] = x[q].c[0];
+            constexpr uint32_t es = sizeof(Cx<R>);
+            bstore(x[q].c[0], mkbuf((Cx<R>*)out0 + base, nbytes_el * es), toff * es, e * es);
         } else {
             Vec<R, 2> v;
             v.c[0] = x[q].c[0];
             v.c[1] = x[q].c[1];
-            ((Vec<R, 2>*)out0)[e] = v;
+            constexpr uint32_t es = sizeof(Vec<R, 2>);
+            bstore(v, mkbuf((Vec<R, 2>*)out0 + base, nbytes_el * es), toff * es, e * es);
             if constexpr (NC == 4) {
                 if (out1) {
                     Vec<R, 2> w;
                     w.c[0] = x[q].c[2];
                     w.c[1] = x[q].c[3];
-                    ((Vec<R, 2>*)out1)[e] = w;
+                    bstore(w, mkbuf((Vec<R, 2>*)out1 + base, nbytes_el * es), toff * es, e * es);
                 }
             }
         }

@@ -112,3 +112,29 @@ def test_fp32_cli_runs(tmp_path):
     assert (g / "Phimax_30.txt").exists() and (g / "phi_30_0.vti").exists()
     e = np.loadtxt(g / "energy.txt")
     assert e.shape == (2, 3) and np.all(np.isfinite(e))
+
+
+@pytest.mark.parametrize("model,extra", [
+    ("misfit", ["--grainx", "2", "--grainy", "2"]),
+    ("misfit", ["--grainx", "2", "--grainy", "2", "--axx", "0.001"]),
+    ("cvd", []),
+])
+def test_3d_cli_matches_oracle(tmp_path, model, extra):
+    """3-D grids (N > 1) and the noise term through the process boundary."""
+    args = ["--steps", "12", "--mod", "6", "--L", "32", "--M", "32", "--N", "16", "--u0", "0.05",
+            "--con0", "0.25"] + extra
+    g, o = tmp_path / "gpu", tmp_path / "cpu"
+    run(CLI, ["--model", model] + args + ["--outdir", str(g)], tmp_path)
+    run(OCLI, ["--model", model] + args + ["--outdir", str(o)], tmp_path)
+    assert sorted(os.listdir(g)) == sorted(os.listdir(o))
+    assert (g / "run_config.txt").read_bytes() == (o / "run_config.txt").read_bytes()
+    kd = 12 if model == "misfit" else 6  # misfit kd = t+1, cvd kd = t (t % mod == 0)
+    for fld in ("phi", "con"):
+        hg, pg, tg, ng = vti_split(g / f"{fld}_{kd}_0.vti")
+        ho, po, to, no = vti_split(o / f"{fld}_{kd}_0.vti")
+        assert hg == ho and tg == to and ng == no == 4 * 32 * 32 * 16 + 4  # int32 word = payload + 4
+        assert np.abs(pg - po).max() < 1e-5
+    eg = np.loadtxt(g / "energy.txt")
+    eo = np.loadtxt(o / "energy.txt")
+    assert np.array_equal(eg[:, 0], eo[:, 0])
+    assert np.allclose(eg[:, 1:], eo[:, 1:], rtol=2e-5, atol=1e-9)
