@@ -31,50 +31,20 @@
 
 namespace pfc {
 
-// Buffer access to workgroup-uniform arrays: the descriptor (base, byte
-// range) lives in SGPRs, the per-thread byte offset in one VGPR and the
-// uniform part of the offset (per unrolled element) in an SGPR, so no 64-bit
-// address arithmetic is done per element.  Element sizes 8 or 16*k bytes.
-typedef __amdgpu_buffer_rsrc_t Buf;
-__device__ __forceinline__ Buf mkbuf(const void* base, size_t bytes) {
-    const uint32_t n = bytes > 0xffffffffull ? 0xffffffffu : (uint32_t)bytes;
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)n, 0x00020000);
+// Accesses to workgroup-uniform arrays: a uniform base pointer plus a 32-bit
+// per-thread byte offset and a 32-bit uniform byte offset (per unrolled
+// element), so no 64-bit index arithmetic is done per element.
+struct Buf {
+    char* p;
+};
+__device__ __forceinline__ Buf mkbuf(const void* base, size_t) { return Buf{(char*)const_cast<void*>(base)}; }
+template <typename T>
+__device__ __forceinline__ T bload(const Buf& b, uint32_t voff, uint32_t soff) {
+    return *(const T*)(b.p + soff + voff);
 }
 template <typename T>
-__device__ __forceinline__ T bload(Buf r, uint32_t voff, uint32_t soff) {
-    T out;
-    if constexpr (sizeof(T) == 8) {
-        typedef float f2 __attribute__((ext_vector_type(2)));
-        const f2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-        __builtin_memcpy(&out, &v, 8);
-    } else {
-        static_assert(sizeof(T) % 16 == 0, "element size");
-        typedef float f4 __attribute__((ext_vector_type(4)));
-#pragma unroll
-        for (int c = 0; c < (int)(sizeof(T) / 16); c++) {
-            const f4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * c, soff, 0);
-            __builtin_memcpy((char*)&out + 16 * c, &v, 16);
-        }
-    }
-    return out;
-}
-template <typename T>
-__device__ __forceinline__ void bstore(const T& val, Buf r, uint32_t voff, uint32_t soff) {
-    if constexpr (sizeof(T) == 8) {
-        typedef float f2 __attribute__((ext_vector_type(2)));
-        f2 v;
-        __builtin_memcpy(&v, &val, 8);
-        __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
-    } else {
-        static_assert(sizeof(T) % 16 == 0, "element size");
-        typedef float f4 __attribute__((ext_vector_type(4)));
-#pragma unroll
-        for (int c = 0; c < (int)(sizeof(T) / 16); c++) {
-            f4 v;
-            __builtin_memcpy(&v, (const char*)&val + 16 * c, 16);
-            __builtin_amdgcn_raw_buffer_store_b128(v, r, voff + 16 * c, soff, 0);
-        }
-    }
+__device__ __forceinline__ void bstore(const T& val, const Buf& b, uint32_t voff, uint32_t soff) {
+    *(T*)(b.p + soff + voff) = val;
 }
 
 // ---------------------------------------------------------------------------

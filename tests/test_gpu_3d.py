@@ -142,3 +142,45 @@ def test_step_mask(hip, N):
         x = s1.get_fields()
         y = b.get_fields(i)
         assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), f"instance {i}"
+
+
+def test_concurrent_contexts_bit_identical(hip):
+    """Contexts stepping at the same time on their own streams (a 3-D FP64
+    batch beside an FP32 batch with the noise term) give the results each
+    gives alone: nothing is shared between contexts."""
+    pa = hip.default_params(model=0, L=32, M=32, N=16, precision=hip.FP64, grainx=2, grainy=2, seed=9)
+    pb = hip.default_params(model=0, L=64, M=64, precision=hip.FP32, grainx=4, grainy=4, seed=77, axx=0.001)
+
+    def make(p, B):
+        s = hip.Sim(p, batch=B)
+        phi, con = hip.init_fields(p)
+        for i in range(B):
+            s.set_fields(phi, con, inst=i)
+        s.set_seed(p.seed)
+        return s
+
+    ref = {}
+    for key, p in (("a", pa), ("b", pb)):
+        s = make(p, 1)
+        s.step(30)
+        ref[key] = s.get_fields()
+        s.close()
+    for graph in (8, 0):
+        for rep in range(3):
+            a, b = make(pa, 4), make(pb, 4)
+            for i in range(1, 4):
+                a.set_active(i, False)
+                b.set_active(i, False)
+            if graph:
+                a.set_graph(graph)
+                b.set_graph(graph)
+            for _ in range(30 // 6):
+                a.step(6)
+                b.step(6)
+            a.sync()
+            b.sync()
+            for key, s in (("a", a), ("b", b)):
+                x = s.get_fields(0)
+                assert np.array_equal(x[0], ref[key][0]) and np.array_equal(x[1], ref[key][1]), (key, graph, rep)
+            a.close()
+            b.close()

@@ -58,7 +58,9 @@ def test_mix_equals_alone(hip):
         assert st.steps_done == steps and st.t_submit <= st.t_start <= st.t_end
         phi, con = srv.result(jid)
         (aphi, acon), e = _alone(hip, p, steps)
-        assert np.array_equal(phi, aphi) and np.array_equal(con, acon), f"job {jid}"
+        d = max(np.abs(phi - aphi).max(), np.abs(con - acon).max())
+        print(f"job {jid} L={p.L} M={p.M} N={p.N} prec={p.precision} steps={steps} max|mix-alone|={d:.3e}", flush=True)
+        assert np.array_equal(phi, aphi) and np.array_equal(con, acon), f"job {jid}: {d:.3e}"
         assert st.energy == e[0] and st.density == e[1]
     srv.close()
 
