@@ -144,8 +144,8 @@ const char* pfc_last_error(void);
 
 /* ------------------------------------------------------------------------
  * Request loop with continuous batching.  The reference runs one job per
- * pfc-exp-cli process, queued one at a time by the GUI (MainWindow.cpp:
- * 2920-2960, misfit.cpp:311-590); these entry points have no counterpart
+ * pfc-exp-cli process, queued one at a time by the GUI (MainWindow.cpp:2862-
+ * 2976,3453-3521, misfit.cpp:311-590); these entry points have no counterpart
  * there.  A job is one run (what one CLI invocation computes); jobs of one
  * class (model, precision, grid, sig/dt/dx/axx, narrowing) share a batch
  * context, are admitted into free slots as they arrive while the others keep

@@ -1,11 +1,12 @@
 // pfc_serve.cpp - continuous-batching request loop (pfc_server_* in pfc_hip.h).
 //
 // The reference runs one simulation per pfc-exp-cli process and the GUI
-// queues jobs one at a time (MainWindow.cpp:2920-2960).  Here a job is a
-// slot in a batch context: every job class (model, precision, grid, and the
-// parameters that enter the kernels) owns one context of up to max_slots
-// instances, created when the class first arrives and sized against the
-// HBM budget.  A scheduler thread admits queued jobs into free slots while
+// queues jobs one at a time (MainWindow.cpp:2862-2976,3453-3521).  Here a
+// job is a slot in a batch context: jobs of one class (model, precision,
+// grid, and the parameters that enter the kernels) share contexts of up to
+// max_slots instances, created when no slot of the class is free and sized
+// against the HBM budget.  A scheduler thread admits queued jobs into free
+// slots while
 // the others keep stepping (idle slots are masked with pfc_set_active), steps
 // every busy context concurrently (one HIP stream each) to the next job
 // event, and hands checkpoints to a writer thread that writes the files the
