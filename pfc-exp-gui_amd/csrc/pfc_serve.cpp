@@ -6,12 +6,12 @@
 // grid, and the parameters that enter the kernels) share contexts of up to
 // max_slots instances, created when no slot of the class is free and sized
 // against the HBM budget.  A scheduler thread admits queued jobs into free
-// slots while
-// the others keep stepping (idle slots are masked with pfc_set_active), steps
-// every busy context concurrently (one HIP stream each) to the next job
-// event, and hands checkpoints to a writer thread that writes the files the
-// CLI would write (run_config.txt, energy.txt, phi/con VTI + PVTI,
-// Phimax_<kd>.txt, checkpoint_timestamps.txt) while stepping continues.
+// slots while the others keep stepping (idle slots are masked with
+// pfc_set_active), steps every busy context concurrently (one HIP stream
+// each) to the next job event, and hands checkpoints to a writer thread that
+// writes the files the CLI would write (run_config.txt, energy.txt, phi/con
+// VTI + PVTI, Phimax_<kd>.txt, checkpoint_timestamps.txt) while stepping
+// continues.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
