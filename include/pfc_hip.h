@@ -116,6 +116,27 @@ int pfc_phimax_elastic(int L, int M, int N, float dx, const float* phi, double* 
 int pfc_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
                const double* z, double* strain, unsigned int* nb);
 
+/* cryst_order (elastic.cpp:3013-3389): per-atom bond-order parameters of the
+ * atoms from pfc_phimax_elastic, neighbours within 1.2 (minimum image), at
+ * most 16 stored.  q4 and q6 are the reference's two columns (both from
+ * l = 6; q6 accumulated onto q4's sum); nb (may be NULL) the neighbour count. */
+int pfc_cryst_order(int L, int M, int N, float dx, int n, const double* x, const double* y, const double* z,
+                    double* q4, double* q6, unsigned int* nb);
+
+/* GR (cvd.cpp:1277-1312): pair histogram of the n CVD atom fractions that
+ * pfc_phimax_cvd returns (2 per pair, bins of a/25, a = 1, pairs closer than
+ * a/sqrt(2)/2 left out).  *maxbin = int(L*dx/delr); hist (may be NULL, then
+ * only *maxbin is set) holds *maxbin counts.  The reference calls it on every
+ * CVD checkpoint (cvd.cpp:1253). */
+int pfc_cvd_gr(int L, int M, int N, float dx, int n, const float* pc_x, const float* pc_y, const float* pc_z,
+               unsigned int* hist, int* maxbin);
+/* Q6_GLO (cvd.cpp:1347-1510): per-entry neighbour count and local Q6
+ * (neighbours at a*sqrt(3)/2*[0.9,1.2], at most 24 used), the global Q6 and
+ * the 30-bin histogram (width 0.05) of the local values.  The reference calls
+ * it on CVD checkpoints with kd % 20 == 0 (cvd.cpp:1254). */
+int pfc_cvd_q6(int L, int M, int N, float dx, int n, const float* pc_x, const float* pc_y, const float* pc_z,
+               unsigned int* nb, float* ql, float* qglobal, unsigned long long* histq);
+
 /* Capture steps in HIP graphs of `chunk` steps (0 disables). */
 int pfc_set_graph(pfc_ctx* ctx, int chunk);
 

@@ -76,6 +76,21 @@ def load():
         lib.pfo_phimax_elastic.argtypes = [ctypes.c_int] * 3 + [ctypes.c_float, fp, ctypes.POINTER(Atoms)]
         lib.pfo_strain.argtypes = [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_float, ctypes.c_int, dp, dp, dp, dp,
                                                          ctypes.POINTER(ctypes.c_uint)]
+        lib.pfo_cryst_order.argtypes = [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int, dp, dp, dp, dp, dp,
+                                                            ctypes.POINTER(ctypes.c_uint)]
+        lib.pfo_cryst_order.restype = None
+        fpp = ctypes.POINTER(ctypes.c_float)
+        lib.pfo_cvd_gr_bins.argtypes = [ctypes.c_int, ctypes.c_float]
+        lib.pfo_cvd_gr.argtypes = [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int, fpp, fpp, fpp,
+                                                       ctypes.POINTER(ctypes.c_uint)]
+        lib.pfo_cvd_gr.restype = None
+        lib.pfo_cvd_q6.argtypes = [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int, fpp, fpp, fpp,
+                                                       ctypes.POINTER(ctypes.c_uint), fpp, fpp,
+                                                       ctypes.POINTER(ctypes.c_ulonglong)]
+        lib.pfo_cvd_q6.restype = None
+        for fn in (lib.pfo_ylm_r, lib.pfo_ylm_c):
+            fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float]
+            fn.restype = ctypes.c_float
         lib.pfo_fft3d.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.pfo_fft1d.argtypes = [dp, ctypes.c_int, ctypes.c_int]
         lib.pfo_ktables.argtypes = [ctypes.POINTER(Model), fp, fp, fp]
@@ -193,6 +208,55 @@ def strain(L, M, N, dx, benchel, x, y, z):
                    _dp(np.ascontiguousarray(y)), _dp(np.ascontiguousarray(z)), _dp(s),
                    nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)))
     return s, nb
+
+
+def cryst_order(L, M, N, dx, x, y, z):
+    """Per-atom (q4, q6, neighbour count) of cryst_order."""
+    lib = load()
+    n = len(x)
+    q4 = np.zeros(n)
+    q6 = np.zeros(n)
+    nb = np.zeros(n, np.uint32)
+    lib.pfo_cryst_order(L, M, N, ctypes.c_float(dx), n, _dp(np.ascontiguousarray(x, np.float64)),
+                        _dp(np.ascontiguousarray(y, np.float64)), _dp(np.ascontiguousarray(z, np.float64)),
+                        _dp(q4), _dp(q6), nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)))
+    return q4, q6, nb
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def cvd_gr(L, M, N, dx, x, y, z):
+    """GR pair histogram of the CVD atom fractions (2 per pair)."""
+    lib = load()
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    hist = np.zeros(lib.pfo_cvd_gr_bins(L, ctypes.c_float(dx)), np.uint32)
+    lib.pfo_cvd_gr(L, M, N, ctypes.c_float(dx), x.size, _fp(x), _fp(y), _fp(z),
+                   hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)))
+    return hist
+
+
+def cvd_q6(L, M, N, dx, x, y, z):
+    """Q6_GLO of the CVD atom fractions: (neighbour counts, local Q6,
+    global Q6, 30-bin histogram of the local values)."""
+    lib = load()
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    n = x.size
+    nb = np.zeros(n, np.uint32)
+    ql = np.zeros(n, np.float32)
+    qg = np.zeros(1, np.float32)
+    hq = np.zeros(30, np.uint64)
+    lib.pfo_cvd_q6(L, M, N, ctypes.c_float(dx), n, _fp(x), _fp(y), _fp(z),
+                   nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), _fp(ql), _fp(qg),
+                   hq.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong)))
+    return nb, ql, float(qg[0]), hq
+
+
+def ylm(l, m, theta, fai):
+    """(real, imaginary) spherical harmonic as the reference evaluates it (float)."""
+    lib = load()
+    return lib.pfo_ylm_r(l, m, theta, fai), lib.pfo_ylm_c(l, m, theta, fai)
 
 
 def fft3d(z, L, M, N, sign):

@@ -935,6 +935,364 @@ void pfo_strain(int L, int M, int N, float dx, float benchel, int n, const doubl
 }
 
 /* ------------------------------------------------------------------------ */
+/* cryst_order (elastic.cpp:3013-3389) with the spherical harmonics of        */
+/* pfc_common.h:274-346.  Float/double types as the reference: the harmonics  */
+/* are float, angles and the averages double, pi the double 3.1415926.        */
+/* Quirks kept: both "q4" and "q6" loops use l = 6 (:3155,3237) and the second */
+/* loop keeps adding onto the first one's per-atom sum (qlmb_m is not reset), */
+/* so q6 = sqrt(xishu * (S + S)) summed term by term; atoms with no or more   */
+/* than CNmax neighbours get q = 0 but still count as neighbours of others.   */
+/* ------------------------------------------------------------------------ */
+static unsigned long pfo_facto(int h) { /* pfc_common.h:274-278 (int product) */
+    int g = 1;
+    for (int i = 1; i <= h; i++) g = g * i;
+    return (unsigned long)g;
+}
+
+static float pfo_lg_poly(int l, float x) { /* pfc_common.h:280-291 */
+    float plm2 = 1, plm1 = x, pl = 0;
+    if (l == 0) return 1;
+    if (l == 1) return x;
+    for (int i = 2; i <= l; i++) {
+        pl = ((2 * i - 1) * x * (float)plm1 - (i - 1) * plm2) / (float)i;
+        plm2 = plm1;
+        plm1 = pl;
+    }
+    return pl;
+}
+
+static float pfo_plgndr(int l, int m, float x) { /* pfc_common.h:293-320 */
+    float fact, pll = 0, pmm, pmmp1, somx2;
+    if (m < 0) m = -m;
+    if (m == 0) return pfo_lg_poly(l, x);
+    pmm = 1.0f;
+    if (m > 0) {
+        somx2 = (float)sqrt((1.0 - x) * (1.0 + x));
+        fact = 1.0f;
+        for (int i = 1; i <= m; i++) {
+            pmm *= -fact * somx2;
+            fact = (float)(fact + 2.0);
+        }
+    }
+    if (l == m) return pmm;
+    pmmp1 = x * (2 * m + 1) * pmm;
+    if (l == m + 1) return pmmp1;
+    for (int ll = m + 2; ll <= l; ll++) {
+        pll = (x * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pmm) / (ll - m);
+        pmm = pmmp1;
+        pmmp1 = pll;
+    }
+    return pll;
+}
+
+static float pfo_ylm_norm(int l, int m, double sgn) {
+    return (float)(sgn * sqrt((2 * l + 1) / 4. / PFO_PI) *
+                   (double)sqrtf((float)pfo_facto(l - m) / (float)pfo_facto(l + m)));
+}
+
+float pfo_ylm_r(int l, int m, float theta, float fai) { /* pfc_common.h:322-328 */
+    if (m < 0) m = -m;
+    const float fnorm = pfo_ylm_norm(l, m, pow(-1.0, m));
+    const float plm = pfo_plgndr(l, m, cosf(theta));
+    const float comp_m = cosf(m * fai);
+    return fnorm * plm * comp_m;
+}
+
+float pfo_ylm_c(int l, int m, float theta, float fai) { /* pfc_common.h:330-344 */
+    float fnorm, plm, comp_m;
+    if (m < 0) {
+        m = -m;
+        fnorm = pfo_ylm_norm(l, m, pow(-1.0, m + m));
+        plm = pfo_plgndr(l, m, cosf(theta));
+        comp_m = -sinf(m * fai);
+    } else {
+        fnorm = pfo_ylm_norm(l, m, pow(-1.0, m));
+        plm = pfo_plgndr(l, m, cosf(theta));
+        comp_m = sinf(m * fai);
+    }
+    return fnorm * plm * comp_m;
+}
+
+void pfo_cryst_order(int L, int M, int N, float dx, int n, const double* x, const double* y, const double* z,
+                     double* q4, double* q6, unsigned* nb_out) {
+    const int CNmax = 16, LL = 6;
+    const float bench100_h = 1.2f; /* elastic.cpp:105 */
+    const float frame_x = L * dx, frame_y = M * dx, frame_z = N * dx;
+    unsigned* Nb = (unsigned*)calloc((size_t)n, sizeof(unsigned));
+    int* nbi = (int*)calloc((size_t)n * CNmax, sizeof(int));
+    double* theta = (double*)calloc((size_t)n * CNmax, sizeof(double));
+    double* fai = (double*)calloc((size_t)n * CNmax, sizeof(double));
+    double* qr = (double*)calloc((size_t)n, sizeof(double));
+    double* qc = (double*)calloc((size_t)n, sizeof(double));
+    double* qm = (double*)calloc((size_t)n, sizeof(double));
+    float rjx = 0, rjy = 0, rjz = 0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            if (i == j) continue;
+            float r3[9];
+            r3[0] = (float)(x[i] - x[j]);
+            r3[1] = (float)(y[i] - y[j]);
+            r3[2] = (float)(z[i] - z[j]);
+            r3[3] = r3[0] + frame_x; r3[4] = r3[1] + frame_y; r3[5] = r3[2] + frame_z;
+            r3[6] = r3[0] - frame_x; r3[7] = r3[1] - frame_y; r3[8] = r3[2] - frame_z;
+            const float drx = min3f(fabsf(r3[0]), fabsf(r3[3]), fabsf(r3[6]));
+            const float dry = min3f(fabsf(r3[1]), fabsf(r3[4]), fabsf(r3[7]));
+            const float drz = min3f(fabsf(r3[2]), fabsf(r3[5]), fabsf(r3[8]));
+            const float rr = sqrtf(drx * drx + dry * dry + drz * drz);
+            if (!(rr < bench100_h)) continue;
+            Nb[i] = Nb[i] + 1;
+            if (!(Nb[i] <= (unsigned)CNmax)) continue;
+            for (int q = 0; q < 7; q += 3)
+                if (fabsf(fabsf(r3[q]) - drx) < 1e-3) rjx = (float)(x[i] - r3[q]);
+            for (int q = 1; q < 8; q += 3)
+                if (fabsf(fabsf(r3[q]) - dry) < 1e-3) rjy = (float)(y[i] - r3[q]);
+            for (int q = 2; q < 9; q += 3)
+                if (fabsf(fabsf(r3[q]) - drz) < 1e-3) rjz = (float)(z[i] - r3[q]);
+            const size_t s = Nb[i] - 1 + (size_t)CNmax * i;
+            const double rij = rr;
+            theta[s] = acos(((double)rjz - z[i]) / (rij + 1.e-4));
+            fai[s] = atan(((double)rjy - y[i]) / ((double)rjx - x[i] + 1.e-4));
+            if ((double)rjx - x[i] < 0.) fai[s] = fai[s] + PFO_PI;
+            nbi[s] = j;
+        }
+    const double xishu = 4 * PFO_PI / (float)(2 * LL + 1);
+    for (int pass = 0; pass < 2; pass++) {
+        for (int m = -LL; m <= LL; m++) {
+            for (int i = 0; i < n; i++) {
+                qr[i] = 0.0; qc[i] = 0.0;
+                if (Nb[i] > 0 && Nb[i] <= (unsigned)CNmax) {
+                    for (unsigned j = 0; j < Nb[i]; j++) {
+                        const size_t s = j + (size_t)CNmax * i;
+                        qr[i] += pfo_ylm_r(LL, m, (float)theta[s], (float)fai[s]);
+                        qc[i] += pfo_ylm_c(LL, m, (float)theta[s], (float)fai[s]);
+                    }
+                    qr[i] = qr[i] / (float)Nb[i];
+                    qc[i] = qc[i] / (float)Nb[i];
+                }
+            }
+            for (int i = 0; i < n; i++) {
+                double br = 0.0, bc = 0.0;
+                if (Nb[i] > 0 && Nb[i] <= (unsigned)CNmax) {
+                    for (unsigned j = 0; j < Nb[i]; j++) {
+                        const int jj = nbi[j + (size_t)CNmax * i];
+                        br += qr[jj];
+                        bc += qc[jj];
+                    }
+                    br = (qr[i] + br) / (float)(Nb[i] + 1);
+                    bc = (qc[i] + bc) / (float)(Nb[i] + 1);
+                }
+                qm[i] += br * br + bc * bc;
+            }
+        }
+        for (int i = 0; i < n; i++) {
+            if (pass == 0) q4[i] = sqrt(xishu * qm[i]);
+            else q6[i] = sqrt(xishu * qm[i]);
+        }
+    }
+    if (nb_out)
+        for (int i = 0; i < n; i++) nb_out[i] = Nb[i];
+    free(Nb); free(nbi); free(theta); free(fai); free(qr); free(qc); free(qm);
+}
+
+/* q4q6_<kd>.txt (elastic.cpp:3328-3350): atoms with x >= 0. */
+int pfo_write_q4q6(const char* path, const pfo_atoms* a, const double* q4, const double* q6) {
+    FILE* f = fopen(path, "w");
+    if (!f) return -1;
+    int cnt = 0;
+    for (int i = 0; i < a->n; i++)
+        if (a->x[i] >= 0) cnt++;
+    fprintf(f, "%d\nKommentar\n", cnt);
+    for (int i = 0; i < a->n; i++)
+        if (a->x[i] >= 0) fprintf(f, "qc %.4f %.4f %.4f %.4f %.4f\n", a->x[i], a->y[i], a->z[i], q4[i], q6[i]);
+    fclose(f);
+    return cnt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CVD pair histogram and global Q6                                           */
+/* ------------------------------------------------------------------------ */
+#define PFO_CVD_A 1.0f   /* a_lattice (cvd.cpp:33), not a CLI parameter */
+#define PFO_Q6_CNMAX 24  /* cvd.cpp:1348 */
+#define PFO_Q6_NBIN 30   /* cvd.cpp:1496 */
+
+/* delr = a_lattice/25. (cvd.cpp:1278), maxbin = int(L*dx/delr) (:1279). */
+int pfo_cvd_gr_bins(int L, float dx) {
+    const float delr = (float)(PFO_CVD_A / 25.);
+    return (int)(L * dx / delr);
+}
+
+/* Pair distance of entries i, j as GR and Q6_GLO form it (cvd.cpp:1297-1309,
+ * :1378-1390): coordinates are box fractions scaled per axis by the frame,
+ * minimum image by |.| of the three shifts, 1e-6 added after the float sqrt.
+ * r3 keeps the nine shifted components for the neighbour position. */
+static float cvd_pair(const float* x, const float* y, const float* z, int i, int j, float fx, float fy,
+                      float fz, float r3[9], float* drx, float* dry, float* drz) {
+    r3[0] = (x[i] - x[j]) * fx;
+    r3[1] = (y[i] - y[j]) * fy;
+    r3[2] = (z[i] - z[j]) * fz;
+    r3[3] = r3[0] + fx; r3[4] = r3[1] + fy; r3[5] = r3[2] + fz;
+    r3[6] = r3[0] - fx; r3[7] = r3[1] - fy; r3[8] = r3[2] - fz;
+    *drx = min3f(r3[0], r3[3], r3[6]);
+    *dry = min3f(r3[1], r3[4], r3[7]);
+    *drz = min3f(r3[2], r3[5], r3[8]);
+    return (float)((double)sqrtf(*drx * *drx + *dry * *dry + *drz * *drz) + 1e-6);
+}
+
+/* GR (cvd.cpp:1277-1312): every pair i < j with rr > a/sqrt(2)/2 adds 2 to
+ * bin int(rr/delr) when that bin is below maxbin. */
+void pfo_cvd_gr(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
+                unsigned* hist) {
+    const float fx = L * dx, fy = M * dx, fz = N * dx;
+    const float delr = (float)(PFO_CVD_A / 25.);
+    const int maxbin = pfo_cvd_gr_bins(L, dx);
+    for (int b = 0; b < maxbin; b++) hist[b] = 0;
+    for (int i = 0; i < n - 1; i++)
+        for (int j = i + 1; j < n; j++) {
+            float r3[9], drx, dry, drz;
+            const float rr = cvd_pair(x, y, z, i, j, fx, fy, fz, r3, &drx, &dry, &drz);
+            const unsigned bin = (unsigned)(int)(rr / delr);
+            if ((double)rr > PFO_CVD_A / sqrt(2.) * 0.5 && bin < (unsigned)maxbin) hist[bin] += 2;
+        }
+}
+
+/* hist_<kd>.txt (cvd.cpp:1314-1322) and Grq6_<kd>.txt (:1324-1339, nstep = 1);
+ * num_t counts the entries with x >= 0. */
+int pfo_write_cvd_gr(int kd, int L, int M, int N, float dx, int n, const float* x, const unsigned* hist) {
+    const float delr = (float)(PFO_CVD_A / 25.);
+    const int maxbin = pfo_cvd_gr_bins(L, dx);
+    int num_t = 0;
+    for (int i = 0; i < n; i++)
+        if (x[i] >= 0) num_t++;
+    char name[64];
+    snprintf(name, sizeof name, "hist_%d.txt", kd);
+    FILE* f = fopen(name, "w");
+    if (!f) return -1;
+    for (int b = 0; b < maxbin; b++) fprintf(f, "%d %u %d\n", b, hist[b], maxbin);
+    fclose(f);
+    const float rho = (float)num_t / (float)(L * M * N * dx * dx * dx);
+    const float preco = (float)(4. * PFO_PI * rho / 3.);
+    snprintf(name, sizeof name, "Grq6_%d.txt", kd);
+    f = fopen(name, "w");
+    if (!f) return -1;
+    for (int b = 0; b < maxbin; b++) {
+        const float rlower = (float)b * delr;
+        const float rupper = rlower + delr;
+        const float nideal = preco * (rupper * rupper * rupper - rlower * rlower * rlower);
+        const float gr = (float)hist[b] / (float)1 / (float)num_t / nideal;
+        fprintf(f, "%.4f %.4f %u %.4f\n", b * delr, gr, hist[b], nideal);
+    }
+    fclose(f);
+    return 0;
+}
+
+/* Q6_GLO (cvd.cpp:1347-1510).  Neighbours: a*sqrt(3)/2 * [0.9, 1.2] -/+ 1e-3,
+ * at most 24 stored, counted beyond that.  The neighbour position takes,
+ * per axis, every shift within 1e-5 of the minimum (the last wins); theta =
+ * acos(dz/rr), fai = atan(dy/dx) (0 when dy == 0), no quadrant correction.
+ * For each m: qlm = sum_j Y_6m (double sums of float terms), qlmb = qlm/Nb;
+ * per entry qlmb_m += |qlmb|^2 (float); global Qlm = sum_i Nb_i qlmb_i /
+ * sum_i Nb_i (float accumulators in entry order); Q = sqrt(4 pi/13 sum |Qlm|^2). */
+void pfo_cvd_q6(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
+                unsigned* nb, float* ql, float* qglobal, unsigned long long* histq) {
+    const int CN = PFO_Q6_CNMAX, l = 6;
+    const float fx = L * dx, fy = M * dx, fz = N * dx;
+    const double rmax = PFO_CVD_A * sqrt(3.) / 2. * 1.2 + 1e-3, rmin = PFO_CVD_A * sqrt(3.) / 2. * 0.9 - 1e-3;
+    float* rij = (float*)calloc((size_t)n * CN, sizeof(float));
+    float* cx = (float*)calloc((size_t)n * CN, sizeof(float));
+    float* cy = (float*)calloc((size_t)n * CN, sizeof(float));
+    float* cz = (float*)calloc((size_t)n * CN, sizeof(float));
+    float* theta = (float*)calloc((size_t)n * CN, sizeof(float));
+    float* fai = (float*)calloc((size_t)n * CN, sizeof(float));
+    double* qb = (double*)calloc((size_t)n * 2, sizeof(double));
+    float* qbm = (float*)calloc((size_t)n, sizeof(float));
+    float rjx = 0, rjy = 0, rjz = 0;
+    for (int i = 0; i < n; i++) {
+        nb[i] = 0;
+        for (int j = 0; j < n; j++) {
+            if (i == j) continue;
+            float r3[9], drx, dry, drz;
+            const float rr = cvd_pair(x, y, z, i, j, fx, fy, fz, r3, &drx, &dry, &drz);
+            if (!((double)rr <= rmax && (double)rr >= rmin)) continue;
+            nb[i]++;
+            if (nb[i] > (unsigned)CN) continue;
+            const size_t s = nb[i] - 1 + (size_t)CN * i;
+            rij[s] = rr;
+            for (int q = 0; q < 7; q += 3)
+                if ((double)(fabsf(r3[q]) - drx) < 1e-5) rjx = x[i] * fx - r3[q];
+            for (int q = 1; q < 8; q += 3)
+                if ((double)(fabsf(r3[q]) - dry) < 1e-5) rjy = y[i] * fy - r3[q];
+            for (int q = 2; q < 9; q += 3)
+                if ((double)(fabsf(r3[q]) - drz) < 1e-5) rjz = z[i] * fz - r3[q];
+            cx[s] = rjx; cy[s] = rjy; cz[s] = rjz;
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        if (!(nb[i] > 0 && nb[i] <= (unsigned)CN)) continue;
+        for (unsigned j = 0; j < nb[i]; j++) {
+            const size_t s = j + (size_t)CN * i;
+            theta[s] = acosf((cz[s] - z[i] * fz) / rij[s]);
+            const float ddy = cy[s] - y[i] * fy;
+            fai[s] = (ddy == 0.) ? 0.f : atanf(ddy / (cx[s] - x[i] * fx));
+        }
+    }
+    float qm_glob = 0.f;
+    for (int m = -l; m <= l; m++) {
+        for (int i = 0; i < n; i++) {
+            double re = 0.0, im = 0.0;
+            qb[2 * i] = 0.0;
+            qb[2 * i + 1] = 0.0;
+            if (nb[i] > 0 && nb[i] <= (unsigned)CN) {
+                for (unsigned j = 0; j < nb[i]; j++) {
+                    const size_t s = j + (size_t)CN * i;
+                    re += pfo_ylm_r(l, m, theta[s], fai[s]);
+                    im += pfo_ylm_c(l, m, theta[s], fai[s]);
+                }
+                qb[2 * i] = re / (float)nb[i];
+                qb[2 * i + 1] = im / (float)nb[i];
+            }
+        }
+        float sum_nb = 0.f, qr = 0.f, qc = 0.f;
+        for (int i = 0; i < n; i++) {
+            qbm[i] = (float)((double)qbm[i] + (qb[2 * i] * qb[2 * i] + qb[2 * i + 1] * qb[2 * i + 1]));
+            qr = (float)((double)qr + (double)nb[i] * qb[2 * i]);
+            qc = (float)((double)qc + (double)nb[i] * qb[2 * i + 1]);
+            sum_nb = sum_nb + (float)nb[i];
+        }
+        qr = qr / sum_nb;
+        qc = qc / sum_nb;
+        qm_glob = qm_glob + (qr * qr + qc * qc);
+    }
+    const float xishu = (float)(4 * PFO_PI / (float)(2 * l + 1));
+    const float delq = 0.05f;
+    *qglobal = sqrtf(xishu * qm_glob);
+    for (int b = 0; b < PFO_Q6_NBIN; b++) histq[b] = 0;
+    for (int i = 0; i < n; i++) {
+        ql[i] = sqrtf(xishu * qbm[i]);
+        const unsigned bin = (unsigned)(int)(ql[i] / delq);
+        if (bin < PFO_Q6_NBIN) histq[bin]++;
+    }
+    free(rij); free(cx); free(cy); free(cz); free(theta); free(fai); free(qb); free(qbm);
+}
+
+/* Ql_<kd>.txt ("i Nb Ql_L Ql", 6 decimals) and HistQ_<kd>.txt ((i+1)*delq,
+ * count) (cvd.cpp:1449-1510). */
+int pfo_write_cvd_q6(int kd, int n, const unsigned* nb, const float* ql, float qglobal,
+                     const unsigned long long* histq) {
+    char name[64];
+    snprintf(name, sizeof name, "Ql_%d.txt", kd);
+    FILE* f = fopen(name, "w");
+    if (!f) return -1;
+    for (int i = 0; i < n; i++) fprintf(f, "%d %u %.6f %.6f\n", i, nb[i], ql[i], qglobal);
+    fclose(f);
+    snprintf(name, sizeof name, "HistQ_%d.txt", kd);
+    f = fopen(name, "w");
+    if (!f) return -1;
+    for (int b = 0; b < PFO_Q6_NBIN; b++) fprintf(f, "%.4f %llu\n", (b + 1) * 0.05f, histq[b]);
+    fclose(f);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* VTI I/O                                                                    */
 /* ------------------------------------------------------------------------ */
 

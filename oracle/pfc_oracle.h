@@ -127,6 +127,27 @@ int pfo_read_vti(const char* path, int L, int M, int N, float* field);
 int pfo_write_phimax_misfit(const char* path, const pfo_atoms* a);
 int pfo_write_phimax_elastic(const char* path, const pfo_atoms* a);
 int pfo_write_strain(const char* path, const pfo_atoms* a, const double* strain, const unsigned* nb);
+/* cryst_order (elastic.cpp:3013-3389): per-atom q4, q6 and neighbour counts. */
+void pfo_cryst_order(int L, int M, int N, float dx, int n, const double* x, const double* y, const double* z,
+                     double* q4, double* q6, unsigned* nb);
+float pfo_ylm_r(int l, int m, float theta, float fai);
+float pfo_ylm_c(int l, int m, float theta, float fai);
+int pfo_write_q4q6(const char* path, const pfo_atoms* a, const double* q4, const double* q6);
+
+/* CVD GR (cvd.cpp:1277-1344) over the gathered pc_* fractions: number of
+ * bins, and the pair histogram (2 per pair, minimum image). */
+int pfo_cvd_gr_bins(int L, float dx);
+void pfo_cvd_gr(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
+                unsigned* hist);
+/* hist_<kd>.txt and Grq6_<kd>.txt (cvd.cpp:1314-1339). */
+int pfo_write_cvd_gr(int kd, int L, int M, int N, float dx, int n, const float* x, const unsigned* hist);
+/* CVD Q6_GLO (cvd.cpp:1347-1527): local Q6 per entry, the global Q6 and the
+ * 30-bin histogram of the local values. */
+void pfo_cvd_q6(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
+                unsigned* nb, float* ql, float* qglobal, unsigned long long* histq);
+/* Ql_<kd>.txt and HistQ_<kd>.txt (cvd.cpp:1449-1510). */
+int pfo_write_cvd_q6(int kd, int n, const unsigned* nb, const float* ql, float qglobal,
+                     const unsigned long long* histq);
 
 #ifdef __cplusplus
 }

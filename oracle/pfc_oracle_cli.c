@@ -205,6 +205,20 @@ static void cvd_outputs(const pfo_model* p, int kd, const double* phi) {
     for (int i = 0; i < na; i++)
         fprintf(f, "%.4f %.4f %.4f %.4f\n", pz[i] * frame_x, px[i] * frame_y, py[i] * frame_z, pp[i]);
     fclose(f);
+    /* GR every checkpoint, Q6_GLO when kd % 20 == 0 (cvd.cpp:1253-1254) */
+    unsigned* hist = (unsigned*)malloc(sizeof(unsigned) * (size_t)(pfo_cvd_gr_bins(L, p->dx) + 1));
+    pfo_cvd_gr(L, M, N, p->dx, na, px, py, pz, hist);
+    pfo_write_cvd_gr(kd, L, M, N, p->dx, na, px, hist);
+    free(hist);
+    if (kd >= 0 && kd % 20 == 0) {
+        unsigned* nb = (unsigned*)malloc(sizeof(unsigned) * (size_t)na);
+        float* ql = (float*)malloc(sizeof(float) * (size_t)na);
+        unsigned long long histq[30];
+        float qg;
+        pfo_cvd_q6(L, M, N, p->dx, na, px, py, pz, nb, ql, &qg, histq);
+        pfo_write_cvd_q6(kd, na, nb, ql, qg, histq);
+        free(nb); free(ql);
+    }
     free(px); free(py); free(pz); free(pp);
 }
 
@@ -348,6 +362,8 @@ static int run_elastic(cli_t* c) {
     pfo_atoms at = atoms_alloc((int)P);
     double* strain = (double*)malloc(sizeof(double) * P);
     unsigned* nb = (unsigned*)malloc(sizeof(unsigned) * P);
+    double* q4 = (double*)malloc(sizeof(double) * P);
+    double* q6 = (double*)malloc(sizeof(double) * P);
     for (int i = 0; i < nlist; i++) {
         const int kd = steps[i];
         fprintf(stderr, "PFC_PROGRESS step=%d total=%d\n", i + 1, nlist);
@@ -366,8 +382,11 @@ static int run_elastic(cli_t* c) {
         pfo_strain(p->L, p->M, p->N, p->dx, c->benchel, at.n, at.x, at.y, at.z, strain, nb);
         snprintf(name, sizeof name, "strain_%d.txt", kd);
         pfo_write_strain(name, &at, strain, nb);
+        pfo_cryst_order(p->L, p->M, p->N, p->dx, at.n, at.x, at.y, at.z, q4, q6, NULL);
+        snprintf(name, sizeof name, "q4q6_%d.txt", kd);
+        pfo_write_q4q6(name, &at, q4, q6);
     }
-    free(phi); free(strain); free(nb);
+    free(phi); free(strain); free(nb); free(q4); free(q6);
     atoms_free(&at);
     return 0;
 }

@@ -619,6 +619,37 @@ extern "C" int pfc_phimax_elastic(int L, int M, int N, float dx, const float* ph
     return PFC_OK;
 }
 
+extern "C" int pfc_cryst_order(int L, int M, int N, float dx, int n, const double* x, const double* y,
+                               const double* z, double* q4, double* q6, unsigned int* nb) {
+    if (L <= 0 || M <= 0 || N <= 0 || n < 0 || (n > 0 && (!x || !y || !z || !q4 || !q6)))
+        return fail(PFC_E_ARG, "bad arguments");
+    if (n == 0) return PFC_OK;
+    hipError_t e = post_cryst(L, M, N, dx, n, x, y, z, q4, q6, nb);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "cryst_order: %s", hipGetErrorString(e));
+    return PFC_OK;
+}
+
+extern "C" int pfc_cvd_gr(int L, int M, int N, float dx, int n, const float* pc_x, const float* pc_y,
+                          const float* pc_z, unsigned int* hist, int* maxbin) {
+    if (L <= 0 || M <= 0 || N <= 0 || !(dx > 0.0f) || n < 0 || !maxbin) return fail(PFC_E_ARG, "bad arguments");
+    *maxbin = cvd_gr_bins(L, dx);
+    if (!hist) return PFC_OK;
+    if (n > 0 && (!pc_x || !pc_y || !pc_z)) return fail(PFC_E_ARG, "bad arguments");
+    hipError_t e = post_cvd_gr(L, M, N, dx, n, pc_x, pc_y, pc_z, hist);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "cvd_gr: %s", hipGetErrorString(e));
+    return PFC_OK;
+}
+
+extern "C" int pfc_cvd_q6(int L, int M, int N, float dx, int n, const float* pc_x, const float* pc_y,
+                          const float* pc_z, unsigned int* nb, float* ql, float* qglobal,
+                          unsigned long long* histq) {
+    if (L <= 0 || M <= 0 || N <= 0 || n <= 0 || !pc_x || !pc_y || !pc_z || !nb || !ql || !qglobal || !histq)
+        return fail(PFC_E_ARG, "bad arguments");
+    hipError_t e = post_cvd_q6(L, M, N, dx, n, pc_x, pc_y, pc_z, nb, ql, qglobal, histq);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "cvd_q6: %s", hipGetErrorString(e));
+    return PFC_OK;
+}
+
 extern "C" int pfc_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
                           const double* z, double* strain, unsigned int* nb) {
     if (L <= 0 || M <= 0 || N <= 0 || n < 0 || (n > 0 && (!x || !y || !z || !strain || !nb)))

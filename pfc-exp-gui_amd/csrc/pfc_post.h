@@ -24,6 +24,18 @@ hipError_t post_phimax_elastic(int L, int M, int N, float dx, const float* phi, 
 hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
                        const double* z, double* strain, unsigned* nb);
 
+// cryst_order on host atom positions: per-atom q4, q6, neighbour counts.
+hipError_t post_cryst(int L, int M, int N, float dx, int n, const double* x, const double* y, const double* z,
+                      double* q4, double* q6, unsigned* nb);
+
+// CVD GR: bins (int(L*dx/delr)) and the pair histogram of n atom fractions.
+int cvd_gr_bins(int L, float dx);
+hipError_t post_cvd_gr(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
+                       unsigned* hist);
+// CVD Q6_GLO: neighbour counts, local Q6, global Q6, 30-bin histogram.
+hipError_t post_cvd_q6(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
+                       unsigned* nb, float* ql, float* qglobal, unsigned long long* histq);
+
 // Plain device copy of n 16-byte words (counter calibration); pattern 0:
 // linear, 1: 64-byte pieces of a [n/256][256] array copied column-wise.
 hipError_t launch_copy16(const void* src, void* dst, size_t n, int pattern, hipStream_t s);

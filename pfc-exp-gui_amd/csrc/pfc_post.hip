@@ -553,6 +553,455 @@ hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, cons
 }
 
 // ---------------------------------------------------------------------------
+// cryst_order (elastic.cpp:3013-3389): Lechner-Dellago averaged q_l of every
+// atom, with the reference's spherical harmonics (pfc_common.h:274-346, float)
+// and its quirks: both loops use l = 6 and the second keeps adding onto the
+// first one's per-atom sum, so q6 comes from the sum taken twice; atoms with
+// no or more than 16 neighbours get q = 0 but still count as neighbours.
+// Three passes: neighbour lists in the reference's j order (LDS-tiled like
+// k_strain), per-atom q_6m, neighbour averages and the sums.
+// ---------------------------------------------------------------------------
+constexpr int QL = 6, QM = 2 * QL + 1, CNMAX = 16;
+
+__device__ __forceinline__ float dv_lg_poly(int l, float x) {
+#pragma clang fp contract(off)
+    float plm2 = 1, plm1 = x, pl = 0;
+    if (l == 0) return 1;
+    if (l == 1) return x;
+    for (int i = 2; i <= l; i++) {
+        pl = ((2 * i - 1) * x * (float)plm1 - (i - 1) * plm2) / (float)i;
+        plm2 = plm1;
+        plm1 = pl;
+    }
+    return pl;
+}
+
+__device__ __forceinline__ float dv_plgndr(int l, int m, float x) {
+#pragma clang fp contract(off)
+    float fact, pll = 0, pmm, pmmp1, somx2;
+    if (m < 0) m = -m;
+    if (m == 0) return dv_lg_poly(l, x);
+    pmm = 1.0f;
+    somx2 = (float)sqrt((1.0 - x) * (1.0 + x));
+    fact = 1.0f;
+    for (int i = 1; i <= m; i++) {
+        pmm *= -fact * somx2;
+        fact = (float)(fact + 2.0);
+    }
+    if (l == m) return pmm;
+    pmmp1 = x * (2 * m + 1) * pmm;
+    if (l == m + 1) return pmmp1;
+    for (int ll = m + 2; ll <= l; ll++) {
+        pll = (x * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pmm) / (ll - m);
+        pmm = pmmp1;
+        pmmp1 = pll;
+    }
+    return pll;
+}
+
+__device__ __forceinline__ float dv_ylm_norm(int l, int m, double sgn) {
+#pragma clang fp contract(off)
+    int f1 = 1, f2 = 1;
+    for (int i = 1; i <= l - m; i++) f1 = f1 * i;
+    for (int i = 1; i <= l + m; i++) f2 = f2 * i;
+    return (float)(sgn * sqrt((2 * l + 1) / 4. / 3.1415926) * (double)sqrtf((float)f1 / (float)f2));
+}
+
+// Real and imaginary parts as the reference's Ylmtf_r / Ylmtf_c.
+__device__ __forceinline__ void dv_ylm(int l, int m, float theta, float fai, float& yr, float& yc) {
+#pragma clang fp contract(off)
+    const int am = m < 0 ? -m : m;
+    const float plm = dv_plgndr(l, am, cosf(theta));
+    const float nr = dv_ylm_norm(l, am, (am & 1) ? -1.0 : 1.0);
+    yr = nr * plm * cosf(am * fai);
+    if (m < 0) yc = dv_ylm_norm(l, am, 1.0) * plm * (-sinf(am * fai));
+    else yc = nr * plm * sinf(am * fai);
+}
+
+__global__ void __launch_bounds__(SB) k_cryst_nb(int n, const double* __restrict__ X, const double* __restrict__ Y,
+                                                 const double* __restrict__ Z, float fx, float fy, float fz,
+                                                 unsigned* nbo, int* nbi, double* theta, double* fai) {
+#pragma clang fp contract(off)
+    __shared__ double sx[SB], sy[SB], sz[SB];
+    const int i = blockIdx.x * SB + threadIdx.x;
+    const bool act = i < n;
+    const double xi = act ? X[i] : 0.0, yi = act ? Y[i] : 0.0, zi = act ? Z[i] : 0.0;
+    unsigned Nb = 0;
+    float rjx = 0.f, rjy = 0.f, rjz = 0.f;
+    for (int j0 = 0; j0 < n; j0 += SB) {
+        __syncthreads();
+        const int jl = j0 + threadIdx.x;
+        if (jl < n) { sx[threadIdx.x] = X[jl]; sy[threadIdx.x] = Y[jl]; sz[threadIdx.x] = Z[jl]; }
+        __syncthreads();
+        const int cnt = n - j0 < SB ? n - j0 : SB;
+        if (!act) continue;
+        for (int jj = 0; jj < cnt; jj++) {
+            const int j = j0 + jj;
+            if (j == i) continue;
+            float r3[9];
+            r3[0] = (float)(xi - sx[jj]);
+            r3[1] = (float)(yi - sy[jj]);
+            r3[2] = (float)(zi - sz[jj]);
+            r3[3] = r3[0] + fx; r3[4] = r3[1] + fy; r3[5] = r3[2] + fz;
+            r3[6] = r3[0] - fx; r3[7] = r3[1] - fy; r3[8] = r3[2] - fz;
+            const float drx = min3f(fabsf(r3[0]), fabsf(r3[3]), fabsf(r3[6]));
+            const float dry = min3f(fabsf(r3[1]), fabsf(r3[4]), fabsf(r3[7]));
+            const float drz = min3f(fabsf(r3[2]), fabsf(r3[5]), fabsf(r3[8]));
+            const float rr = sqrtf(drx * drx + dry * dry + drz * drz);
+            if (!(rr < 1.2f)) continue;  // bench100_h (elastic.cpp:105)
+            Nb = Nb + 1;
+            if (Nb > (unsigned)CNMAX) continue;
+#pragma unroll
+            for (int q = 0; q < 7; q += 3)
+                if ((double)fabsf(fabsf(r3[q]) - drx) < 1e-3) rjx = (float)(xi - (double)r3[q]);
+#pragma unroll
+            for (int q = 1; q < 8; q += 3)
+                if ((double)fabsf(fabsf(r3[q]) - dry) < 1e-3) rjy = (float)(yi - (double)r3[q]);
+#pragma unroll
+            for (int q = 2; q < 9; q += 3)
+                if ((double)fabsf(fabsf(r3[q]) - drz) < 1e-3) rjz = (float)(zi - (double)r3[q]);
+            const size_t s = (size_t)i * CNMAX + (Nb - 1);
+            const double rij = rr;
+            theta[s] = acos(((double)rjz - zi) / (rij + 1.e-4));
+            double f = atan(((double)rjy - yi) / ((double)rjx - xi + 1.e-4));
+            if ((double)rjx - xi < 0.) f = f + 3.1415926;
+            fai[s] = f;
+            nbi[s] = j;
+        }
+    }
+    if (act) nbo[i] = Nb;
+}
+
+// qlm[(i*QM + m+QL)*2 + {0,1}] = sum_j Y_6m(j) / Nb (double sums of float terms).
+__global__ void __launch_bounds__(SB) k_cryst_qlm(int n, const unsigned* __restrict__ nbo,
+                                                  const double* __restrict__ theta, const double* __restrict__ fai,
+                                                  double* qlm) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * SB + threadIdx.x;
+    if (i >= n) return;
+    const unsigned Nb = nbo[i];
+    const bool ok = Nb > 0 && Nb <= (unsigned)CNMAX;
+    for (int m = -QL; m <= QL; m++) {
+        double re = 0.0, im = 0.0;
+        if (ok) {
+            for (unsigned j = 0; j < Nb; j++) {
+                const size_t s = (size_t)i * CNMAX + j;
+                float yr, yc;
+                dv_ylm(QL, m, (float)theta[s], (float)fai[s], yr, yc);
+                re += yr;
+                im += yc;
+            }
+            re = re / (float)Nb;
+            im = im / (float)Nb;
+        }
+        qlm[((size_t)i * QM + (m + QL)) * 2] = re;
+        qlm[((size_t)i * QM + (m + QL)) * 2 + 1] = im;
+    }
+}
+
+__global__ void __launch_bounds__(SB) k_cryst_q(int n, const unsigned* __restrict__ nbo, const int* __restrict__ nbi,
+                                                const double* __restrict__ qlm, double* q4, double* q6) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * SB + threadIdx.x;
+    if (i >= n) return;
+    const unsigned Nb = nbo[i];
+    const bool ok = Nb > 0 && Nb <= (unsigned)CNMAX;
+    const double xishu = 4 * 3.1415926 / (float)(2 * QL + 1);
+    double acc = 0.0;
+    for (int pass = 0; pass < 2; pass++) {
+        for (int m = 0; m < QM; m++) {
+            double br = 0.0, bc = 0.0;
+            if (ok) {
+                for (unsigned j = 0; j < Nb; j++) {
+                    const int jj = nbi[(size_t)i * CNMAX + j];
+                    br += qlm[((size_t)jj * QM + m) * 2];
+                    bc += qlm[((size_t)jj * QM + m) * 2 + 1];
+                }
+                br = (qlm[((size_t)i * QM + m) * 2] + br) / (float)(Nb + 1);
+                bc = (qlm[((size_t)i * QM + m) * 2 + 1] + bc) / (float)(Nb + 1);
+            }
+            acc += br * br + bc * bc;
+        }
+        if (pass == 0) q4[i] = sqrt(xishu * acc);
+        else q6[i] = sqrt(xishu * acc);
+    }
+}
+
+hipError_t post_cryst(int L, int M, int N, float dx, int n, const double* x, const double* y, const double* z,
+                      double* q4, double* q6, unsigned* nb) {
+    hipError_t e = hipSuccess;
+    Scratch sc;
+    double* dx_ = sc.get<double>(n, e);
+    double* dy_ = sc.get<double>(n, e);
+    double* dz_ = sc.get<double>(n, e);
+    unsigned* dn = sc.get<unsigned>(n, e);
+    int* di = sc.get<int>((size_t)n * CNMAX, e);
+    double* dth = sc.get<double>((size_t)n * CNMAX, e);
+    double* dfa = sc.get<double>((size_t)n * CNMAX, e);
+    double* dq = sc.get<double>((size_t)n * QM * 2, e);
+    double* d4 = sc.get<double>(n, e);
+    double* d6 = sc.get<double>(n, e);
+    if (e != hipSuccess) return e;
+    if ((e = hipMemcpy(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dz_, z, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    const float fx = L * dx, fy = M * dx, fz = N * dx;
+    const dim3 g((n + SB - 1) / SB);
+    hipLaunchKernelGGL(k_cryst_nb, g, dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, dn, di, dth, dfa);
+    hipLaunchKernelGGL(k_cryst_qlm, g, dim3(SB), 0, 0, n, dn, dth, dfa, dq);
+    hipLaunchKernelGGL(k_cryst_q, g, dim3(SB), 0, 0, n, dn, di, dq, d4, d6);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpy(q4, d4, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+    if ((e = hipMemcpy(q6, d6, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+    if (nb) e = hipMemcpy(nb, dn, sizeof(unsigned) * n, hipMemcpyDeviceToHost);
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// CVD GR and Q6_GLO (cvd.cpp:1277-1510) on the gathered atom fractions pc_*.
+// Pair distances: fractions times the frame per axis, minimum image by |.|
+// of the three shifts, 1e-6 added after the float sqrt (a_lattice = 1).
+// GR: every pair i < j adds 2 to bin int(rr/delr); blocks cover an i tile
+// times a run of j tiles (the triangle's empty blocks return at once) and
+// count into an LDS histogram when it fits.  Q6_GLO: neighbour lists in j
+// order (window a*sqrt(3)/2*[0.9,1.2], 24 stored), per-entry q_6m and local
+// Q6, then the global Q6 from float accumulators taken in entry order, as
+// the reference adds them (one lane per m).
+// ---------------------------------------------------------------------------
+constexpr int GRJT = 8;          // j tiles per GR block
+constexpr int GR_LDS_BINS = 12288;  // 48 KB of dynamic LDS
+constexpr int Q6CN = 24, Q6NBIN = 30, QB = 64;
+
+__device__ __forceinline__ float cvd_pair(float xi, float yi, float zi, float xj, float yj, float zj, float fx,
+                                          float fy, float fz, float* r3, float& drx, float& dry, float& drz) {
+#pragma clang fp contract(off)
+    r3[0] = (xi - xj) * fx;
+    r3[1] = (yi - yj) * fy;
+    r3[2] = (zi - zj) * fz;
+    r3[3] = r3[0] + fx; r3[4] = r3[1] + fy; r3[5] = r3[2] + fz;
+    r3[6] = r3[0] - fx; r3[7] = r3[1] - fy; r3[8] = r3[2] - fz;
+    drx = min3f(r3[0], r3[3], r3[6]);
+    dry = min3f(r3[1], r3[4], r3[7]);
+    drz = min3f(r3[2], r3[5], r3[8]);
+    return (float)((double)sqrtf(drx * drx + dry * dry + drz * drz) + 1e-6);
+}
+
+template <bool LDSH>
+__global__ void __launch_bounds__(SB) k_cvd_gr(int n, const float* __restrict__ X, const float* __restrict__ Y,
+                                               const float* __restrict__ Z, float fx, float fy, float fz,
+                                               float delr, double rmin, int maxbin, unsigned* hist) {
+#pragma clang fp contract(off)
+    extern __shared__ unsigned sh[];
+    __shared__ float sx[SB], sy[SB], sz[SB];
+    const int i0 = blockIdx.x * SB;
+    const int jt0 = blockIdx.y * GRJT * SB;  // first j of this block
+    if (jt0 + GRJT * SB <= i0 + 1) return;   // every j <= every i: no pair
+    if (LDSH)
+        for (int b = threadIdx.x; b < maxbin; b += SB) sh[b] = 0;
+    const int i = i0 + threadIdx.x;
+    const bool act = i < n - 1;
+    const float xi = act ? X[i] : 0.f, yi = act ? Y[i] : 0.f, zi = act ? Z[i] : 0.f;
+    for (int t = 0; t < GRJT; t++) {
+        const int j0 = jt0 + t * SB;
+        if (j0 >= n) break;
+        __syncthreads();
+        const int jl = j0 + threadIdx.x;
+        if (jl < n) { sx[threadIdx.x] = X[jl]; sy[threadIdx.x] = Y[jl]; sz[threadIdx.x] = Z[jl]; }
+        __syncthreads();
+        if (!act) continue;
+        const int cnt = n - j0 < SB ? n - j0 : SB;
+        const int jj0 = i + 1 - j0 > 0 ? i + 1 - j0 : 0;
+        for (int jj = jj0; jj < cnt; jj++) {
+            float r3[9], drx, dry, drz;
+            const float rr = cvd_pair(xi, yi, zi, sx[jj], sy[jj], sz[jj], fx, fy, fz, r3, drx, dry, drz);
+            const unsigned bin = (unsigned)(int)(rr / delr);
+            if ((double)rr > rmin && bin < (unsigned)maxbin) atomicAdd(LDSH ? &sh[bin] : &hist[bin], 2u);
+        }
+    }
+    if (LDSH) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < maxbin; b += SB)
+            if (sh[b]) atomicAdd(&hist[b], sh[b]);
+    }
+}
+
+// Neighbour list of entry i in j order with its (theta, fai) as the
+// reference forms them from the stored neighbour position.
+__global__ void __launch_bounds__(QB) k_q6_nb(int n, const float* __restrict__ X, const float* __restrict__ Y,
+                                              const float* __restrict__ Z, float fx, float fy, float fz, double rlo,
+                                              double rhi, unsigned* nbo, float* theta, float* fai) {
+#pragma clang fp contract(off)
+    __shared__ float sx[QB], sy[QB], sz[QB];
+    const int i = blockIdx.x * QB + threadIdx.x;
+    const bool act = i < n;
+    const float xi = act ? X[i] : 0.f, yi = act ? Y[i] : 0.f, zi = act ? Z[i] : 0.f;
+    unsigned Nb = 0;
+    float rjx = 0.f, rjy = 0.f, rjz = 0.f;
+    for (int j0 = 0; j0 < n; j0 += QB) {
+        __syncthreads();
+        const int jl = j0 + threadIdx.x;
+        if (jl < n) { sx[threadIdx.x] = X[jl]; sy[threadIdx.x] = Y[jl]; sz[threadIdx.x] = Z[jl]; }
+        __syncthreads();
+        if (!act) continue;
+        const int cnt = n - j0 < QB ? n - j0 : QB;
+        for (int jj = 0; jj < cnt; jj++) {
+            if (j0 + jj == i) continue;
+            float r3[9], drx, dry, drz;
+            const float rr = cvd_pair(xi, yi, zi, sx[jj], sy[jj], sz[jj], fx, fy, fz, r3, drx, dry, drz);
+            if (!((double)rr <= rhi && (double)rr >= rlo)) continue;
+            Nb = Nb + 1;
+            if (Nb > (unsigned)Q6CN) continue;
+#pragma unroll
+            for (int q = 0; q < 7; q += 3)
+                if ((double)(fabsf(r3[q]) - drx) < 1e-5) rjx = xi * fx - r3[q];
+#pragma unroll
+            for (int q = 1; q < 8; q += 3)
+                if ((double)(fabsf(r3[q]) - dry) < 1e-5) rjy = yi * fy - r3[q];
+#pragma unroll
+            for (int q = 2; q < 9; q += 3)
+                if ((double)(fabsf(r3[q]) - drz) < 1e-5) rjz = zi * fz - r3[q];
+            const size_t s = (size_t)i * Q6CN + (Nb - 1);
+            theta[s] = acosf((rjz - zi * fz) / rr);
+            const float ddy = rjy - yi * fy;
+            fai[s] = (ddy == 0.) ? 0.f : atanf(ddy / (rjx - xi * fx));
+        }
+    }
+    if (act) nbo[i] = Nb;
+}
+
+// Per entry: qb[m][i] = sum_j Y_6m / Nb (0 without stored neighbours), local
+// Q6 = sqrt(xishu * sum_m |qb|^2) (float accumulator over m), its histogram.
+__global__ void __launch_bounds__(SB) k_q6_local(int n, const unsigned* __restrict__ nbo,
+                                                 const float* __restrict__ theta, const float* __restrict__ fai,
+                                                 double2* qb, float* ql, unsigned long long* histq) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * SB + threadIdx.x;
+    if (i >= n) return;
+    const unsigned Nb = nbo[i];
+    const bool ok = Nb > 0 && Nb <= (unsigned)Q6CN;
+    float qbm = 0.f;
+    for (int m = -QL; m <= QL; m++) {
+        double re = 0.0, im = 0.0;
+        if (ok) {
+            for (unsigned j = 0; j < Nb; j++) {
+                const size_t s = (size_t)i * Q6CN + j;
+                float yr, yc;
+                dv_ylm(QL, m, theta[s], fai[s], yr, yc);
+                re += yr;
+                im += yc;
+            }
+            re = re / (float)Nb;
+            im = im / (float)Nb;
+        }
+        qb[(size_t)(m + QL) * n + i] = make_double2(re, im);
+        qbm = (float)((double)qbm + (re * re + im * im));
+    }
+    const float xishu = (float)(4 * 3.1415926 / (float)(2 * QL + 1));
+    const float q = sqrtf(xishu * qbm);
+    ql[i] = q;
+    const unsigned bin = (unsigned)(int)(q / 0.05f);
+    if (bin < (unsigned)Q6NBIN) atomicAdd(&histq[bin], 1ull);
+}
+
+// Global Q6: lane m+6 runs the reference's float accumulators over the
+// entries in order; lane 0 adds the 13 |Q_6m|^2 in m order.
+__global__ void __launch_bounds__(64) k_q6_global(int n, const unsigned* __restrict__ nbo,
+                                                  const double2* __restrict__ qb, float* qglobal) {
+#pragma clang fp contract(off)
+    __shared__ float part[QM];
+    const int t = threadIdx.x;
+    if (t < QM) {
+        float qr = 0.f, qc = 0.f, sum_nb = 0.f;
+        const double2* q = qb + (size_t)t * n;
+        for (int i = 0; i < n; i++) {
+            const unsigned nb = nbo[i];
+            const double2 v = q[i];
+            qr = (float)((double)qr + (double)nb * v.x);
+            qc = (float)((double)qc + (double)nb * v.y);
+            sum_nb = sum_nb + (float)nb;
+        }
+        qr = qr / sum_nb;
+        qc = qc / sum_nb;
+        part[t] = qr * qr + qc * qc;
+    }
+    __syncthreads();
+    if (t == 0) {
+        float acc = 0.f;
+        for (int m = 0; m < QM; m++) acc = acc + part[m];
+        const float xishu = (float)(4 * 3.1415926 / (float)(2 * QL + 1));
+        qglobal[0] = sqrtf(xishu * acc);
+    }
+}
+
+int cvd_gr_bins(int L, float dx) {
+    const float delr = (float)(1.0f / 25.);
+    return (int)(L * dx / delr);
+}
+
+hipError_t post_cvd_gr(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
+                       unsigned* hist) {
+    hipError_t e = hipSuccess;
+    const int maxbin = cvd_gr_bins(L, dx);
+    if (maxbin <= 0) return hipSuccess;
+    Scratch sc;
+    float* dx_ = sc.get<float>(n, e);
+    float* dy_ = sc.get<float>(n, e);
+    float* dz_ = sc.get<float>(n, e);
+    unsigned* dh = sc.get<unsigned>(maxbin, e);
+    if (e != hipSuccess) return e;
+    if ((e = hipMemcpy(dx_, x, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dy_, y, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dz_, z, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemset(dh, 0, sizeof(unsigned) * maxbin)) != hipSuccess) return e;
+    const float fx = L * dx, fy = M * dx, fz = N * dx;
+    const float delr = (float)(1.0f / 25.);
+    const double rmin = 1.0f / sqrt(2.) * 0.5;
+    if (n > 1) {
+        const dim3 g((n + SB - 1) / SB, (n + GRJT * SB - 1) / (GRJT * SB));
+        if (maxbin <= GR_LDS_BINS)
+            hipLaunchKernelGGL(k_cvd_gr<true>, g, dim3(SB), sizeof(unsigned) * maxbin, 0, n, dx_, dy_, dz_, fx, fy,
+                               fz, delr, rmin, maxbin, dh);
+        else
+            hipLaunchKernelGGL(k_cvd_gr<false>, g, dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, delr, rmin, maxbin,
+                               dh);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipMemcpy(hist, dh, sizeof(unsigned) * maxbin, hipMemcpyDeviceToHost);
+}
+
+hipError_t post_cvd_q6(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
+                       unsigned* nb, float* ql, float* qglobal, unsigned long long* histq) {
+    hipError_t e = hipSuccess;
+    Scratch sc;
+    float* dx_ = sc.get<float>(n, e);
+    float* dy_ = sc.get<float>(n, e);
+    float* dz_ = sc.get<float>(n, e);
+    unsigned* dn = sc.get<unsigned>(n, e);
+    float* dth = sc.get<float>((size_t)n * Q6CN, e);
+    float* dfa = sc.get<float>((size_t)n * Q6CN, e);
+    double2* dq = sc.get<double2>((size_t)n * QM, e);
+    float* dql = sc.get<float>(n, e);
+    float* dg = sc.get<float>(1, e);
+    unsigned long long* dhq = sc.get<unsigned long long>(Q6NBIN, e);
+    if (e != hipSuccess) return e;
+    if ((e = hipMemcpy(dx_, x, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dy_, y, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(dz_, z, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemset(dhq, 0, sizeof(unsigned long long) * Q6NBIN)) != hipSuccess) return e;
+    const float fx = L * dx, fy = M * dx, fz = N * dx;
+    const double rhi = 1.0f * sqrt(3.) / 2. * 1.2 + 1e-3, rlo = 1.0f * sqrt(3.) / 2. * 0.9 - 1e-3;
+    hipLaunchKernelGGL(k_q6_nb, dim3((n + QB - 1) / QB), dim3(QB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, rlo, rhi, dn,
+                       dth, dfa);
+    hipLaunchKernelGGL(k_q6_local, dim3((n + SB - 1) / SB), dim3(SB), 0, 0, n, dn, dth, dfa, dq, dql, dhq);
+    hipLaunchKernelGGL(k_q6_global, dim3(1), dim3(64), 0, 0, n, dn, dq, dg);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpy(nb, dn, sizeof(unsigned) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+    if ((e = hipMemcpy(ql, dql, sizeof(float) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+    if ((e = hipMemcpy(qglobal, dg, sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess) return e;
+    return hipMemcpy(histq, dhq, sizeof(unsigned long long) * Q6NBIN, hipMemcpyDeviceToHost);
+}
+
+// ---------------------------------------------------------------------------
 // Misfit noise field (Gauss, misfit.cpp:951-981).  The reference draws, point
 // by point in memory order, Marsaglia polar pairs (v1, v2) = two successive
 // u11 draws of one splitmix64 stream until 0 < v1^2 + v2^2 < 1.  splitmix64's

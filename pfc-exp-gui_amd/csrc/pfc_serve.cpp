@@ -83,6 +83,12 @@ struct Bundle {
     std::vector<double> x, y, z, ph;
     std::vector<float> co, px, py, pz, pp;
     float phic = 0, pmax = 0;
+    // CVD GR histogram; Q6_GLO when kd % 20 == 0 (cvd.cpp:1253-1254)
+    std::vector<unsigned> hist, q6nb;
+    std::vector<float> q6l;
+    float q6g = 0;
+    unsigned long long q6h[30] = {};
+    bool has_q6 = false;
 };
 
 int next_event(const Job& j) {
@@ -166,6 +172,9 @@ void write_bundle(const Bundle& b) {
     } else {
         pfcio::write_cvd_atoms(b.kd, p.L, p.M, p.N, p.dx, b.phic, b.pmax, b.na, b.px.data(), b.py.data(),
                                b.pz.data(), b.pp.data(), j.outdir);
+        pfcio::write_cvd_gr(b.kd, p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.hist.data(), (int)b.hist.size(),
+                            j.outdir);
+        if (b.has_q6) pfcio::write_cvd_q6(b.kd, b.na, b.q6nb.data(), b.q6l.data(), b.q6g, b.q6h, j.outdir);
     }
     FILE* f = std::fopen((d + "energy.txt").c_str(), "a");
     if (f) {
@@ -226,6 +235,24 @@ bool capture(Batch& bt, int slot, Bundle& b, const double* e_all, std::string& e
         b.px.resize(P); b.py.resize(P); b.pz.resize(P); b.pp.resize(P);
         if (pfc_phimax_cvd(bt.ctx, slot, b.px.data(), b.py.data(), b.pz.data(), b.pp.data(), (int)P, &b.na, &b.phic,
                            &b.pmax)) {
+            err = pfc_last_error();
+            return false;
+        }
+        int nbin = 0;
+        bool ok = pfc_cvd_gr(p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.py.data(), b.pz.data(), nullptr, &nbin) == 0;
+        if (ok) {
+            b.hist.resize((size_t)nbin);
+            ok = pfc_cvd_gr(p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.py.data(), b.pz.data(), b.hist.data(),
+                            &nbin) == 0;
+        }
+        if (ok && b.kd >= 0 && b.kd % 20 == 0 && b.na > 0) {
+            b.q6nb.resize((size_t)b.na);
+            b.q6l.resize((size_t)b.na);
+            ok = pfc_cvd_q6(p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.py.data(), b.pz.data(), b.q6nb.data(),
+                            b.q6l.data(), &b.q6g, b.q6h) == 0;
+            b.has_q6 = ok;
+        }
+        if (!ok) {
             err = pfc_last_error();
             return false;
         }

@@ -251,6 +251,26 @@ void write_fields(pfc_ctx* ctx, int kd, const Opts& o, std::vector<float>& a, st
     pfcio::write_pvti("con_" + std::to_string(kd) + ".pvti", "con", "con_", kd, L, M, N);
 }
 
+// CVD order outputs of a checkpoint: GR every time, Q6_GLO when kd % 20 == 0
+// (cvd.cpp:1253-1254), over the atom fractions of pfc_phimax_cvd.
+int cvd_order_files(int kd, const pfc_params& p, int na, const float* px, const float* py, const float* pz,
+                    const std::string& dir) {
+    int nbin = 0;
+    if (pfc_cvd_gr(p.L, p.M, p.N, p.dx, na, px, py, pz, nullptr, &nbin)) return 1;
+    std::vector<unsigned> hist(static_cast<size_t>(nbin));
+    if (pfc_cvd_gr(p.L, p.M, p.N, p.dx, na, px, py, pz, hist.data(), &nbin)) return 1;
+    pfcio::write_cvd_gr(kd, p.L, p.M, p.N, p.dx, na, px, hist.data(), nbin, dir);
+    if (kd >= 0 && kd % 20 == 0 && na > 0) {
+        std::vector<unsigned> nb(static_cast<size_t>(na));
+        std::vector<float> ql(static_cast<size_t>(na));
+        float qg = 0.f;
+        unsigned long long hq[30];
+        if (pfc_cvd_q6(p.L, p.M, p.N, p.dx, na, px, py, pz, nb.data(), ql.data(), &qg, hq)) return 1;
+        pfcio::write_cvd_q6(kd, na, nb.data(), ql.data(), qg, hq, dir);
+    }
+    return 0;
+}
+
 float b0_of(const pfc_params& p) {
     if (p.dx > 0.0f && p.dt > 0.0f && p.axx >= 0.0f)
         return static_cast<float>(std::sqrt(static_cast<double>(p.axx / p.dx / p.dx / p.dx / p.dt)));
@@ -352,6 +372,7 @@ int run_sim(Kind k, int argc, char** argv) {
                     return gpu_fail("pfc_phimax_cvd");
                 pfcio::write_cvd_atoms(last, o.p.L, o.p.M, o.p.N, o.p.dx, phic, pmax, na, px.data(), py.data(), pz.data(),
                                        pp.data());
+                if (cvd_order_files(last, o.p, na, px.data(), py.data(), pz.data(), "")) return gpu_fail("pfc_cvd_gr/q6");
                 write_fields(ctx, last, o, fa, fb);
                 if (ck) { std::fprintf(ck, "step %d timestamp_ms %lld\n", last, pfcio::now_ms()); std::fflush(ck); }
             }
@@ -398,7 +419,7 @@ int run_elastic(int argc, char** argv) {
     const int L = o.p.L, M = o.p.M, N = o.p.N;
     const size_t P = static_cast<size_t>(L) * M * N;
     std::vector<float> phi(P);
-    std::vector<double> x(P), y(P), z(P), pm(P), st(P);
+    std::vector<double> x(P), y(P), z(P), pm(P), st(P), q4(P), q6(P);
     std::vector<unsigned> nb(P);
     for (size_t idx = 0; idx < steps.size(); idx++) {
         const int kd = steps[idx];
@@ -421,6 +442,10 @@ int run_elastic(int argc, char** argv) {
         if (pfc_strain(L, M, N, o.p.dx, o.benchel, n, x.data(), y.data(), z.data(), st.data(), nb.data()))
             return gpu_fail("pfc_strain");
         pfcio::write_strain("strain_" + std::to_string(kd) + ".txt", n, x.data(), y.data(), z.data(), st.data(), nb.data());
+        if (pfc_cryst_order(L, M, N, o.p.dx, n, x.data(), y.data(), z.data(), q4.data(), q6.data(), nullptr))
+            return gpu_fail("pfc_cryst_order");
+        pfcio::write_q4q6("q4q6_" + std::to_string(kd) + ".txt", kd, n, x.data(), y.data(), z.data(), q4.data(),
+                          q6.data());
     }
     return 0;
 }

@@ -118,6 +118,27 @@ bool write_strain(const std::string& path, int n, const double* x, const double*
     return std::fclose(f) == 0;
 }
 
+bool write_q4q6(const std::string& path, int kd, int n, const double* x, const double* y, const double* z,
+                const double* q4, const double* q6) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    int cnt = 0;
+    for (int i = 0; i < n; i++) cnt += x[i] >= 0;
+    std::fprintf(f, "%d\nKommentar\n", cnt);
+    double x0 = 1e30, x1 = -1e30, y0 = 1e30, y1 = -1e30;
+    for (int i = 0; i < n; i++) {
+        if (!(x[i] >= 0)) continue;
+        std::fprintf(f, "qc %.4f %.4f %.4f %.4f %.4f\n", x[i], y[i], z[i], q4[i], q6[i]);
+        if (x[i] < x0) x0 = x[i];
+        if (x[i] > x1) x1 = x[i];
+        if (y[i] < y0) y0 = y[i];
+        if (y[i] > y1) y1 = y[i];
+    }
+    std::fprintf(stderr, "q4q6_%d.txt: atoms=%d x=[%s,%s] y=[%s,%s]\n", kd, cnt, g6(x0).c_str(), g6(x1).c_str(),
+                 g6(y0).c_str(), g6(y1).c_str());
+    return std::fclose(f) == 0;
+}
+
 bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float phimax, int n, const float* px,
                      const float* py, const float* pz, const float* pphi, const std::string& dir) {
     const std::string pre = dir.empty() ? std::string() : dir + "/";
@@ -148,6 +169,67 @@ bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float ph
     std::fprintf(f, "%d\nKommentar\n", n);
     for (int i = 0; i < n; i++)
         std::fprintf(f, "%.4f %.4f %.4f %.4f\n", pz[i] * frame_x, px[i] * frame_y, py[i] * frame_z, pphi[i]);
+    return std::fclose(f) == 0;
+}
+
+// The reference runs on x86, where 0/0 is the negative default NaN and
+// iostream prints it as "-nan"; NaNs are printed that way whatever sign the
+// device produced.
+static void put_fixed(FILE* f, double v, int prec) {
+    if (std::isnan(v)) std::fputs("-nan", f);
+    else std::fprintf(f, "%.*f", prec, v);
+}
+
+bool write_cvd_gr(int kd, int L, int M, int N, float dx, int n, const float* px, const unsigned* hist, int maxbin,
+                  const std::string& dir) {
+    const std::string pre = dir.empty() ? std::string() : dir + "/";
+    const float delr = (float)(1.0f / 25.);
+    int num_t = 0;
+    for (int i = 0; i < n; i++) num_t += px[i] >= 0;
+    char name[64];
+    std::snprintf(name, sizeof name, "hist_%d.txt", kd);
+    FILE* f = std::fopen((pre + name).c_str(), "w");
+    if (!f) return false;
+    for (int b = 0; b < maxbin; b++) std::fprintf(f, "%d %u %d\n", b, hist[b], maxbin);
+    std::fclose(f);
+    const float rho = (float)num_t / (float)(L * M * N * dx * dx * dx);
+    const float preco = (float)(4. * 3.1415926 * rho / 3.);
+    std::snprintf(name, sizeof name, "Grq6_%d.txt", kd);
+    f = std::fopen((pre + name).c_str(), "w");
+    if (!f) return false;
+    for (int b = 0; b < maxbin; b++) {
+        const float rlower = (float)b * delr;
+        const float rupper = rlower + delr;
+        const float nideal = preco * (rupper * rupper * rupper - rlower * rlower * rlower);
+        const float gr = (float)hist[b] / (float)1 / (float)num_t / nideal;
+        std::fprintf(f, "%.4f ", b * delr);
+        put_fixed(f, gr, 4);
+        std::fprintf(f, " %u ", hist[b]);
+        put_fixed(f, nideal, 4);
+        std::fputc('\n', f);
+    }
+    return std::fclose(f) == 0;
+}
+
+bool write_cvd_q6(int kd, int n, const unsigned* nb, const float* ql, float qglobal, const unsigned long long* histq,
+                  const std::string& dir) {
+    const std::string pre = dir.empty() ? std::string() : dir + "/";
+    char name[64];
+    std::snprintf(name, sizeof name, "Ql_%d.txt", kd);
+    FILE* f = std::fopen((pre + name).c_str(), "w");
+    if (!f) return false;
+    for (int i = 0; i < n; i++) {
+        std::fprintf(f, "%d %u ", i, nb[i]);
+        put_fixed(f, ql[i], 6);
+        std::fputc(' ', f);
+        put_fixed(f, qglobal, 6);
+        std::fputc('\n', f);
+    }
+    std::fclose(f);
+    std::snprintf(name, sizeof name, "HistQ_%d.txt", kd);
+    f = std::fopen((pre + name).c_str(), "w");
+    if (!f) return false;
+    for (int b = 0; b < 30; b++) std::fprintf(f, "%.4f %llu\n", (b + 1) * 0.05f, histq[b]);
     return std::fclose(f) == 0;
 }
 

@@ -24,11 +24,22 @@ bool write_phimax_elastic(const std::string& path, int n, const double* x, const
 // strain_<kd>.txt (elastic.cpp:3738-3752).
 bool write_strain(const std::string& path, int n, const double* x, const double* y, const double* z,
                   const double* strain, const unsigned* nb);
+// q4q6_<kd>.txt of cryst_order (elastic.cpp:3328-3350): atoms with x >= 0,
+// "qc x y z q4 q6" with 4 decimals; plus its summary line on stderr.
+bool write_q4q6(const std::string& path, int kd, int n, const double* x, const double* y, const double* z,
+                const double* q4, const double* q6);
 // CVD checkpoint outputs of atompositionCFGC2 (cvd.cpp:1086-1249):
 // phimax.txt (append), phiball_<kd>.cfg and Phimax_<kd>.txt.
 // Files go to `dir` (empty: the working directory).
 bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float phimax, int n, const float* px,
                      const float* py, const float* pz, const float* pphi, const std::string& dir = "");
+// hist_<kd>.txt and Grq6_<kd>.txt of GR (cvd.cpp:1314-1339) from the pair
+// histogram; num_t counts the px >= 0 entries, nstep = 1.
+bool write_cvd_gr(int kd, int L, int M, int N, float dx, int n, const float* px, const unsigned* hist, int maxbin,
+                  const std::string& dir = "");
+// Ql_<kd>.txt and HistQ_<kd>.txt of Q6_GLO (cvd.cpp:1449-1510).
+bool write_cvd_q6(int kd, int n, const unsigned* nb, const float* ql, float qglobal, const unsigned long long* histq,
+                  const std::string& dir = "");
 
 // Default iostream formatting of a double (%g, 6 significant digits).
 std::string g6(double v);

@@ -24,7 +24,7 @@ SYMBOLS = [
     "pfc_set_active",
     "pfc_step", "pfc_sync", "pfc_get_fields", "pfc_get_fields_f32", "pfc_energy",
     "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic",
-    "pfc_strain", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
+    "pfc_strain", "pfc_cryst_order", "pfc_cvd_gr", "pfc_cvd_q6", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
     "pfc_device_memory", "pfc_copy_probe", "pfc_last_error",
     "pfc_server_create", "pfc_server_submit", "pfc_server_poll", "pfc_server_result", "pfc_server_error",
     "pfc_server_cancel", "pfc_server_drain", "pfc_server_stats", "pfc_server_destroy",
@@ -104,6 +104,13 @@ def load():
         "pfc_phimax_elastic": [ctypes.c_int] * 3 + [ctypes.c_float, fp, dp, dp, dp, dp, ctypes.c_int, ip],
         "pfc_strain": [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_float, ctypes.c_int, dp, dp, dp, dp,
                                             ctypes.POINTER(ctypes.c_uint)],
+        "pfc_cryst_order": [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int, dp, dp, dp, dp, dp,
+                                                 ctypes.POINTER(ctypes.c_uint)],
+        "pfc_cvd_gr": [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int, fp, fp, fp,
+                                            ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_int)],
+        "pfc_cvd_q6": [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int, fp, fp, fp,
+                                            ctypes.POINTER(ctypes.c_uint), fp, fp,
+                                            ctypes.POINTER(ctypes.c_ulonglong)],
         "pfc_set_graph": [vp, ctypes.c_int],
         "pfc_time_kernel": [vp, ctypes.c_int, ctypes.c_int],
         "pfc_kernel_time": [vp, dp, ip],
@@ -287,6 +294,47 @@ def strain(L, M, N, dx, benchel, x, y, z):
     _check(lib.pfc_strain(L, M, N, ctypes.c_float(dx), ctypes.c_float(benchel), n, _dp(x), _dp(y), _dp(z), _dp(s),
                           nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))))
     return s, nb
+
+
+def cryst_order(L, M, N, dx, x, y, z):
+    """Per-atom (q4, q6, neighbour count) of cryst_order."""
+    lib = load()
+    x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+    n = x.size
+    q4 = np.zeros(n, np.float64)
+    q6 = np.zeros(n, np.float64)
+    nb = np.zeros(n, np.uint32)
+    _check(lib.pfc_cryst_order(L, M, N, ctypes.c_float(dx), n, _dp(x), _dp(y), _dp(z), _dp(q4), _dp(q6),
+                               nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))))
+    return q4, q6, nb
+
+
+def cvd_gr(L, M, N, dx, x, y, z):
+    """GR pair histogram of CVD atom fractions (pfc_phimax_cvd's pc_x/y/z)."""
+    lib = load()
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    nbin = ctypes.c_int(0)
+    _check(lib.pfc_cvd_gr(L, M, N, ctypes.c_float(dx), x.size, _fp(x), _fp(y), _fp(z), None, ctypes.byref(nbin)))
+    hist = np.zeros(nbin.value, np.uint32)
+    _check(lib.pfc_cvd_gr(L, M, N, ctypes.c_float(dx), x.size, _fp(x), _fp(y), _fp(z),
+                          hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), ctypes.byref(nbin)))
+    return hist
+
+
+def cvd_q6(L, M, N, dx, x, y, z):
+    """Q6_GLO of CVD atom fractions: (neighbour counts, local Q6, global Q6,
+    30-bin histogram)."""
+    lib = load()
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    n = x.size
+    nb = np.zeros(n, np.uint32)
+    ql = np.zeros(n, np.float32)
+    qg = np.zeros(1, np.float32)
+    hq = np.zeros(30, np.uint64)
+    _check(lib.pfc_cvd_q6(L, M, N, ctypes.c_float(dx), n, _fp(x), _fp(y), _fp(z),
+                          nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), _fp(ql), _fp(qg),
+                          hq.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong))))
+    return nb, ql, float(qg[0]), hq
 
 
 class Server:

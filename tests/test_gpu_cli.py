@@ -84,6 +84,16 @@ def test_cvd_cli_matches_oracle(tmp_path):
         assert cg == co and np.abs(ag - ao).max() <= 2e-4
         assert (g / f"phiball_{kd}.cfg").read_text().splitlines()[:15] == \
             (o / f"phiball_{kd}.cfg").read_text().splitlines()[:15]
+    for kd in (0, 10, 20):   # GR every checkpoint: integer histogram, host-side normalisation
+        assert (g / f"hist_{kd}.txt").read_bytes() == (o / f"hist_{kd}.txt").read_bytes()
+        assert (g / f"Grq6_{kd}.txt").read_bytes() == (o / f"Grq6_{kd}.txt").read_bytes()
+    for kd in (0, 20):       # Q6_GLO when kd % 20 == 0
+        qg, qo = np.loadtxt(g / f"Ql_{kd}.txt", ndmin=2), np.loadtxt(o / f"Ql_{kd}.txt", ndmin=2)
+        assert qg.shape == qo.shape and np.array_equal(qg[:, :2], qo[:, :2])
+        assert np.abs(qg[:, 2:] - qo[:, 2:]).max() <= 3e-6
+        hg, ho = np.loadtxt(g / f"HistQ_{kd}.txt"), np.loadtxt(o / f"HistQ_{kd}.txt")
+        assert np.array_equal(hg[:, 0], ho[:, 0]) and np.abs(hg[:, 1] - ho[:, 1]).sum() <= 2
+    assert not (g / "Ql_10.txt").exists()
     pg = (g / "phimax.txt").read_text().splitlines()
     po = (o / "phimax.txt").read_text().splitlines()
     assert len(pg) == len(po) == 18
@@ -103,6 +113,14 @@ def test_elastic_cli_bit_identical(tmp_path):
         for f in (f"Phimax_{kd}.txt", f"strain_{kd}.txt"):
             assert (g / f).read_bytes() == (o / f).read_bytes(), f
     assert not (g / "Phimax_0.txt").exists()
+    for kd in (20, 40):
+        a, b = (g / f"q4q6_{kd}.txt").read_text().splitlines(), (o / f"q4q6_{kd}.txt").read_text().splitlines()
+        assert a[:2] == b[:2] and len(a) == len(b) > 2
+        ta = np.array([[float(t) for t in l.split()[1:]] for l in a[2:]])
+        tb = np.array([[float(t) for t in l.split()[1:]] for l in b[2:]])
+        assert np.array_equal(ta[:, :3], tb[:, :3])
+        assert np.abs(ta[:, 3:] - tb[:, 3:]).max() <= 1.5e-4   # one unit of the 4th decimal
+        assert f"q4q6_{kd}.txt: atoms={a[0]}" in r.stderr
 
 
 def test_fp32_cli_runs(tmp_path):

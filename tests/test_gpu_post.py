@@ -58,6 +58,96 @@ def test_phimax_elastic_and_strain(hip, oracle):
     assert np.array_equal(s, os_)
 
 
+def test_cryst_order(hip, oracle):
+    """q4/q6 of the atoms of a grown crystal: neighbour lists and counts exact,
+    q within float transcendental round-off (device vs host cosf/sinf)."""
+    p, sim = _state(hip, 0, 0, L=128, M=128, steps=60, grainx=4, grainy=4)
+    phi, _ = sim.get_fields()
+    g = hip.phimax_elastic(p.L, p.M, 1, p.dx, phi.astype(np.float32))
+    q4, q6, nb = hip.cryst_order(p.L, p.M, 1, p.dx, g[0], g[1], g[2])
+    o4, o6, onb = oracle.cryst_order(p.L, p.M, 1, p.dx, g[0], g[1], g[2])
+    d = max(np.abs(q4 - o4).max(), np.abs(q6 - o6).max())
+    print(f"cryst_order atoms={len(q4)} max|dq|={d:.3e} nb range {nb.min()}..{nb.max()}", flush=True)
+    assert len(q4) > 100
+    assert np.array_equal(nb, onb)
+    assert d < 1e-5
+    assert np.allclose(q6, np.sqrt(2.0) * q4, rtol=1e-6)
+
+
+def test_cryst_order_3d_lattice(hip, oracle):
+    """A periodic fcc lattice (nearest distance 1, 12 neighbours) in a 3-D
+    box, jittered: every atom ordered, GPU = oracle."""
+    rng = np.random.default_rng(3)
+    a, cells, dx = np.sqrt(2.0), 6, np.float32(np.sqrt(2.0) / 8)
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    pts = np.array([(np.array(c) + b) * a for c in np.ndindex(cells, cells, cells) for b in base])
+    pts += 0.01 * rng.standard_normal(pts.shape)
+    L = M = N = cells * 8
+    q4, q6, nb = hip.cryst_order(L, M, N, float(dx), pts[:, 0], pts[:, 1], pts[:, 2])
+    o4, o6, onb = oracle.cryst_order(L, M, N, float(dx), pts[:, 0], pts[:, 1], pts[:, 2])
+    assert np.array_equal(nb, onb) and (nb == 12).all()
+    assert max(np.abs(q4 - o4).max(), np.abs(q6 - o6).max()) < 1e-5
+    assert np.all(q4 > 0.5)
+
+
+def _q6_close(g, o, tol=1e-5):
+    nb, ql, qg, hq = g
+    onb, oql, oqg, ohq = o
+    assert np.array_equal(nb, onb)
+    assert np.abs(ql - oql).max() < tol and abs(qg - oqg) < tol
+    # a local value within tol of a bin edge (multiples of 0.05) may change bin
+    edge = np.abs(oql / 0.05 - np.round(oql / 0.05)) * 0.05 < tol
+    assert np.abs(hq.astype(np.int64) - ohq.astype(np.int64)).sum() <= 2 * edge.sum()
+
+
+def test_cvd_gr_q6_field(hip, oracle):
+    """GR and Q6_GLO of the atoms of a grown 3-D CVD field: histogram exact,
+    neighbour counts exact, Q6 within float transcendental round-off."""
+    p = hip.default_params(model=1, L=32, M=32, N=16, precision=hip.FP64)
+    phi, con = hip.init_fields(p)
+    sim = hip.Sim(p)
+    sim.set_fields(phi, con)
+    sim.step(20)
+    px, py, pz, pp, phic, pmax = sim.phimax_cvd()
+    print(f"cvd atoms={len(px)}", flush=True)
+    h = hip.cvd_gr(p.L, p.M, p.N, p.dx, px, py, pz)
+    oh = oracle.cvd_gr(p.L, p.M, p.N, p.dx, px, py, pz)
+    assert len(h) == len(oh) and h.sum() > 0
+    assert np.array_equal(h, oh)
+    _q6_close(hip.cvd_q6(p.L, p.M, p.N, p.dx, px, py, pz), oracle.cvd_q6(p.L, p.M, p.N, p.dx, px, py, pz))
+
+
+@pytest.mark.parametrize("L,N,n", [(64, 64, 4000), (4096, 16, 3000)])
+def test_cvd_gr_q6_sizes(hip, oracle, L, N, n):
+    """Many i/j tiles (the triangle's block skip), and a histogram too large
+    for LDS (L*dx/delr > 12288 bins: global atomics)."""
+    rng = np.random.default_rng(L)
+    M, dx = 64, 0.125
+    f = rng.random((n, 3)).astype(np.float32)
+    f[:, 0] *= 64.0 / L                   # keep the atoms dense along x
+    h = hip.cvd_gr(L, M, N, dx, f[:, 0], f[:, 1], f[:, 2])
+    oh = oracle.cvd_gr(L, M, N, dx, f[:, 0], f[:, 1], f[:, 2])
+    print(f"L={L} n={n} bins={len(h)} pairs={h.sum() // 2}", flush=True)
+    assert np.array_equal(h, oh)
+    _q6_close(hip.cvd_q6(L, M, N, dx, f[:, 0], f[:, 1], f[:, 2]), oracle.cvd_q6(L, M, N, dx, f[:, 0], f[:, 1], f[:, 2]))
+
+
+def test_cvd_q6_lattice(hip, oracle):
+    """Jittered fcc (12 neighbours in the window): GPU = oracle, local and
+    global Q6 near the fcc value 0.5745."""
+    rng = np.random.default_rng(4)
+    a, cells, L = 0.9 * 2 ** 0.5, 6, 48
+    dx = float(np.float32(cells * a / L))
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    pts = np.array([(np.array(c) + b) * a for c in np.ndindex(cells, cells, cells) for b in base])
+    pts += 0.01 * rng.standard_normal(pts.shape) + 0.05
+    f = (pts / np.float32(L * np.float32(dx))).astype(np.float32)
+    g = hip.cvd_q6(L, L, L, dx, f[:, 0], f[:, 1], f[:, 2])
+    o = oracle.cvd_q6(L, L, L, dx, f[:, 0], f[:, 1], f[:, 2])
+    _q6_close(g, o)
+    assert (g[0] == 12).all() and abs(g[2] - 0.5745) < 0.02
+
+
 def test_elastic_3d_field(hip, oracle):
     """N > 1 exercises the z stencil and its wrap arithmetic."""
     rng = np.random.default_rng(7)

@@ -96,5 +96,7 @@ def test_outdir_files_and_cancel(hip, tmp_path):
     vals = np.frombuffer(raw[start:start + 4 * phi.size], dtype=np.float32).reshape(p.M, p.L)  # [m][l]
     assert np.array_equal(vals, phi.reshape(p.L, p.M).T.astype(np.float32))
     cnames = set(os.listdir(tmp_path / "cvd"))
-    assert {"phimax.txt", "phiball_5.cfg", "Phimax_12.txt", "phi_10_0.vti"} <= cnames
+    assert {"phimax.txt", "phiball_5.cfg", "Phimax_12.txt", "phi_10_0.vti", "hist_5.txt", "Grq6_12.txt"} <= cnames
+    assert not any(n.startswith("Ql_") for n in cnames)  # Q6_GLO only when kd % 20 == 0
+    assert len(open(tmp_path / "cvd" / "hist_10.txt").read().splitlines()) == int(np.float32(32) * np.float32(q.dx) / np.float32(0.04))
     srv.close()

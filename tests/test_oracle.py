@@ -412,3 +412,258 @@ def test_vti_roundtrip_and_layout(oracle, tmp_path):
     back = np.zeros(L * M * N, np.float32)
     n = lib.pfo_read_vti(p, L, M, N, back.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
     assert n == L * M * N and np.array_equal(back, f.astype(np.float32))
+
+
+# ---------------------------------------------------------------------------
+# cryst_order (elastic.cpp:3013-3389): Lechner-Dellago averaged q_l.
+# Pinned against textbook Steinhardt values computed here with scipy's
+# spherical harmonics on perfect lattices (every atom equivalent, so the
+# neighbour average equals the atom's own q_lm).  The reference's "q4"
+# column is l = 6 (:3155) and its q6 column keeps adding onto the same
+# per-atom sum, i.e. sqrt(2) times it.
+# ---------------------------------------------------------------------------
+def _textbook_q(l, vecs):
+    """Steinhardt q_l of one neighbour shell, with the reference's angles:
+    polar = acos(dz/(r + 1e-4)), azimuth = atan(dy/(dx + 1e-4)) (+pi for
+    dx < 0) (elastic.cpp:3099-3104)."""
+    import scipy.special as sps
+    r = np.linalg.norm(vecs, axis=1)
+    polar = np.arccos(vecs[:, 2] / (r + 1e-4))
+    azim = np.arctan(vecs[:, 1] / (vecs[:, 0] + 1e-4)) + np.where(vecs[:, 0] < 0, 3.1415926, 0.0)
+    tot = 0.0
+    for m in range(-l, l + 1):
+        if hasattr(sps, "sph_harm_y"):
+            y = sps.sph_harm_y(l, m, polar, azim)
+        else:
+            y = sps.sph_harm(m, l, azim, polar)
+        tot += abs(y.mean()) ** 2
+    return np.sqrt(4 * np.pi / (2 * l + 1) * tot)
+
+
+def _lattice(kind, cells, a, off=0.137):
+    basis = {"sc": [(0, 0, 0)], "bcc": [(0, 0, 0), (0.5, 0.5, 0.5)],
+             "fcc": [(0, 0, 0), (0.5, 0.5, 0), (0.5, 0, 0.5), (0, 0.5, 0.5)]}[kind]
+    pts = [((i + bx) * a + off, (j + by) * a + off, (k + bz) * a + off)
+           for i in range(cells) for j in range(cells) for k in range(cells) for bx, by, bz in basis]
+    return np.array(pts)
+
+
+@pytest.mark.parametrize("kind,a,nn", [("sc", 1.0, 6), ("bcc", 1.3, 8), ("fcc", 2 ** 0.5, 12)])
+def test_cryst_order_lattices(oracle, kind, a, nn):
+    cells, L = 4, 16
+    dx = float(np.float32(cells * a / L))  # box = L*dx = cells*a
+    pts = _lattice(kind, cells, a)
+    q4, q6, nb = oracle.cryst_order(L, L, L, dx, pts[:, 0], pts[:, 1], pts[:, 2])
+    first = {"sc": [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)],
+             "bcc": [(sx * .5, sy * .5, sz * .5) for sx in (1, -1) for sy in (1, -1) for sz in (1, -1)],
+             "fcc": [v for v in [(sx * .5, sy * .5, 0) for sx in (1, -1) for sy in (1, -1)] +
+                     [(sx * .5, 0, sz * .5) for sx in (1, -1) for sz in (1, -1)] +
+                     [(0, sy * .5, sz * .5) for sy in (1, -1) for sz in (1, -1)]]}[kind]
+    want = _textbook_q(6, np.array(first) * a)
+    assert np.all(nb == nn)
+    assert np.abs(q4 - want).max() < 2e-5, (q4[:3], want)
+    assert np.abs(q6 - np.sqrt(2.0) * q4).max() < 1e-6
+
+
+def test_cryst_order_cap_and_isolated(oracle):
+    # a dense blob: every atom has > 16 neighbours within 1.2 -> q = 0;
+    # one far atom without neighbours -> q = 0 and Nb = 0.
+    g = np.linspace(0.0, 0.9, 3)
+    blob = np.array([(x + 2, y + 2, z + 2) for x in g for y in g for z in g])
+    pts = np.vstack([blob, [[6.0, 6.0, 6.0]]])
+    q4, q6, nb = oracle.cryst_order(32, 32, 32, 0.25, pts[:, 0], pts[:, 1], pts[:, 2])
+    assert nb[-1] == 0 and q4[-1] == 0 and q6[-1] == 0
+    assert nb[13] == 26 and q4[13] == 0 and q6[13] == 0  # the blob's centre
+
+
+def test_ylm_reference_conventions(oracle):
+    """The reference's Ylm: the Condon-Shortley phase twice (in P_l^m and in
+    the (-1)^m normalisation), so (-1)^m times the textbook Y_lm for m >= 0;
+    negative m reuses |m| in the real part and drops the (-1)^m sign in the
+    imaginary part.  |Y|^2 and hence q_l are unaffected."""
+    import scipy.special as sps
+    th, fa = 0.7, 1.1
+    for m in range(0, 7):
+        yr, yc = oracle.ylm(6, m, th, fa)
+        if hasattr(sps, "sph_harm_y"):
+            t = sps.sph_harm_y(6, m, th, fa)
+        else:
+            t = sps.sph_harm(m, 6, fa, th)
+        sgn = (-1) ** m
+        assert abs(yr - sgn * t.real) < 2e-6 and abs(yc - sgn * t.imag) < 2e-6, (m, yr, yc, t)
+        yr2, yc2 = oracle.ylm(6, -m, th, fa)
+        assert yr2 == yr
+        assert yc2 == pytest.approx(-(-1) ** m * yc, abs=1e-7)
+
+
+# ---------------------------------------------------------------------------
+# CVD GR / Q6_GLO (cvd.cpp:1277-1510)
+# ---------------------------------------------------------------------------
+def _frac_lattice(kind, cells, a, box, off=0.01):
+    return (_lattice(kind, cells, a, off=off * a) / box).astype(np.float32)
+
+
+def test_cvd_gr_sc_shells(oracle):
+    """Simple cubic, spacing 1: the histogram holds n times the coordination
+    number of each shell (6, 12, 8, 6, 24, 24 at 1, sqrt2, sqrt3, 2, sqrt5,
+    sqrt6), 2 per pair; nothing inside a/sqrt(2)/2."""
+    L, dx = 64, 0.125                       # box 8 = 8 cells
+    f = _frac_lattice("sc", 8, 1.0, 8.0)
+    h = oracle.cvd_gr(L, L, L, dx, f[:, 0], f[:, 1], f[:, 2])
+    n = len(f)
+    assert len(h) == int(np.float32(L * dx) / np.float32(0.04)) == 200
+    want = {int(np.sqrt(d2) / 0.04): c for d2, c in ((1, 6), (2, 12), (3, 8), (4, 6), (5, 24), (6, 24))}
+    for b, c in want.items():
+        assert h[b] == c * n, (b, h[b], c * n)
+    assert h[:25].sum() == 0
+
+
+def test_cvd_gr_random_bruteforce(oracle):
+    """Random fractions against a float64 minimum-image pair count: bins may
+    differ only by pairs within 1e-4 of a bin edge."""
+    rng = np.random.default_rng(5)
+    L, M, N, dx = 48, 48, 16, 0.125
+    f = rng.random((300, 3)).astype(np.float32)
+    h = oracle.cvd_gr(L, M, N, dx, f[:, 0], f[:, 1], f[:, 2]).astype(np.int64)
+    fr = np.array([L, M, N], np.float64) * dx
+    d = (f[:, None, :].astype(np.float64) - f[None, :, :]) * fr
+    d = np.minimum(np.abs(d), np.minimum(np.abs(d + fr), np.abs(d - fr)))
+    r = np.sqrt((d ** 2).sum(-1)) + 1e-6
+    iu = np.triu_indices(len(f), 1)
+    r = r[iu]
+    b = np.floor(r / 0.04).astype(np.int64)
+    keep = (r > 1 / np.sqrt(2) / 2) & (b < len(h))
+    want = 2 * np.bincount(b[keep], minlength=len(h))[:len(h)]
+    edge = np.abs(r / 0.04 - np.round(r / 0.04)) < 1e-4
+    assert np.abs(h - want).sum() <= 4 * edge.sum()
+    assert h.sum() > 0
+
+
+def test_cvd_gr_files(oracle, tmp_path):
+    """Grq6: nideal telescopes to 4/3 pi rho R^3 and gr * nideal * num_t
+    gives back the histogram."""
+    rng = np.random.default_rng(6)
+    L, dx = 64, 0.125
+    f = rng.random((200, 3)).astype(np.float32)
+    h = oracle.cvd_gr(L, L, L, dx, f[:, 0], f[:, 1], f[:, 2])
+    lib = oracle.load()
+    import ctypes
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        lib.pfo_write_cvd_gr(40, L, L, L, ctypes.c_float(dx), len(f),
+                             f[:, 0].copy().ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                             h.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)))
+    finally:
+        os.chdir(cwd)
+    hist = np.loadtxt(tmp_path / "hist_40.txt")
+    assert hist.shape == (200, 3) and np.array_equal(hist[:, 1], h) and (hist[:, 2] == 200).all()
+    g = np.loadtxt(tmp_path / "Grq6_40.txt")
+    rho = 200 / 8.0 ** 3
+    assert abs(g[:, 3].sum() - 4 / 3 * np.pi * rho * (200 * 0.04) ** 3) < 1e-2
+    # gr * nideal * num_t = hist, up to the 4-decimal rounding of gr and nideal
+    err = np.abs(g[:, 1] * g[:, 3] * 200 - g[:, 2])
+    assert (err <= 200 * 5e-5 * (g[:, 1] + g[:, 3] + 1e-4) + 1e-9).all()
+    assert np.allclose(g[:, 0], np.arange(200) * 0.04, atol=6e-5)
+
+
+def _textbook_q6_glo(vecs_list):
+    """Local Steinhardt q6 per atom and the bond-weighted global Q6 with the
+    angles of Q6_GLO: polar acos(dz/(r + 1e-6)), azimuth atan(dy/dx) without
+    the quadrant term, 0 when dy == 0 (cvd.cpp:1414-1422)."""
+    import scipy.special as sps
+
+    def y6(v):
+        r = np.linalg.norm(v, axis=1) + 1e-6
+        polar = np.arccos(v[:, 2] / r)
+        with np.errstate(divide="ignore"):
+            azim = np.where(v[:, 1] == 0, 0.0, np.arctan(v[:, 1] / v[:, 0]))
+        out = []
+        for m in range(-6, 7):
+            out.append(sps.sph_harm_y(6, m, polar, azim) if hasattr(sps, "sph_harm_y")
+                       else sps.sph_harm(m, 6, azim, polar))
+        return np.array(out)            # [13, nb]
+
+    loc, tot, nbond = [], np.zeros(13, complex), 0
+    for v in vecs_list:
+        y = y6(v)
+        loc.append(np.sqrt(4 * np.pi / 13 * (np.abs(y.mean(1)) ** 2).sum()))
+        tot += y.sum(1)
+        nbond += len(v)
+    return np.array(loc), np.sqrt(4 * np.pi / 13 * (np.abs(tot / nbond) ** 2).sum())
+
+
+@pytest.mark.parametrize("kind,a,nn", [("sc", 0.9, 6), ("bcc", 1.0, 14), ("fcc", 0.9 * 2 ** 0.5, 12)])
+def test_cvd_q6_lattices(oracle, kind, a, nn):
+    """Perfect lattices in a periodic box: the window a*sqrt(3)/2*[0.9,1.2]
+    takes 6 (sc), 8+6 (bcc, a = 1) and 12 (fcc) neighbours; local and global
+    Q6 equal the textbook values (0.3536, 0.5107, 0.5745)."""
+    cells = 4
+    L = 32
+    box = cells * a
+    dx = float(np.float32(box / L))
+    f = _frac_lattice(kind, cells, a, L * np.float32(dx))
+    nb, ql, qg, hq = oracle.cvd_q6(L, L, L, dx, f[:, 0], f[:, 1], f[:, 2])
+    want = {"sc": 0.35355, "bcc": 0.51069, "fcc": 0.57452}[kind]
+    assert (nb == nn).all()
+    assert np.abs(ql - want).max() < 5e-5 and abs(qg - want) < 5e-5
+    assert hq[int(want / 0.05)] == len(f) and hq.sum() == len(f)
+
+
+def test_cvd_q6_jitter_textbook(oracle):
+    """Jittered fcc: per-atom local Q6 and the global Q6 against scipy's
+    Y_6m over the neighbours found by a float64 brute-force search."""
+    rng = np.random.default_rng(11)
+    a, cells, L = 0.9 * 2 ** 0.5, 4, 32
+    box = cells * a
+    dx = float(np.float32(box / L))
+    pts = _lattice("fcc", cells, a, off=0.05) + 0.03 * rng.standard_normal((4 * cells ** 3, 3))
+    fb = np.float32(L * np.float32(dx))
+    f = (pts / fb).astype(np.float32)
+    nb, ql, qg, hq = oracle.cvd_q6(L, L, L, dx, f[:, 0], f[:, 1], f[:, 2])
+    p = f.astype(np.float64) * float(fb)
+    vecs = []
+    for i in range(len(p)):
+        d = p - p[i]
+        d = d - float(fb) * np.round(d / float(fb))
+        r = np.sqrt((d ** 2).sum(1)) + 1e-6
+        sel = (r <= np.sqrt(3) / 2 * 1.2 + 1e-3) & (r >= np.sqrt(3) / 2 * 0.9 - 1e-3)
+        sel[i] = False
+        vecs.append(d[sel])
+    assert np.array_equal(nb, [len(v) for v in vecs])
+    loc, glob = _textbook_q6_glo(vecs)
+    assert np.abs(ql - loc).max() < 2e-5
+    assert abs(qg - glob) < 2e-5
+    assert 0.3 < ql.mean() < 0.58
+
+
+def test_cvd_q6_cap(oracle):
+    """More than 24 neighbours: local Q6 = 0, yet the count enters the global
+    normalisation sum_i Nb_i."""
+    import scipy.special as sps
+    rng = np.random.default_rng(2)
+    L, dx = 32, 0.25
+    fb = L * dx
+    p = rng.random((400, 3)) * 3.0 + 2.0
+    f = (p / fb).astype(np.float32)
+    nb, ql, qg, hq = oracle.cvd_q6(L, L, L, dx, f[:, 0], f[:, 1], f[:, 2])
+    assert nb.max() > 24 and nb.min() <= 24
+    assert (ql[nb > 24] == 0).all() and (ql[(nb > 0) & (nb <= 24)] > 0).all()
+    p = f.astype(np.float64) * fb
+    tot = np.zeros(13, complex)
+    for i in range(len(p)):
+        if not 0 < nb[i] <= 24:
+            continue
+        d = p - p[i]
+        r = np.sqrt((d ** 2).sum(1)) + 1e-6
+        sel = (r <= np.sqrt(3) / 2 * 1.2 + 1e-3) & (r >= np.sqrt(3) / 2 * 0.9 - 1e-3)
+        sel[i] = False
+        v = d[sel]
+        polar = np.arccos(v[:, 2] / r[sel])
+        azim = np.where(v[:, 1] == 0, 0.0, np.arctan(v[:, 1] / v[:, 0]))
+        for k, m in enumerate(range(-6, 7)):
+            y = (sps.sph_harm_y(6, m, polar, azim) if hasattr(sps, "sph_harm_y")
+                 else sps.sph_harm(m, 6, azim, polar))
+            tot[k] += y.sum()
+    want = np.sqrt(4 * np.pi / 13 * (np.abs(tot / nb.astype(np.float64).sum()) ** 2).sum())
+    assert abs(qg - want) < 2e-5, (qg, want)
