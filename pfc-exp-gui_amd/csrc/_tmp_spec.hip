@@ -1,0 +1,1041 @@
+// pfc_step.hip - fused spectral step kernels for the Misfit / CVD binary PFC
+// models on gfx950.
+//
+// The reference step (misfit.cpp:477-498, cvd.cpp:400-419) performs eight
+// state-changing c2c FFTs over full complex-double arrays plus five
+// elementwise sweeps.  Here a step is two kernels over half spectra:
+//
+//   row pass    per row l: c2r along m of S and AB -> psi, con, a, b in
+//               registers -> in1+in2+in3 and inc (UPDATEINN3, UPDATEINFMIX,
+//               UPDATEINFCMIX second half, UPDATEINC2MIX, UPDATEINC2MIXC)
+//               -> r2c along m -> IN.
+//   column pass per column tile: FFT along l of S and IN -> semi-implicit
+//               update (MAINEQUATIONC2_NOI with b0 = 0 / MAINEQUATIONC2MIX)
+//               -> inverse FFT along l of the new state -> S, and of
+//               C2A/C2B times the new psi_hat -> AB (UPDATEINFCMIX first half).
+//
+// Both passes stream each array through HBM once; all transforms and the
+// pointwise algebra happen in LDS and VGPRs.
+#include <hip/hip_ext.h>
+
+#include "pfc_fft.hpp"
+#include "pfc_kernels.h"
+
+// Launch; with a start/stop event pair the events take the kernel's own
+// start and completion times (hipExtLaunchKernelGGL).
+#define PFC_LAUNCH(kern, grid, block, s, t0, t1, ...)                                  \
+    do {                                                                               \
+        if (t0) hipExtLaunchKernelGGL(kern, grid, block, 0, s, t0, t1, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(kern, grid, block, 0, s, __VA_ARGS__);                 \
+    } while (0)
+
+namespace pfc {
+
+// Accesses to workgroup-uniform arrays: a uniform base pointer plus a 32-bit
+// per-thread byte offset and a 32-bit uniform byte offset (per unrolled
+// element), so no 64-bit index arithmetic is done per element.
+struct Buf {
+    char* p;
+};
+__device__ __forceinline__ Buf mkbuf(const void* base, size_t) { return Buf{(char*)const_cast<void*>(base)}; }
+template <typename T>
+__device__ __forceinline__ T bload(const Buf& b, uint32_t voff, uint32_t soff) {
+    return *(const T*)(b.p + soff + voff);
+}
+template <typename T>
+__device__ __forceinline__ void bstore(const T& val, const Buf& b, uint32_t voff, uint32_t soff) {
+    *(T*)(b.p + soff + voff) = val;
+}
+
+// ---------------------------------------------------------------------------
+// Pointwise physics
+// ---------------------------------------------------------------------------
+
+// Nonlinear terms at one point (misfit.cpp:777, 792, 826, 840, 858-867).
+// Returns inn, inc and in3.
+__device__ __forceinline__ void nonlinear(const PhysArgs& p, double ph, double c, double a, double b,
+                                          double& inn, double& inc, double& in3) {
+#pragma clang fp contract(off)
+    const double c0 = (double)p.c0;
+    const double lc = log(c / c0);
+    const double lc1 = log((1.0 - c) / (1.0 - c0));
+    const double ph2 = ph * ph;
+    const double in1 = -(double)p.niu * ph2 / 2.0 + (double)p.ksi * (ph2 * ph) / 3.0;
+    const double in2 = (double)p.w * (c * lc + (1.0 - c) * lc1);
+    const double cc = c * c;
+    const double cm = 1.0 - c;
+    const double h1 = 1.0 - 3.0 * cc + 2.0 * (cc * c);
+    const double h2 = 1.0 - 3.0 * (cm * cm) + 2.0 * (cm * cm * cm);
+    in3 = -h1 * a - h2 * b;
+    inn = in1 + in2 + in3;
+    const double wp = (double)p.w * (ph + 1.0);
+    const double t1 = wp * lc;
+    const double t2 = wp * lc1;
+    const double t3 = 3.0 * ph * (a - b) * cc;
+    const double t4 = 3.0 * ph * (-1.0 * a + b) * c;
+    if (p.narrow)
+        inc = (double)((((float)t1 - (float)t2) - (float)t3) - (float)t4);
+    else
+        inc = t1 - t2 - t3 - t4;
+}
+
+__device__ __forceinline__ void nonlinear(const PhysArgs& p, float ph, float c, float a, float b, float& inn,
+                                          float& inc, float& in3) {
+    const float c0 = p.c0;
+    const float lc = logf(c / c0);
+    const float lc1 = logf((1.0f - c) / (1.0f - c0));
+    const float ph2 = ph * ph;
+    const float in1 = -p.niu * ph2 * 0.5f + p.ksi * (ph2 * ph) * (1.0f / 3.0f);
+    const float in2 = p.w * (c * lc + (1.0f - c) * lc1);
+    const float cc = c * c;
+    const float cm = 1.0f - c;
+    const float h1 = 1.0f - 3.0f * cc + 2.0f * (cc * c);
+    const float h2 = 1.0f - 3.0f * (cm * cm) + 2.0f * (cm * cm * cm);
+    in3 = -h1 * a - h2 * b;
+    inn = in1 + in2 + in3;
+    const float wp = p.w * (ph + 1.0f);
+    inc = wp * lc - wp * lc1 - 3.0f * ph * (a - b) * cc - 3.0f * ph * (b - a) * c;
+}
+
+// Per-point energy without the gradient term (ENERGYC2, misfit.cpp:1054-1058);
+// the alpha*grac term is summed in k space by the column pass (Parseval).
+template <typename R>
+__device__ __forceinline__ double energy_point(const PhysArgs& p, R phr, R cr, R in3r) {
+#pragma clang fp contract(off)
+    const double ph = (double)phr, c = (double)cr, in3 = (double)in3r;
+    const double c0 = (double)p.c0;
+    const double ph2 = ph * ph;
+    const double eid = ph2 / 2.0 - (double)p.niu * (ph2 * ph) / 6.0 + (double)p.ksi * (ph2 * ph2) / 12.0;
+    const double emix = (ph + 1.0) * (double)p.w * (c * log(c / c0) + (1.0 - c) * log((1.0 - c) / (1.0 - c0)));
+    const double eex = 0.5 * ph * in3;
+    if (p.narrow) return (double)((float)eid + (float)emix + (float)eex);
+    return eid + emix + eex;
+}
+
+// Noise term of MAINEQUATIONC2_NOI for one mode (misfit.cpp:1002-1003):
+// dt*b0*noisf*sqrt(ks)*sqrt(|0.5 +- 0.5 sel|)*(0.5 - 0.5 tanh((sqrt(ks) - 2 pi)/0.1)),
+// sel = 1 on the four self-conjugate bins (l in {0, L/2}, m in {0, M/2}).
+__device__ __forceinline__ Cx<double> noise_term(const PhysArgs& p, float ks, bool sel, Cx<double> nz) {
+#pragma clang fp contract(off)
+    const float sks = sqrtf(ks);
+    const double damp = 0.5 - 0.5 * tanh(((double)sks - 2.0 * 3.1415926) / 0.1);
+    const double s = sel ? 1.0 : 0.0;
+    const double a = (double)p.db * nz.x * (double)sks * sqrt(fabs(0.5 + 0.5 * s)) * damp;
+    const double b = (double)p.db * nz.y * (double)sks * sqrt(fabs(0.5 - 0.5 * s)) * damp;
+    return Cx<double>{a, b};
+}
+
+__device__ __forceinline__ Cx<float> noise_term(const PhysArgs& p, float ks, bool sel, Cx<float> nz) {
+    const float sks = sqrtf(ks);
+    const float damp = 0.5f - 0.5f * tanhf((sks - 6.2831852f) * 10.0f);
+    const float ca = sel ? 1.0f : 0.70710678f, cb = sel ? 0.0f : 0.70710678f;
+    return Cx<float>{p.db * nz.x * sks * ca * damp, p.db * nz.y * sks * cb * damp};
+}
+
+// Semi-implicit update of one mode (misfit.cpp:1002-1010, cvd.cpp:872-880);
+// nt is the noise term (zero when b0 = 0).
+__device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<double, 2>& s,
+                                            const Vec<double, 2>& o, Cx<double> nt) {
+#pragma clang fp contract(off)
+    const float mk = p.mn_dt * ks;
+    double r1 = s.c[0].x - (double)mk * o.c[0].x + nt.x;
+    double r2 = s.c[0].y - (double)mk * o.c[0].y + nt.y;
+    const float mkc = p.mc_dt * ks;
+    const float den = (p.mc_dt_a * ks) * ks;
+    double r3 = s.c[1].x - (double)mkc * o.c[1].x;
+    double r4 = s.c[1].y - (double)mkc * o.c[1].y;
+    if (p.narrow) {
+        r1 = (double)(float)r1; r2 = (double)(float)r2;
+        r3 = (double)(float)r3; r4 = (double)(float)r4;
+    }
+    const double d1 = 1.0 + (double)mk;
+    const double d2 = 1.0 + (double)den;
+    s.c[0].x = r1 / d1; s.c[0].y = r2 / d1;
+    s.c[1].x = r3 / d2; s.c[1].y = r4 / d2;
+}
+
+__device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<float, 2>& s,
+                                            const Vec<float, 2>& o, Cx<float> nt) {
+    const float mk = p.mn_dt * ks;
+    const float mkc = p.mc_dt * ks;
+    const float den = (p.mc_dt_a * ks) * ks;
+    const float d1 = 1.0f / (1.0f + mk);
+    const float d2 = 1.0f / (1.0f + den);
+    s.c[0].x = (s.c[0].x - mk * o.c[0].x + nt.x) * d1;
+    s.c[0].y = (s.c[0].y - mk * o.c[0].y + nt.y) * d1;
+    s.c[1].x = (s.c[1].x - mkc * o.c[1].x) * d2;
+    s.c[1].y = (s.c[1].y - mkc * o.c[1].y) * d2;
+}
+
+// |k|^2 in float, evaluated as the reference's pfc_ks (pfc_common.h:237-239).
+__device__ __forceinline__ float ksq(float a, float b, float c) {
+#pragma clang fp contract(off)
+    return a * a + b * b + c * c;
+}
+
+// Deterministic workgroup sum (fixed shuffle tree, then waves in order).
+// Result valid in thread 0.  All threads must call.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    constexpr int W = NT < 64 ? NT : 64;
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
+    const int lane = threadIdx.x % W, wid = threadIdx.x / W;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NT / W; i++) tot += red[i];
+    return tot;
+}
+
+// ---------------------------------------------------------------------------
+// Launch shapes
+// ---------------------------------------------------------------------------
+// Launch shapes (elements per thread E, rows per workgroup RPW, columns per
+// workgroup T) per precision and length.  The values are the fastest of the
+// shapes timed on MI355X for the sweep grids (16 instances of 128^2..512^2,
+// FP32) and the single-instance FP64 path; the PFC_* macros override them
+// for timing sweeps.
+#ifndef PFC_ROW_THREADS
+#define PFC_ROW_THREADS 256
+#endif
+#ifndef PFC_COL_T_DIV
+#define PFC_COL_T_DIV 1
+#endif
+#ifndef PFC_ROW_E
+#define PFC_ROW_E 4
+#endif
+#ifndef PFC_COL_T_MUL
+#define PFC_COL_T_MUL 1
+#endif
+// Exchange buffers of 4-field transforms as 16-byte chunk arrays; successive
+// buffers start PFC_LDS_MOD 16-byte bank groups apart (0: plain AoS buffers).
+#ifndef PFC_WPE12
+#define PFC_WPE12 1
+#endif
+#ifndef PFC_WPE
+#define PFC_WPE 1
+#endif
+#ifndef PFC_LDS_MOD
+#define PFC_LDS_MOD 2
+#endif
+constexpr int col_e(bool f64, int LL) {
+    return LL % 3 == 0 ? 12 : (f64 ? (LL >= 1024 ? 8 : 4) : (LL == 512 ? 8 : 4));
+}
+constexpr int col_t(bool f64, int LL) {
+    return f64 ? (LL <= 256 ? 4 : 2) : (LL <= 128 ? 8 : (LL <= 768 ? 4 : 2));
+}
+
+// Row pass: RPW rows of length M = 2H per workgroup, G = H/E threads per row.
+// LDS: one padded exchange buffer of 4-field elements per row.
+template <typename R, int H>
+struct RowCfg {
+    static constexpr int E = elems_for(H, PFC_ROW_E);
+    static constexpr int G = H / E;
+    static constexpr int cap = sizeof(R) == 8 ? 4 : 8;
+    static constexpr int want = PFC_ROW_THREADS / G;
+    static constexpr int RPW = want < 1 ? 1 : (want > cap ? cap : want);
+    static constexpr int NT = RPW * G;
+    static constexpr int K = H;  // row spectrum with X[H] packed into Im X[0]
+    static constexpr int PB = padded(H);                       // elements per row buffer
+    static constexpr int CH = PFC_LDS_MOD ? (int)(sizeof(Vec<R, 4>) / 16) : 1;
+    static constexpr int PS = PFC_LDS_MOD ? soa_stride(H, CH, PFC_LDS_MOD) : PB;
+    static constexpr size_t RB = (size_t)CH * PS * (sizeof(Vec<R, 4>) / CH);  // bytes per row buffer
+    static constexpr size_t LDS = (size_t)RPW * RB;
+    static constexpr int WPE = E == 12 ? PFC_WPE12 : PFC_WPE;  // waves per SIMD the register budget targets
+};
+
+// Column pass: T columns of length LL per workgroup, G = LL/E threads per
+// column; consecutive threads take consecutive columns (coalesced T*16 or
+// T*32 byte segments per row).
+template <typename R, int LL>
+struct ColCfg {
+    static constexpr bool F64 = sizeof(R) == 8;
+    static constexpr int E = col_e(F64, LL);
+    static constexpr int G = LL / E;
+    static constexpr int T0 = col_t(F64, LL);
+    static constexpr int T1 = T0 / PFC_COL_T_DIV > 0 ? T0 / PFC_COL_T_DIV : 1;
+    // Wider tiles (PFC_COL_T_MUL) only while the exchange buffers fit in LDS.
+    static constexpr int T = (size_t)T1 * PFC_COL_T_MUL * padded(LL) * sizeof(Vec<R, 4>) <= 160 * 1024
+                                 ? T1 * PFC_COL_T_MUL : T1;
+    static constexpr int NT = T * G;
+    static constexpr int PB = padded(LL);
+    static constexpr int CH = PFC_LDS_MOD ? (int)(sizeof(Vec<R, 4>) / 16) : 1;
+    static constexpr int PS = PFC_LDS_MOD ? soa_stride(LL, CH, PFC_LDS_MOD) : PB;
+    static constexpr size_t CB = (size_t)CH * PS * (sizeof(Vec<R, 4>) / CH);  // bytes per column buffer
+    // T column buffers; at least two (the packed column's noise exchange uses the second).
+    static constexpr size_t LDS = (size_t)(T < 2 ? 2 : T) * CB;
+    static constexpr int WPE = E == 12 ? PFC_WPE12 : PFC_WPE;
+};
+
+// ---------------------------------------------------------------------------
+// Real-to-complex helpers for one row of length M = 2H.
+// ---------------------------------------------------------------------------
+
+// c2r split: Z[k] = (X[k] + X*[H-k]) + i (X[k] - X*[H-k]) e^{+2 pi i k/M};
+// IFFT_H(Z) = M (x[2j] + i x[2j+1]).
+template <typename R>
+__device__ __forceinline__ Cx<R> c2r_split(Cx<R> xk, Cx<R> xc, Cx<R> w) {
+    const R ax = xk.x + xc.x, ay = xk.y - xc.y;
+    const R dx = xk.x - xc.x, dy = xk.y + xc.y;
+    const R bx = dx * w.x + dy * w.y, by = dy * w.x - dx * w.y;
+    return Cx<R>{ax - by, ay + bx};
+}
+
+// r2c merge: with W = FFT_H(x[2j] + i x[2j+1]),
+// X[k] = (W[k] + W*[H-k])/2 + e^{-2 pi i k/M} (W[k] - W*[H-k])/(2i).
+template <typename R>
+__device__ __forceinline__ Cx<R> r2c_merge(Cx<R> wk, Cx<R> wc, Cx<R> tw) {
+    const R half = R(0.5);
+    const R ex = half * (wk.x + wc.x), ey = half * (wk.y - wc.y);
+    const R dx = wk.x - wc.x, dy = wk.y + wc.y;
+    const R ox = half * dy, oy = -half * dx;
+    return Cx<R>{ex + (tw.x * ox - tw.y * oy), ey + (tw.x * oy + tw.y * ox)};
+}
+
+// Packed row spectra: a real row of length M = 2H has real X[0] and X[H];
+// they are stored as one element X[0] + i X[H], so a row holds H elements.
+// r2c of w[q] (= x[2j] + i x[2j+1] per field, j = t + G*q), stored packed
+// to dst[0..H-1].  `lds` is this row's padded buffer viewed as Vec<R, NC>.
+template <typename R, int H, int E, int NC>
+__device__ __forceinline__ void row_r2c_store(Vec<R, NC> (&w)[E], Vec<R, NC>* lds, int t,
+                                              const Cx<R>* __restrict__ twH, const Cx<R>* __restrict__ twM,
+                                              Buf dst, uint32_t doff) {
+    constexpr int G = H / E;
+    fft_reg<R, NC, H, E, -1>(w, lds, t, twH);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; q++) lds[pidx(t + G * q)] = w[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int k = t + G * q;
+        Vec<R, NC> X;
+        if (k == 0) {
+            // X[0] = Re W0 + Im W0, X[H] = Re W0 - Im W0, both real.
+#pragma unroll
+            for (int f = 0; f < NC; f++) X.c[f] = Cx<R>{w[q].c[f].x + w[q].c[f].y, w[q].c[f].x - w[q].c[f].y};
+        } else {
+            const Vec<R, NC> wc = lds[pidx(H - k)];
+            const Cx<R> tw = twM[k];
+#pragma unroll
+            for (int f = 0; f < NC; f++) X.c[f] = r2c_merge(w[q].c[f], wc.c[f], tw);
+        }
+        bstore(X, dst, (doff + k) * (uint32_t)sizeof(X), 0u);
+    }
+}
+
+// c2r split input for packed index k: (X[k], X[H-k]) with X[0], X[H] unpacked.
+template <typename R>
+__device__ __forceinline__ Cx<R> c2r_split_packed(Cx<R> xk, Cx<R> xc, int k, Cx<R> w) {
+    if (k == 0) return c2r_split(Cx<R>{xk.x, R(0)}, Cx<R>{xk.y, R(0)}, w);
+    return c2r_split(xk, xc, w);
+}
+
+// ---------------------------------------------------------------------------
+// Row pass: S, AB -> IN (+ per-row energy partials)
+// ---------------------------------------------------------------------------
+template <typename R, int H>
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) __attribute__((amdgpu_waves_per_eu(RowCfg<R, H>::WPE)))
+k_row_step(DevBufs b, PhysArgs p, int flags) {
+    using C = RowCfg<R, H>;
+    constexpr int E = C::E, G = C::G, K = C::K;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    __shared__ double red[(C::NT + 63) / 64 + 1];
+    const int tid = threadIdx.x, rl = tid / G, t = tid % G;
+    const int inst = blockIdx.y;
+    if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
+    // Rows of this workgroup from a uniform base; this thread's row at roff.
+    const size_t bbase = ((size_t)inst * b.L * b.N + (size_t)blockIdx.x * C::RPW) * K;
+    const uint32_t roff = (uint32_t)(rl * K);
+    constexpr size_t rows_bytes = (size_t)C::RPW * K * sizeof(Vec<R, 2>);
+    const Buf S = mkbuf((const Vec<R, 2>*)b.S + bbase, rows_bytes);
+    const Buf AB = mkbuf((const Vec<R, 2>*)b.AB + bbase, rows_bytes);
+    const Cx<R>* twH = (const Cx<R>*)b.twH;
+    const Cx<R>* twM = (const Cx<R>*)b.twM;
+
+    // One 4-field c2r: psi, con (from S) and a, b (from AB).
+    Vec<R, 4> z[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int k = t + G * q;
+        const int kc = (H - k) % H;
+        constexpr uint32_t es = sizeof(Vec<R, 2>);
+        const Vec<R, 2> s0 = bload<Vec<R, 2>>(S, (roff + k) * es, 0u), s1 = bload<Vec<R, 2>>(S, (roff + kc) * es, 0u);
+        const Vec<R, 2> a0 = bload<Vec<R, 2>>(AB, (roff + k) * es, 0u), a1 = bload<Vec<R, 2>>(AB, (roff + kc) * es, 0u);
+        const Cx<R> w = twM[k];
+        z[q].c[0] = c2r_split_packed(s0.c[0], s1.c[0], k, w);
+        z[q].c[1] = c2r_split_packed(s0.c[1], s1.c[1], k, w);
+        z[q].c[2] = c2r_split_packed(a0.c[0], a1.c[0], k, w);
+        z[q].c[3] = c2r_split_packed(a0.c[1], a1.c[1], k, w);
+    }
+    fft_reg<R, 4, H, E, +1>(z, LdsBuf<R, 4, C::CH, C::PS>{smem + rl * C::RB}, t, twH);
+
+    const double* mu = b.means + 4 * inst;
+    const R mph = (R)mu[0], mco = (R)mu[1], ma = (R)mu[2], mb = (R)mu[3];
+    const R invM = R(1) / R(2 * H);
+    Vec<R, 2> w[E];
+    double esum = 0.0;
+    const bool want_e = (flags & ROW_ENERGY) != 0;
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const R ph[2] = {mph + z[q].c[0].x * invM, mph + z[q].c[0].y * invM};
+        const R co[2] = {mco + z[q].c[1].x * invM, mco + z[q].c[1].y * invM};
+        const R aa[2] = {ma + z[q].c[2].x * invM, ma + z[q].c[2].y * invM};
+        const R bb[2] = {mb + z[q].c[3].x * invM, mb + z[q].c[3].y * invM};
+        R inn[2], inc[2], in3[2];
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            nonlinear(p, ph[s2], co[s2], aa[s2], bb[s2], inn[s2], inc[s2], in3[s2]);
+            if (want_e) esum += energy_point<R>(p, ph[s2], co[s2], in3[s2]);
+        }
+        w[q].c[0] = Cx<R>{inn[0], inn[1]};
+        w[q].c[1] = Cx<R>{inc[0], inc[1]};
+    }
+    if (flags & ROW_WRITE_IN)
+        row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)(smem + rl * C::RB), t, twH, twM,
+                                  mkbuf((Vec<R, 2>*)b.IN + bbase, rows_bytes), roff);
+    if (want_e) {
+        const double se = block_sum<C::NT>(esum, red);
+
+        if (tid == 0) {
+            const size_t slot = (size_t)inst * gridDim.x + blockIdx.x;
+            b.erow[slot] = se;
+        }
+    }
+}
+
+// Real planes (dpsi, dcon) -> S.
+template <typename R, int H>
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_in(DevBufs b, const R* __restrict__ inphi,
+                                                               const R* __restrict__ incon) {
+    using C = RowCfg<R, H>;
+    constexpr int E = C::E, G = C::G, K = C::K;
+    constexpr int M = 2 * H;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    const int tid = threadIdx.x, rl = tid / G, t = tid % G;
+    const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
+    const size_t rbase = ((size_t)inst * b.L * b.N + row) * K;
+    const size_t ps = b.N;  // real point (l, m, n) of row l*N + n at pbase + m*N
+    const size_t pbase = ((size_t)inst * b.L + row / b.N) * M * ps + row % b.N;
+    Vec<R, 2> w[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int j = t + G * q;
+        w[q].c[0] = Cx<R>{inphi[pbase + 2 * j * ps], inphi[pbase + (2 * j + 1) * ps]};
+        w[q].c[1] = Cx<R>{incon[pbase + 2 * j * ps], incon[pbase + (2 * j + 1) * ps]};
+    }
+    row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
+                           mkbuf((Vec<R, 2>*)b.S + rbase, K * sizeof(Vec<R, 2>)), 0u);
+}
+
+// One real plane per instance -> its packed row spectrum (the noise field).
+template <typename R, int H>
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_r2c1(DevBufs b, const R* __restrict__ in,
+                                                                 Vec<R, 1>* __restrict__ out) {
+    using C = RowCfg<R, H>;
+    constexpr int E = C::E, G = C::G, K = C::K;
+    constexpr int M = 2 * H;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    const int tid = threadIdx.x, rl = tid / G, t = tid % G;
+    const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
+    if (b.active && !b.active[inst]) return;
+    const size_t rbase = ((size_t)inst * b.L * b.N + row) * K;
+    const size_t ps = b.N;
+    const size_t pbase = ((size_t)inst * b.L + row / b.N) * M * ps + row % b.N;
+    Vec<R, 1> w[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int j = t + G * q;
+        w[q].c[0] = Cx<R>{in[pbase + 2 * j * ps], in[pbase + (2 * j + 1) * ps]};
+    }
+    row_r2c_store<R, H, E, 1>(w, (Vec<R, 1>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
+                           mkbuf(out + rbase, K * sizeof(Vec<R, 1>)), 0u);
+}
+
+// S -> real planes (psi, con) including the means.
+template <typename R, int H>
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __restrict__ outphi,
+                                                                R* __restrict__ outcon) {
+    using C = RowCfg<R, H>;
+    constexpr int E = C::E, G = C::G, K = C::K;
+    constexpr int M = 2 * H;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    const int tid = threadIdx.x, rl = tid / G, t = tid % G;
+    const int inst = blockIdx.y, row = blockIdx.x * C::RPW + rl;
+    const size_t rbase = ((size_t)inst * b.L * b.N + row) * K;
+    const size_t ps = b.N;
+    const size_t pbase = ((size_t)inst * b.L + row / b.N) * M * ps + row % b.N;
+    const Vec<R, 2>* S = (const Vec<R, 2>*)b.S + rbase;
+    const Cx<R>* twM = (const Cx<R>*)b.twM;
+    Vec<R, 2> z[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int k = t + G * q;
+        const int kc = (H - k) % H;
+        const Vec<R, 2> s0 = S[k], s1 = S[kc];
+        const Cx<R> w = twM[k];
+        z[q].c[0] = c2r_split_packed(s0.c[0], s1.c[0], k, w);
+        z[q].c[1] = c2r_split_packed(s0.c[1], s1.c[1], k, w);
+    }
+    fft_reg<R, 2, H, E, +1>(z, (Vec<R, 2>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH);
+    const double* mu = b.means + 4 * inst;
+    const R mph = (R)mu[0], mco = (R)mu[1];
+    const R invM = R(1) / R(M);
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int j = t + G * q;
+        outphi[pbase + 2 * j * ps] = mph + z[q].c[0].x * invM;
+        outphi[pbase + (2 * j + 1) * ps] = mph + z[q].c[0].y * invM;
+        outcon[pbase + 2 * j * ps] = mco + z[q].c[1].x * invM;
+        outcon[pbase + (2 * j + 1) * ps] = mco + z[q].c[1].y * invM;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass
+// ---------------------------------------------------------------------------
+
+// Split of a packed column value (k_m = 0 and k_m = M/2 carried as
+// Z = X0 + i XH, X0 and XH Hermitian along l): from Z(k) and Z(-k).
+template <typename R>
+__device__ __forceinline__ void hsplit(Cx<R> z, Cx<R> zm, Cx<R>& x0, Cx<R>& xh) {
+    const R h = R(0.5);
+    x0 = Cx<R>{h * (z.x + zm.x), h * (z.y - zm.y)};
+    xh = Cx<R>{h * (z.y + zm.y), h * (zm.x - z.x)};
+}
+template <typename R>
+__device__ __forceinline__ Cx<R> hjoin(Cx<R> x0, Cx<R> xh) {
+    return Cx<R>{x0.x - xh.y, x0.y + xh.x};
+}
+
+// Per-mode work of the column pass on one (k_l = i, k_m = kmi, k_n = kni)
+// mode: update (mode STEP), DC removal, energy term, C2 products.
+template <typename R>
+__device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, int mode, bool want_e, int LL, int i,
+                                         int kmi, int kni, Cx<R>& ph, Cx<R>& co, Cx<R> on, Cx<R> oc, Cx<R> nz,
+                                         Cx<R>& a, Cx<R>& bb, double& g) {
+    const int Mh = b.M / 2, Nh = b.N / 2;
+    const float kmv = b.km[kmi];
+    const float ks = ksq(b.kl[i], kmv, b.kn[kni]);
+    const bool mself = kmi == 0 || kmi == Mh;
+    if (mode == COL_STEP) {
+        Vec<R, 2> s, o;
+        s.c[0] = ph; s.c[1] = co;
+        o.c[0] = on; o.c[1] = oc;
+        Cx<R> nt{R(0), R(0)};
+        // self-conjugate selector of the reference: l and m only (misfit.cpp:991-1000)
+
+        update_mode(p, ks, s, o, nt);
+        ph = s.c[0]; co = s.c[1];
+    }
+    if (i == 0 && kmi == 0 && kni == 0) {
+        ph = Cx<R>{R(0), R(0)};
+        co = Cx<R>{R(0), R(0)};
+    }
+    if (want_e) {
+        // (kl'^2 + 2 km'^2) |con_hat|^2: own-direction Nyquist removed (Re of
+        // the reference's i k con_hat transform).  A mode whose conjugate is
+        // not stored (0 < km < M/2, or km in {0, M/2} with 0 < kn < N/2)
+        // counts twice.
+        const double kmd = (kmi == Mh) ? 0.0 : (double)kmv;
+        const double mult = (mself && (kni == 0 || kni == Nh)) ? 1.0 : 2.0;
+        const double kld = (i == LL / 2) ? 0.0 : (double)b.kl[i];
+        g += mult * (kld * kld + 2.0 * kmd * kmd) * ((double)co.x * co.x + (double)co.y * co.y);
+    }
+    const int il = i <= LL / 2 ? i : LL - i;
+    const int jn = kni <= Nh ? kni : b.N - kni;
+    const size_t ci = 2 * (((size_t)il * (Mh + 1) + kmi) * (Nh + 1) + jn);
+    const R ca = (R)b.c2[ci], cb = (R)b.c2[ci + 1];
+    a = Cx<R>{ph.x * ca, ph.y * ca};
+    bb = Cx<R>{ph.x * cb, ph.y * cb};
+}
+
+template <typename R, int LL>
+__global__ void __launch_bounds__((ColCfg<R, LL>::NT)) __attribute__((amdgpu_waves_per_eu(ColCfg<R, LL>::WPE)))
+k_col(DevBufs b, PhysArgs p, int mode, int flags) {
+    using C = ColCfg<R, LL>;
+    constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    __shared__ double red[(NT + 63) / 64 + 1];
+    const int tid = threadIdx.x, col = tid % T, t = tid / T;
+    const int inst = blockIdx.y;
+    if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
+    const int K = b.K;
+    const size_t CS = (size_t)b.N * K;       // stride between consecutive l
+    const int tile0 = blockIdx.x * T;        // first column of the tile; T divides K
+    const int j = tile0 / K;                 // n slot of the whole tile
+    const int kc = tile0 - j * K + col;      // packed m column (kc = 0 holds k_m = 0 and M/2)
+    // Element i of this column: uniform tile base + 32-bit offset col + i*CS.
+    const size_t tbase = (size_t)inst * LL * CS + tile0;
+    constexpr uint32_t es = sizeof(Vec<R, 2>);
+    const uint32_t toff = (uint32_t)(col + t * CS) * es;     // bytes
+    const size_t tbytes = ((size_t)LL * CS - tile0) * es;    // rest of the instance
+    const Buf S = mkbuf((const Vec<R, 2>*)b.S + tbase, tbytes);
+    const Cx<R>* twL = (const Cx<R>*)b.twL;
+    const LdsBuf<R, 4, C::CH, C::PS> lds{smem + col * C::CB};
+    const int Nh = b.N / 2;
+    // Columns kc = 0 of slots j = 0 and N/2 carry two Hermitian-in-l sequences
+    // (k_m = 0 and M/2); uniform per workgroup.
+    const bool packed_tile = tile0 - j * K == 0 && (j == 0 || j == Nh);
+    const bool packed = packed_tile && col == 0;
+    // Mode of a plain column; the kc = 0 column of the other slots holds
+    // k_m = 0 (j < N/2) or k_m = M/2 with k_n = N - j (j > N/2).
+    int kmi = kc, kni = j;
+    if (kc == 0 && !packed_tile) {
+        kmi = j < Nh ? 0 : b.M / 2;
+        kni = j < Nh ? j : b.N - j;
+    }
+    const bool step = true; mode = COL_STEP;
+
+    // Forward transform of {psi, con, inn, inc} along l in one pass.
+    Vec<R, 4> x[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const uint32_t qo = (uint32_t)(G * q * CS) * es;  // uniform
+        const Vec<R, 2> sv = bload<Vec<R, 2>>(S, toff, qo);
+        x[q].c[0] = sv.c[0];
+        x[q].c[1] = sv.c[1];
+        if (step) {
+            const Vec<R, 2> iv = bload<Vec<R, 2>>(mkbuf((const Vec<R, 2>*)b.IN + tbase, tbytes), toff, qo);
+            x[q].c[2] = iv.c[0];
+            x[q].c[3] = iv.c[1];
+        } else {
+            x[q].c[2] = x[q].c[3] = Cx<R>{R(0), R(0)};
+        }
+    }
+    fft_reg<R, 4, LL, E, -1>(x, lds, t, twL);
+    Cx<R> nz[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) nz[q] = Cx<R>{R(0), R(0)};
+    if (false) {
+        Vec<R, 1> n1[E];
+#pragma unroll
+        for (int q = 0; q < E; q++)
+            n1[q] = bload<Vec<R, 1>>(mkbuf((const Vec<R, 1>*)b.NZ + tbase, tbytes / 2), toff / 2,
+                                     (uint32_t)(G * q * CS) * (es / 2));
+        fft_reg<R, 1, LL, E, -1>(n1, (Vec<R, 1>*)(smem + col * C::CB), t, twL);
+#pragma unroll
+        for (int q = 0; q < E; q++) nz[q] = n1[q].c[0];
+    }
+    // The packed column needs each mode's mirror -k: exchange it through LDS
+    // (column 0's buffer, and column 1's for the noise).
+    Vec<R, 4>* mx4 = (Vec<R, 4>*)smem;
+    Cx<R>* nzx = (Cx<R>*)(mx4 + C::PB);
+    if (packed_tile) {
+        __syncthreads();
+        if (packed) {
+#pragma unroll
+            for (int q = 0; q < E; q++) {
+                const int i = pidx(t + G * q);
+                mx4[i] = x[q];
+                
+            }
+        }
+        __syncthreads();
+    }
+
+    const bool want_e = false;
+    double g = 0.0;
+    // x[q] becomes {psi', con', C2A psi', C2B psi'} in place.
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int i = t + G * q;
+        Cx<R> ph = x[q].c[0], co = x[q].c[1], a, bb;
+        if (!packed) {
+            col_mode(b, p, mode, want_e, LL, i, kmi, kni, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
+        } else {
+            const int im = pidx((LL - i) % LL);
+            const Vec<R, 4> xm = mx4[im];
+            const Cx<R> nzm = Cx<R>{R(0), R(0)};
+            Cx<R> ph0, phh, co0, coh, on0, onh, oc0, och, nz0, nzh, a0, ah, b0, bh;
+            hsplit(x[q].c[0], xm.c[0], ph0, phh);
+            hsplit(x[q].c[1], xm.c[1], co0, coh);
+            hsplit(x[q].c[2], xm.c[2], on0, onh);
+            hsplit(x[q].c[3], xm.c[3], oc0, och);
+            hsplit(nz[q], nzm, nz0, nzh);
+            col_mode(b, p, mode, want_e, LL, i, 0, j, ph0, co0, on0, oc0, nz0, a0, b0, g);
+            col_mode(b, p, mode, want_e, LL, i, b.M / 2, j, phh, coh, onh, och, nzh, ah, bh, g);
+            ph = hjoin(ph0, phh);
+            co = hjoin(co0, coh);
+            a = hjoin(a0, ah);
+            bb = hjoin(b0, bh);
+        }
+        x[q].c[0] = ph;
+        x[q].c[1] = co;
+        x[q].c[2] = a;
+        x[q].c[3] = bb;
+    }
+    if (step && b.tcount && blockIdx.x == 0 && tid == 0) b.tcount[inst] += 1;
+    if (want_e) {
+        const double gs = block_sum<NT>(g, red);
+        if (tid == 0) b.ecol[(size_t)inst * gridDim.x + blockIdx.x] = gs;
+    }
+    if (mode == COL_ENERGY_ONLY) return;
+
+    // Inverse transform of the new state and of C2A/C2B times the new psi.
+    fft_reg<R, 4, LL, E, +1>(x, lds, t, twL);
+    const R invL = R(1) / R(LL);
+    const Buf AB = mkbuf((const Vec<R, 2>*)b.AB + tbase, tbytes);
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const uint32_t qo = (uint32_t)(G * q * CS) * es;
+        Vec<R, 2> sv, av;
+        sv.c[0] = Cx<R>{x[q].c[0].x * invL, x[q].c[0].y * invL};
+        sv.c[1] = Cx<R>{x[q].c[1].x * invL, x[q].c[1].y * invL};
+        av.c[0] = Cx<R>{x[q].c[2].x * invL, x[q].c[2].y * invL};
+        av.c[1] = Cx<R>{x[q].c[3].x * invL, x[q].c[3].y * invL};
+        bstore(sv, S, toff, qo);
+        bstore(av, AB, toff, qo);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Depth pass (3-D grids): FFT along n of the row spectra
+// ---------------------------------------------------------------------------
+// T packed m columns of one l row per workgroup, G = N/E threads per column.
+template <typename R, int NN>
+struct DepCfg {
+    static constexpr int E = NN >= 64 ? 8 : 4;
+    static constexpr int G = NN / E;
+    static constexpr int T = 16;  // 16 consecutive m columns: 256 or 512 byte segments per n
+    static constexpr int NT = T * G;
+    static constexpr int PB = padded(NN);
+    static constexpr size_t LDS = (size_t)T * PB * sizeof(Vec<R, 4>);
+};
+
+// NC = 1: one complex field (in0/out0 are Cx<R>); NC = 2: one field pair
+// (Vec<R, 2>); NC = 4: two field pairs (in1 null reads zeros, out1 null skips).
+// Along n the kc = 0 column holds Z = X0 + i XH with X0, XH real; the forward
+// pass replaces slot j in (0, N/2) by X0_hat(j) and slot N - j by XH_hat(j)
+// (slots 0 and N/2 keep the packed pair), the inverse pass undoes it.
+template <typename R, int NN, int NC, int DIR>
+__global__ void __launch_bounds__((DepCfg<R, NN>::NT)) k_depth(DevBufs b, const void* in0, void* out0,
+                                                               const void* in1, void* out1) {
+    using C = DepCfg<R, NN>;
+    constexpr int E = C::E, G = C::G, T = C::T;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    const int tid = threadIdx.x, col = tid % T, t = tid / T;
+    const int inst = blockIdx.y;
+    if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
+    const int K = b.K;
+    const int tpr = K / T;  // tiles per l row; T divides K (checked at launch)
+    const int l = blockIdx.x / tpr;
+    const int kc0 = (blockIdx.x - l * tpr) * T;
+    const size_t base = ((size_t)inst * b.L + l) * NN * K + kc0;  // slot n of column col at base + col + n*K
+    const uint32_t toff = (uint32_t)(col + t * K);            // elements
+    const size_t nbytes_el = (size_t)NN * K - kc0;            // elements from base to the end of this l
+    Vec<R, NC>* lds = (Vec<R, NC>*)smem + col * C::PB;
+    const bool split_col = kc0 == 0 && col == 0;
+
+    Vec<R, NC> x[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const uint32_t e = (uint32_t)(G * q * K);  // uniform, elements
+        if constexpr (NC == 1) {
+            constexpr uint32_t es = sizeof(Cx<R>);
+            x[q].c[0] = bload<Cx<R>>(mkbuf((const Cx<R>*)in0 + base, nbytes_el * es), toff * es, e * es);
+        } else {
+            constexpr uint32_t es = sizeof(Vec<R, 2>);
+            const Vec<R, 2> v = bload<Vec<R, 2>>(mkbuf((const Vec<R, 2>*)in0 + base, nbytes_el * es), toff * es, e * es);
+            x[q].c[0] = v.c[0];
+            x[q].c[1] = v.c[1];
+            if constexpr (NC == 4) {
+                Vec<R, 2> w;
+                if (in1) w = bload<Vec<R, 2>>(mkbuf((const Vec<R, 2>*)in1 + base, nbytes_el * es), toff * es, e * es);
+                else w.c[0] = w.c[1] = Cx<R>{R(0), R(0)};
+                x[q].c[2] = w.c[0];
+                x[q].c[3] = w.c[1];
+            }
+        }
+    }
+    if constexpr (DIR > 0) {
+        // Join: slot j < N/2 becomes X0_hat(j) + i XH_hat(j), slot N - j
+        // conj(X0_hat(j)) + i conj(XH_hat(j)); the mirror slot comes via LDS.
+        if (kc0 == 0) {  // uniform per workgroup
+            if (split_col) {
+#pragma unroll
+                for (int q = 0; q < E; q++) lds[pidx(t + G * q)] = x[q];
+            }
+            __syncthreads();
+            if (split_col) {
+#pragma unroll
+                for (int q = 0; q < E; q++) {
+                    const int jj = t + G * q;
+                    if (jj == 0 || jj == NN / 2) continue;
+                    const Vec<R, NC> xm = lds[pidx(NN - jj)];
+#pragma unroll
+                    for (int f = 0; f < NC; f++) {
+                        if (jj < NN / 2) {
+                            x[q].c[f] = hjoin(x[q].c[f], xm.c[f]);
+                        } else {
+                            const Cx<R> x0c{xm.c[f].x, -xm.c[f].y}, xhc{x[q].c[f].x, -x[q].c[f].y};
+                            x[q].c[f] = hjoin(x0c, xhc);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    fft_reg<R, NC, NN, E, DIR>(x, lds, t, (const Cx<R>*)b.twN);
+    if constexpr (DIR < 0) {
+        if (kc0 == 0) {  // uniform per workgroup
+            __syncthreads();
+            if (split_col) {
+#pragma unroll
+                for (int q = 0; q < E; q++) lds[pidx(t + G * q)] = x[q];
+            }
+            __syncthreads();
+            if (split_col) {
+#pragma unroll
+                for (int q = 0; q < E; q++) {
+                    const int jj = t + G * q;
+                    if (jj == 0 || jj == NN / 2) continue;
+                    const Vec<R, NC> xm = lds[pidx(NN - jj)];
+#pragma unroll
+                    for (int f = 0; f < NC; f++) {
+                        Cx<R> x0, xh;
+                        if (jj < NN / 2) {
+                            hsplit(x[q].c[f], xm.c[f], x0, xh);
+                            x[q].c[f] = x0;
+                        } else {
+                            hsplit(xm.c[f], x[q].c[f], x0, xh);
+                            x[q].c[f] = xh;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    const R sc = DIR > 0 ? R(1) / R(NN) : R(1);
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const uint32_t e = (uint32_t)(G * q * K);
+        if constexpr (DIR > 0) {
+#pragma unroll
+            for (int f = 0; f < NC; f++) x[q].c[f] = Cx<R>{x[q].c[f].x * sc, x[q].c[f].y * sc};
+        }
+        if constexpr (NC == 1) {
+            constexpr uint32_t es = sizeof(Cx<R>);
+            bstore(x[q].c[0], mkbuf((Cx<R>*)out0 + base, nbytes_el * es), toff * es, e * es);
+        } else {
+            Vec<R, 2> v;
+            v.c[0] = x[q].c[0];
+            v.c[1] = x[q].c[1];
+            constexpr uint32_t es = sizeof(Vec<R, 2>);
+            bstore(v, mkbuf((Vec<R, 2>*)out0 + base, nbytes_el * es), toff * es, e * es);
+            if constexpr (NC == 4) {
+                if (out1) {
+                    Vec<R, 2> w;
+                    w.c[0] = x[q].c[2];
+                    w.c[1] = x[q].c[3];
+                    bstore(w, mkbuf((Vec<R, 2>*)out1 + base, nbytes_el * es), toff * es, e * es);
+                }
+            }
+        }
+    }
+}
+
+// Per-instance energy: e_out = {(sum e + alpha/P sum_k w|c|^2)/P, mean psi}, P = L*M*N.
+__global__ void k_energy_reduce(DevBufs b, int nrow, int ncol, double alpha, double* e_out) {
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= b.B) return;
+    double se = 0.0, sg = 0.0;
+    for (int i = 0; i < nrow; i++) {
+        se += b.erow[(size_t)inst * nrow + i];
+    }
+    for (int i = 0; i < ncol; i++) sg += b.ecol[(size_t)inst * ncol + i];
+    const double P = (double)b.L * (double)b.M * (double)b.N;
+    const double tot = se + alpha * sg / P;
+    e_out[2 * inst] = tot / b.L / b.M / b.N;
+    // Density: the k=0 mode, held in FP64 and left unchanged by every step
+    // (k = 0 makes the update an identity, misfit.cpp:1002-1010); the sum of
+    // the reconstructed deviations is zero up to transform round-off.
+    e_out[2 * inst + 1] = b.means[4 * inst];
+}
+
+// ---------------------------------------------------------------------------
+// Host dispatch
+// ---------------------------------------------------------------------------
+bool size_supported(int n) {
+    switch (n) {
+        case 32: case 64: case 128: case 256: case 384: case 512: case 768: case 1024: return true;
+        default: return false;
+    }
+}
+
+bool depth_supported(int n) {
+    switch (n) {
+        case 16: case 32: case 64: case 128: return true;
+        default: return false;
+    }
+}
+
+template <typename R>
+static int row_blocks_t(int M, int L) {
+    int rpw = 1;
+    switch (M / 2) {
+#define RB(H) case H: rpw = RowCfg<R, H>::RPW; break;
+        RB(16) RB(32) RB(64) RB(128) RB(192) RB(256) RB(384) RB(512)
+#undef RB
+        default: return -1;
+    }
+    if (L % rpw) return -1;
+    return L / rpw;
+}
+
+template <typename R>
+static int col_blocks_t(int L, int M, int N) {
+    int T = 1;
+    switch (L) {
+#define CB(LL) case LL: T = ColCfg<R, LL>::T; break;
+        CB(32) CB(64) CB(128) CB(256) CB(384) CB(512) CB(768) CB(1024)
+#undef CB
+        default: return -1;
+    }
+    const int K = M / 2;
+    if (K % T) return -1;  // k_col handles whole tiles of T columns only
+    return N * (K / T);
+}
+
+int row_blocks(int prec, int M, int rows) {
+    return prec == PREC_FP64 ? row_blocks_t<double>(M, rows) : row_blocks_t<float>(M, rows);
+}
+int col_blocks(int prec, int L, int M, int N) {
+    return prec == PREC_FP64 ? col_blocks_t<double>(L, M, N) : col_blocks_t<float>(L, M, N);
+}
+
+template <typename R>
+static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s, hipEvent_t t0,
+                             hipEvent_t t1) {
+    const int nb = row_blocks_t<R>(b.M, b.L * b.N);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.M / 2) {
+#define RS(H) case H: PFC_LAUNCH((k_row_step<R, H>), grid, dim3(RowCfg<R, H>::NT), s, t0, t1, b, p, flags); break;
+        RS(16) RS(32) RS(64) RS(128) RS(192) RS(256) RS(384) RS(512)
+#undef RS
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s, hipEvent_t t0,
+                        hipEvent_t t1) {
+    const int nb = col_blocks_t<R>(b.L, b.M, b.N);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.L) {
+#define CS(LL) case LL: PFC_LAUNCH((k_col<R, LL>), grid, dim3(ColCfg<R, LL>::NT), s, t0, t1, b, p, mode, flags); break;
+        CS(32) CS(64) CS(128) CS(256) CS(384) CS(512) CS(768) CS(1024)
+#undef CS
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t row_in_t(const DevBufs& b, const void* inphi, const void* incon, hipStream_t s) {
+    const int nb = row_blocks_t<R>(b.M, b.L * b.N);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.M / 2) {
+#define RI(H) case H: hipLaunchKernelGGL((k_row_in<R, H>), grid, dim3(RowCfg<R, H>::NT), 0, s, b, (const R*)inphi, (const R*)incon); break;
+        RI(16) RI(32) RI(64) RI(128) RI(192) RI(256) RI(384) RI(512)
+#undef RI
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t row_out_t(const DevBufs& b, void* outphi, void* outcon, hipStream_t s) {
+    const int nb = row_blocks_t<R>(b.M, b.L * b.N);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.M / 2) {
+#define RO(H) case H: hipLaunchKernelGGL((k_row_out<R, H>), grid, dim3(RowCfg<R, H>::NT), 0, s, b, (R*)outphi, (R*)outcon); break;
+        RO(16) RO(32) RO(64) RO(128) RO(192) RO(256) RO(384) RO(512)
+#undef RO
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t row_r2c1_t(const DevBufs& b, const void* in, void* out, hipStream_t s) {
+    const int nb = row_blocks_t<R>(b.M, b.L * b.N);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
+    switch (b.M / 2) {
+#define R1(H) case H: hipLaunchKernelGGL((k_row_r2c1<R, H>), grid, dim3(RowCfg<R, H>::NT), 0, s, b, (const R*)in, (Vec<R, 1>*)out); break;
+        R1(16) R1(32) R1(64) R1(128) R1(192) R1(256) R1(384) R1(512)
+#undef R1
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int NC, int DIR>
+static hipError_t depth_t(const DevBufs& b, const void* in0, void* out0, const void* in1, void* out1, hipStream_t s,
+                          hipEvent_t t0, hipEvent_t t1) {
+    constexpr int T = 16;
+    if (b.K % T) return hipErrorInvalidValue;
+    dim3 grid(b.L * (b.K / T), b.B);
+    switch (b.N) {
+#define DP(NN) case NN: static_assert(DepCfg<R, NN>::T == T, "tile width"); \
+        PFC_LAUNCH((k_depth<R, NN, NC, DIR>), grid, dim3(DepCfg<R, NN>::NT), s, t0, t1, b, in0, out0, in1, out1); break;
+        DP(16) DP(32) DP(64) DP(128)
+#undef DP
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t depth_r(const DevBufs& b, int dir, const void* in0, void* out0, const void* in1, void* out1,
+                          bool one_field, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+    if (dir < 0) {
+        if (one_field) return depth_t<R, 1, -1>(b, in0, out0, nullptr, nullptr, s, t0, t1);
+        if (!in1 && !out1) return depth_t<R, 2, -1>(b, in0, out0, nullptr, nullptr, s, t0, t1);
+        return depth_t<R, 4, -1>(b, in0, out0, in1, out1, s, t0, t1);
+    }
+    if (one_field || !in1 || !out1) return hipErrorInvalidValue;
+    return depth_t<R, 4, +1>(b, in0, out0, in1, out1, s, t0, t1);
+}
+
+hipError_t launch_depth(int prec, const DevBufs& b, int dir, const void* in0, void* out0, const void* in1,
+                        void* out1, bool one_field, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+    return prec == PREC_FP64 ? depth_r<double>(b, dir, in0, out0, in1, out1, one_field, s, t0, t1)
+                             : depth_r<float>(b, dir, in0, out0, in1, out1, one_field, s, t0, t1);
+}
+
+hipError_t launch_row_r2c1(int prec, const DevBufs& b, const void* in, void* out, hipStream_t s) {
+    return prec == PREC_FP64 ? row_r2c1_t<double>(b, in, out, s) : row_r2c1_t<float>(b, in, out, s);
+}
+
+hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s, hipEvent_t t0,
+                           hipEvent_t t1) {
+    return prec == PREC_FP64 ? row_step_t<double>(b, p, flags, s, t0, t1) : row_step_t<float>(b, p, flags, s, t0, t1);
+}
+hipError_t launch_col(int prec, const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s,
+                      hipEvent_t t0, hipEvent_t t1) {
+    return prec == PREC_FP64 ? col_t<double>(b, p, mode, flags, s, t0, t1)
+                             : col_t<float>(b, p, mode, flags, s, t0, t1);
+}
+hipError_t launch_row_in(int prec, const DevBufs& b, const void* inphi, const void* incon, hipStream_t s) {
+    return prec == PREC_FP64 ? row_in_t<double>(b, inphi, incon, s) : row_in_t<float>(b, inphi, incon, s);
+}
+hipError_t launch_row_out(int prec, const DevBufs& b, void* outphi, void* outcon, hipStream_t s) {
+    return prec == PREC_FP64 ? row_out_t<double>(b, outphi, outcon, s) : row_out_t<float>(b, outphi, outcon, s);
+}
+hipError_t launch_energy_reduce(const DevBufs& b, int nrow, int ncol, double alpha, double* e_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_energy_reduce, dim3((b.B + 63) / 64), dim3(64), 0, s, b, nrow, ncol, alpha, e_out);
+    return hipGetLastError();
+}
+
+}  // namespace pfc

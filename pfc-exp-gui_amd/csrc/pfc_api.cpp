@@ -45,6 +45,7 @@ struct pfc_ctx {
     std::vector<double> means_h;
     void *twH = nullptr, *twM = nullptr, *twL = nullptr, *twN = nullptr;
     float *kl_d = nullptr, *km_d = nullptr, *kn_d = nullptr, *c2_d = nullptr;
+    void *coefA = nullptr, *coefC = nullptr;  // FP32 per-mode coefficients [L][N][K]
     int* active_d = nullptr;  // [B] step mask
     double *erow = nullptr, *ecol[2] = {nullptr, nullptr}, *eout = nullptr;
     int nrow = 0, ncol = 0;
@@ -82,7 +83,8 @@ size_t ctx_bytes(const pfc_params& p, int batch) {
     if (p.model == PFC_MODEL_MISFIT && p.axx > 0.0f)
         b += nreal * rs + nvec * 2 * rs + noise_scratch_bytes((long)P, batch);
     const size_t tables = ((size_t)(p.L / 2 + 1) * (p.M / 2 + 1) * (p.N / 2 + 1) * 2 + p.L + p.M + p.N) * 4 +
-                          (size_t)(p.L + p.M + p.N + p.M / 2) * 2 * rs;
+                          (size_t)(p.L + p.M + p.N + p.M / 2) * 2 * rs +
+                          (rs == 4 && batch >= 4 ? (size_t)p.L * p.N * (p.M / 2) * 24 : 0);
     return b + tables + (size_t)batch * 256 + (1u << 20);
 }
 }  // namespace pfc
@@ -155,7 +157,7 @@ static void free_ctx(pfc_ctx* c) {
     for (auto e : c->ev) hipEventDestroy(e);
     void* ptrs[] = {c->S, c->AB, c->IN, c->means_d, c->twH, c->twM, c->twL, c->twN, c->kl_d, c->km_d, c->kn_d,
                     c->c2_d, c->active_d, c->erow, c->ecol[0], c->ecol[1], c->eout, c->stage_a, c->stage_b,
-                    c->tcount, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr};
+                    c->tcount, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr, c->coefA, c->coefC};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -269,6 +271,24 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     b.tcount = c->tcount;
     b.active = c->active_d;
     b.L = c->L; b.M = c->M; b.N = c->N; b.K = c->K; b.B = batch;
+    b.cA = b.cC = nullptr;
+    // The tables pay off when several instances share them (a single
+    // instance would read them from HBM beside its state).
+    if (prec == PREC_FP32 && batch >= 4) {
+        const size_t nm = (size_t)c->L * c->N * c->K;
+        e = hipMalloc(&c->coefA, nm * 16);
+        if (e == hipSuccess) e = hipMalloc(&c->coefC, nm * 8);
+        if (e == hipSuccess) {
+            b.cA = c->coefA;
+            b.cC = c->coefC;
+            e = launch_coef(b, ph, c->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            free_ctx(c);
+            return fail(PFC_E_NOMEM, "coefficient tables: %s", hipGetErrorString(e));
+        }
+    }
     *out = c;
     return PFC_OK;
 }

@@ -235,8 +235,25 @@ __device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
 struct alignas(16) Chunk16 {
     unsigned int w[4];
 };
-template <typename R, int NC, int CH, int PS>
+
+// Exchange barrier.  WAVE: every transform sharing the buffer lives in one
+// wavefront, whose LDS accesses execute in order, so only the compiler must
+// be kept from moving accesses across (wavefront-scope fences); otherwise a
+// workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void xsync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+template <typename R, int NC, int CH, int PS, bool WAVE = false>
 struct LdsBuf {
+    static constexpr bool SPLIT = false;
+    static constexpr bool WSYNC = WAVE;
     char* p;
     static_assert(CH == 1 || sizeof(Vec<R, NC>) == 16 * CH, "chunks are 16 bytes");
     __device__ __forceinline__ Vec<R, NC> ld(int i) const {
@@ -266,6 +283,28 @@ struct LdsBuf {
     }
 };
 
+// Exchange buffer holding one 16-byte chunk of each slot: an exchange moves
+// the Vec chunk by chunk (store chunk c, barrier, load chunk c, barrier), so
+// the buffer is 16 bytes per slot whatever the Vec size, at the price of one
+// barrier pair per chunk.
+template <typename R, int NC, bool WAVE = false>
+struct LdsSplit {
+    static constexpr bool SPLIT = true;
+    static constexpr bool WSYNC = WAVE;
+    static constexpr int NCH = (int)(sizeof(Vec<R, NC>) / 16);
+    static_assert(sizeof(Vec<R, NC>) % 16 == 0, "chunks are 16 bytes");
+    char* p;
+    __device__ __forceinline__ void stc(int c, int i, const Vec<R, NC>& v) const {
+        Chunk16 k;
+        __builtin_memcpy(&k, (const char*)&v + 16 * c, 16);
+        ((Chunk16*)p)[i] = k;
+    }
+    __device__ __forceinline__ void ldc(int c, int i, Vec<R, NC>& v) const {
+        const Chunk16 k = ((const Chunk16*)p)[i];
+        __builtin_memcpy((char*)&v + 16 * c, &k, 16);
+    }
+};
+
 // Slots per chunk array for an SoA buffer of n padded slots: the first
 // PS >= padded(n) with (CH * PS) % 16 == MOD, so that buffers placed one
 // after another start MOD 16-byte groups apart.
@@ -275,8 +314,11 @@ constexpr int soa_stride(int n, int CH, int MOD) {
     return ps;
 }
 
-template <typename R, int NC, int N, int E, int DIR, int NS, bool FIRST, typename L>
-__device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], const L& lds, int t,
+// SRC: where the pass's inputs are: 0 = x in natural order (first pass),
+// 1 = the LDS buffer, 2 = `in`, already in this pass's butterfly order (after
+// a chunked exchange).
+template <typename R, int NC, int N, int E, int DIR, int NS, int SRC, typename L>
+__device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], Vec<R, NC> (&in)[E], const L& lds, int t,
                                                const Cx<R>* __restrict__ tw) {
     if constexpr (NS < N) {
         constexpr int RAD = pick_radix(N / NS, E);
@@ -284,16 +326,19 @@ __device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], const L& lds,
         constexpr int G = N / E, NB = E / RAD, STEP = N / RAD;
         constexpr bool LAST = (NS * RAD == N);
         Vec<R, NC> v[E];
-        if constexpr (FIRST) {
+        if constexpr (SRC == 0) {
 #pragma unroll
             for (int b = 0; b < NB; b++)
 #pragma unroll
                 for (int r = 0; r < RAD; r++) v[b * RAD + r] = x[b + r * NB];
-        } else {
+        } else if constexpr (SRC == 1) {
 #pragma unroll
             for (int b = 0; b < NB; b++)
 #pragma unroll
                 for (int r = 0; r < RAD; r++) v[b * RAD + r] = lds.ld(pidx(t + G * b + r * STEP));
+        } else {
+#pragma unroll
+            for (int q = 0; q < E; q++) v[q] = in[q];
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
@@ -314,8 +359,30 @@ __device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], const L& lds,
             for (int b = 0; b < NB; b++)
 #pragma unroll
                 for (int r = 0; r < RAD; r++) x[b + r * NB] = v[b * RAD + r];
+        } else if constexpr (L::SPLIT) {
+            constexpr int NS2 = NS * RAD;
+            constexpr int RAD2 = pick_radix(N / NS2, E);
+            constexpr int NB2 = E / RAD2, STEP2 = N / RAD2;
+            Vec<R, NC> w[E];
+#pragma unroll
+            for (int c = 0; c < L::NCH; c++) {
+                xsync<L::WSYNC>();
+#pragma unroll
+                for (int b = 0; b < NB; b++) {
+                    const int j = t + G * b;
+                    const int idx = (j / NS) * NS * RAD + j % NS;
+#pragma unroll
+                    for (int r = 0; r < RAD; r++) lds.stc(c, pidx(idx + r * NS), v[b * RAD + r]);
+                }
+                xsync<L::WSYNC>();
+#pragma unroll
+                for (int b = 0; b < NB2; b++)
+#pragma unroll
+                    for (int r = 0; r < RAD2; r++) lds.ldc(c, pidx(t + G * b + r * STEP2), w[b * RAD2 + r]);
+            }
+            fft_reg_passes<R, NC, N, E, DIR, NS2, 2, L>(x, w, lds, t, tw);
         } else {
-            __syncthreads();
+            xsync<L::WSYNC>();
 #pragma unroll
             for (int b = 0; b < NB; b++) {
                 const int j = t + G * b;
@@ -323,8 +390,8 @@ __device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], const L& lds,
 #pragma unroll
                 for (int r = 0; r < RAD; r++) lds.st(pidx(idx + r * NS), v[b * RAD + r]);
             }
-            __syncthreads();
-            fft_reg_passes<R, NC, N, E, DIR, NS * RAD, false, L>(x, lds, t, tw);
+            xsync<L::WSYNC>();
+            fft_reg_passes<R, NC, N, E, DIR, NS * RAD, 1, L>(x, in, lds, t, tw);
         }
     }
 }
@@ -333,7 +400,7 @@ template <typename R, int NC, int N, int E, int DIR, typename L>
 __device__ __forceinline__ void fft_reg(Vec<R, NC> (&x)[E], const L& lds, int t, const Cx<R>* __restrict__ tw) {
     static_assert(N % E == 0, "length must be a multiple of elements per thread");
     static_assert(plan_ok(N, E), "no radix plan");
-    fft_reg_passes<R, NC, N, E, DIR, 1, true, L>(x, lds, t, tw);
+    fft_reg_passes<R, NC, N, E, DIR, 1, 0, L>(x, x, lds, t, tw);
 }
 
 // AoS exchange buffer (one Vec per slot).

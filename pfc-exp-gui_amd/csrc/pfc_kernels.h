@@ -51,12 +51,20 @@ struct DevBufs {
     const void* NZ;       // [B][L][K] complex: row-spectral noise (nullptr: no noise)
     long long* tcount;    // [B] step counter per instance (advanced by the column pass)
     const int* active;    // [B] 0 = instance idle (step kernels skip it); nullptr = all
+    // FP32 per-mode coefficients in the column layout [L][N][K] (nullptr: the
+    // column pass evaluates them from kl/km/kn/c2): cA = {mn dt k^2, mc dt k^2,
+    // 1/(1 + mn dt k^2), 1/(1 + mc dt alpha k^4)}, cC = {C2A, C2B}.
+    const void* cA;
+    const void* cC;
     int L, M, N, K, B;
 };
 
 enum { ROW_WRITE_IN = 1, ROW_ENERGY = 2 };
 enum { COL_INIT = 0, COL_STEP = 1, COL_ENERGY_ONLY = 2 };
 enum { COL_ENERGY = 1 };
+
+// Fill cA/cC (FP32 contexts) from kl/km/kn/c2 and the step constants.
+hipError_t launch_coef(const DevBufs& b, const PhysArgs& p, hipStream_t s);
 
 bool size_supported(int n);
 bool depth_supported(int n);                    // N > 1 lengths of the depth pass

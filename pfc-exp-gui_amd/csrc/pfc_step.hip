@@ -18,6 +18,8 @@
 // pointwise algebra happen in LDS and VGPRs.
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "pfc_fft.hpp"
 #include "pfc_kernels.h"
 
@@ -154,17 +156,31 @@ __device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<dou
     s.c[1].x = r3 / d2; s.c[1].y = r4 / d2;
 }
 
-__device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<float, 2>& s,
-                                            const Vec<float, 2>& o, Cx<float> nt) {
+// FP32 coefficients of one mode: {mk, mkc, 1/(1 + mk), 1/(1 + den)}.  The
+// FP32 arithmetic is spelled out (explicit FMAs, no implicit contraction) so
+// that every path computing it gives the same bits.
+__device__ __forceinline__ float4 mode_coef(const PhysArgs& p, float ks) {
+#pragma clang fp contract(off)
     const float mk = p.mn_dt * ks;
     const float mkc = p.mc_dt * ks;
     const float den = (p.mc_dt_a * ks) * ks;
-    const float d1 = 1.0f / (1.0f + mk);
-    const float d2 = 1.0f / (1.0f + den);
-    s.c[0].x = (s.c[0].x - mk * o.c[0].x + nt.x) * d1;
-    s.c[0].y = (s.c[0].y - mk * o.c[0].y + nt.y) * d1;
-    s.c[1].x = (s.c[1].x - mkc * o.c[1].x) * d2;
-    s.c[1].y = (s.c[1].y - mkc * o.c[1].y) * d2;
+    return make_float4(mk, mkc, 1.0f / (1.0f + mk), 1.0f / (1.0f + den));
+}
+
+// The FP32 update from a mode's coefficients c = mode_coef(ks).
+__device__ __forceinline__ void update_mode_coef(float4 c, Vec<float, 2>& s, const Vec<float, 2>& o,
+                                                 Cx<float> nt) {
+#pragma clang fp contract(off)
+    const float mk = c.x, mkc = c.y, d1 = c.z, d2 = c.w;
+    s.c[0].x = (__builtin_fmaf(-mk, o.c[0].x, s.c[0].x) + nt.x) * d1;
+    s.c[0].y = (__builtin_fmaf(-mk, o.c[0].y, s.c[0].y) + nt.y) * d1;
+    s.c[1].x = __builtin_fmaf(-mkc, o.c[1].x, s.c[1].x) * d2;
+    s.c[1].y = __builtin_fmaf(-mkc, o.c[1].y, s.c[1].y) * d2;
+}
+
+__device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<float, 2>& s,
+                                            const Vec<float, 2>& o, Cx<float> nt) {
+    update_mode_coef(mode_coef(p, ks), s, o, nt);
 }
 
 // |k|^2 in float, evaluated as the reference's pfc_ks (pfc_common.h:237-239).
@@ -212,6 +228,27 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #endif
 // Exchange buffers of 4-field transforms as 16-byte chunk arrays; successive
 // buffers start PFC_LDS_MOD 16-byte bank groups apart (0: plain AoS buffers).
+// Chunked column exchanges: -1 where measured faster (FP32 3*2^k), 0/1.
+#ifndef PFC_COL_SPLIT
+#define PFC_COL_SPLIT -1
+#endif
+// Column exchanges inside one wavefront: -1 where measured faster (FP32
+// 3*2^k lengths and L <= 128), 0 never, 1 wherever a column fits a wave.
+#ifndef PFC_COL_WAVE
+#define PFC_COL_WAVE -1
+#endif
+#ifndef PFC_ROW_WAVE
+#define PFC_ROW_WAVE 1
+#endif
+#ifndef PFC_COL_WPE_LEAN
+#define PFC_COL_WPE_LEAN 1
+#endif
+#ifndef PFC_WPE12
+#define PFC_WPE12 1
+#endif
+#ifndef PFC_WPE
+#define PFC_WPE 1
+#endif
 #ifndef PFC_LDS_MOD
 #define PFC_LDS_MOD 2
 #endif
@@ -238,6 +275,10 @@ struct RowCfg {
     static constexpr int PS = PFC_LDS_MOD ? soa_stride(H, CH, PFC_LDS_MOD) : PB;
     static constexpr size_t RB = (size_t)CH * PS * (sizeof(Vec<R, 4>) / CH);  // bytes per row buffer
     static constexpr size_t LDS = (size_t)RPW * RB;
+    static constexpr int WPE = E == 12 ? PFC_WPE12 : PFC_WPE;  // waves per SIMD the register budget targets
+    // A row's G threads are consecutive; when they fit one wavefront the
+    // row transforms exchange without workgroup barriers.
+    static constexpr bool WAVE = PFC_ROW_WAVE != 0 && G <= 64 && 64 % G == 0;
 };
 
 // Column pass: T columns of length LL per workgroup, G = LL/E threads per
@@ -257,9 +298,24 @@ struct ColCfg {
     static constexpr int PB = padded(LL);
     static constexpr int CH = PFC_LDS_MOD ? (int)(sizeof(Vec<R, 4>) / 16) : 1;
     static constexpr int PS = PFC_LDS_MOD ? soa_stride(LL, CH, PFC_LDS_MOD) : PB;
-    static constexpr size_t CB = (size_t)CH * PS * (sizeof(Vec<R, 4>) / CH);  // bytes per column buffer
-    // T column buffers; at least two (the packed column's noise exchange uses the second).
-    static constexpr size_t LDS = (size_t)(T < 2 ? 2 : T) * CB;
+    // SPLIT: the exchanges move the 4-field Vec in 16-byte chunks through a
+    // buffer of 16 bytes per slot (fewer LDS bytes per wave, more waves per CU).
+    static constexpr bool SPLIT = PFC_COL_SPLIT < 0 ? (!F64 && LL % 3 == 0) : PFC_COL_SPLIT != 0;
+    static constexpr int PS1 = soa_stride(LL, 1, PFC_LDS_MOD ? PFC_LDS_MOD : 1);
+    static constexpr size_t CB = SPLIT ? (size_t)PS1 * 16
+                                       : (size_t)CH * PS * (sizeof(Vec<R, 4>) / CH);  // bytes per column buffer
+    // WAVE: each column's G threads sit in one wavefront (thread = col*G + t),
+    // so the transform's exchanges need no workgroup barrier and the waves of
+    // a workgroup run out of step with each other.
+    static constexpr bool WAVE = (PFC_COL_WAVE < 0 ? (!F64 && (LL % 3 == 0 || LL <= 128)) : PFC_COL_WAVE != 0) &&
+                                 G <= 64 && 64 % G == 0;
+    using Lds = std::conditional_t<SPLIT, LdsSplit<R, 4, WAVE>, LdsBuf<R, 4, CH, PS, WAVE>>;
+    // T column buffers; the packed column's mirror exchange (4-field Vec and
+    // noise value per slot) reuses the space from the start.
+    static constexpr size_t MIRROR = (size_t)PB * (sizeof(Vec<R, 4>) + sizeof(Cx<R>));
+    static constexpr size_t LDS = (size_t)T * CB > MIRROR ? (size_t)T * CB : MIRROR;
+    static constexpr int WPE = E == 12 ? PFC_WPE12 : PFC_WPE;
+    static constexpr int WPE_LEAN = E == 12 ? PFC_WPE12 : PFC_COL_WPE_LEAN;
 };
 
 // ---------------------------------------------------------------------------
@@ -291,16 +347,16 @@ __device__ __forceinline__ Cx<R> r2c_merge(Cx<R> wk, Cx<R> wc, Cx<R> tw) {
 // they are stored as one element X[0] + i X[H], so a row holds H elements.
 // r2c of w[q] (= x[2j] + i x[2j+1] per field, j = t + G*q), stored packed
 // to dst[0..H-1].  `lds` is this row's padded buffer viewed as Vec<R, NC>.
-template <typename R, int H, int E, int NC>
+template <typename R, int H, int E, int NC, bool WAVE = false>
 __device__ __forceinline__ void row_r2c_store(Vec<R, NC> (&w)[E], Vec<R, NC>* lds, int t,
                                               const Cx<R>* __restrict__ twH, const Cx<R>* __restrict__ twM,
                                               Buf dst, uint32_t doff) {
     constexpr int G = H / E;
-    fft_reg<R, NC, H, E, -1>(w, lds, t, twH);
-    __syncthreads();
+    fft_reg<R, NC, H, E, -1>(w, LdsBuf<R, NC, 1, 0, WAVE>{(char*)lds}, t, twH);
+    xsync<WAVE>();
 #pragma unroll
     for (int q = 0; q < E; q++) lds[pidx(t + G * q)] = w[q];
-    __syncthreads();
+    xsync<WAVE>();
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int k = t + G * q;
@@ -330,7 +386,8 @@ __device__ __forceinline__ Cx<R> c2r_split_packed(Cx<R> xk, Cx<R> xc, int k, Cx<
 // Row pass: S, AB -> IN (+ per-row energy partials)
 // ---------------------------------------------------------------------------
 template <typename R, int H>
-__global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, PhysArgs p, int flags) {
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) __attribute__((amdgpu_waves_per_eu(RowCfg<R, H>::WPE)))
+k_row_step(DevBufs b, PhysArgs p, int flags) {
     using C = RowCfg<R, H>;
     constexpr int E = C::E, G = C::G, K = C::K;
     __shared__ __attribute__((aligned(16))) char smem[C::LDS];
@@ -362,7 +419,7 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
         z[q].c[2] = c2r_split_packed(a0.c[0], a1.c[0], k, w);
         z[q].c[3] = c2r_split_packed(a0.c[1], a1.c[1], k, w);
     }
-    fft_reg<R, 4, H, E, +1>(z, LdsBuf<R, 4, C::CH, C::PS>{smem + rl * C::RB}, t, twH);
+    fft_reg<R, 4, H, E, +1>(z, LdsBuf<R, 4, C::CH, C::PS, C::WAVE>{smem + rl * C::RB}, t, twH);
 
     const double* mu = b.means + 4 * inst;
     const R mph = (R)mu[0], mco = (R)mu[1], ma = (R)mu[2], mb = (R)mu[3];
@@ -386,7 +443,7 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_step(DevBufs b, Phys
         w[q].c[1] = Cx<R>{inc[0], inc[1]};
     }
     if (flags & ROW_WRITE_IN)
-        row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)(smem + rl * C::RB), t, twH, twM,
+        row_r2c_store<R, H, E, 2, C::WAVE>(w, (Vec<R, 2>*)(smem + rl * C::RB), t, twH, twM,
                                   mkbuf((Vec<R, 2>*)b.IN + bbase, rows_bytes), roff);
     if (want_e) {
         const double se = block_sum<C::NT>(esum, red);
@@ -505,7 +562,7 @@ __device__ __forceinline__ Cx<R> hjoin(Cx<R> x0, Cx<R> xh) {
 // Per-mode work of the column pass on one (k_l = i, k_m = kmi, k_n = kni)
 // mode: update (mode STEP), DC removal, energy term, C2 products.
 template <typename R>
-__device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, int mode, bool want_e, int LL, int i,
+__device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, int mode, bool want_e, bool use_nz, int LL, int i,
                                          int kmi, int kni, Cx<R>& ph, Cx<R>& co, Cx<R> on, Cx<R> oc, Cx<R> nz,
                                          Cx<R>& a, Cx<R>& bb, double& g) {
     const int Mh = b.M / 2, Nh = b.N / 2;
@@ -518,7 +575,7 @@ __device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, in
         o.c[0] = on; o.c[1] = oc;
         Cx<R> nt{R(0), R(0)};
         // self-conjugate selector of the reference: l and m only (misfit.cpp:991-1000)
-        if (b.NZ) nt = noise_term(p, ks, mself && (i == 0 || i == LL / 2), nz);
+        if (use_nz) nt = noise_term(p, ks, mself && (i == 0 || i == LL / 2), nz);
         update_mode(p, ks, s, o, nt);
         ph = s.c[0]; co = s.c[1];
     }
@@ -544,13 +601,88 @@ __device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, in
     bb = Cx<R>{ph.x * cb, ph.y * cb};
 }
 
-template <typename R, int LL>
-__global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs p, int mode, int flags) {
+// Per-mode work of one column tile after the forward transform; x[q]
+// becomes {psi', con', C2A psi', C2B psi'} in place.  The packed column needs
+// each mode's mirror -k: it is exchanged through `xch` (PB Vecs, then PB noise
+// values), which no other thread may be using (the call starts and ends the
+// exchange with barriers when the tile is packed).
+// COEF (FP32 plain step): cf/cc hold the precomputed coefficients of the
+// column's modes (plain columns only; the packed column evaluates its own).
+template <typename R, int LL, int E, int G, bool COEF = false>
+__device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs& p, int mode, bool want_e,
+                                               bool use_nz, Vec<R, 4> (&x)[E], const Cx<R> (&nz)[E], int t, int j,
+                                               int kmi, int kni, bool packed_tile, bool packed, char* xch,
+                                               double& g, const float4* cf = nullptr, const float2* cc = nullptr) {
+    Vec<R, 4>* mx4 = (Vec<R, 4>*)xch;
+    Cx<R>* nzx = (Cx<R>*)(mx4 + padded(LL));
+    if (packed_tile) {
+        __syncthreads();
+        if (packed) {
+#pragma unroll
+            for (int q = 0; q < E; q++) {
+                const int i = pidx(t + G * q);
+                mx4[i] = x[q];
+                if (use_nz) nzx[i] = nz[q];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int i = t + G * q;
+        Cx<R> ph = x[q].c[0], co = x[q].c[1], a, bb;
+        if (COEF && !packed) {
+            if constexpr (COEF) {
+                Vec<float, 2> sv, ov;
+                sv.c[0] = ph; sv.c[1] = co;
+                ov.c[0] = x[q].c[2]; ov.c[1] = x[q].c[3];
+                update_mode_coef(cf[q], sv, ov, Cx<float>{0.0f, 0.0f});
+                ph = sv.c[0]; co = sv.c[1];
+                if (i == 0 && kmi == 0 && kni == 0) {
+                    ph = Cx<R>{R(0), R(0)};
+                    co = Cx<R>{R(0), R(0)};
+                }
+                a = Cx<R>{ph.x * cc[q].x, ph.y * cc[q].x};
+                bb = Cx<R>{ph.x * cc[q].y, ph.y * cc[q].y};
+            }
+        } else if (!packed) {
+            col_mode(b, p, mode, want_e, use_nz, LL, i, kmi, kni, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
+        } else {
+            const int im = pidx((LL - i) % LL);
+            const Vec<R, 4> xm = mx4[im];
+            const Cx<R> nzm = use_nz ? nzx[im] : Cx<R>{R(0), R(0)};
+            Cx<R> ph0, phh, co0, coh, on0, onh, oc0, och, nz0, nzh, a0, ah, b0, bh;
+            hsplit(x[q].c[0], xm.c[0], ph0, phh);
+            hsplit(x[q].c[1], xm.c[1], co0, coh);
+            hsplit(x[q].c[2], xm.c[2], on0, onh);
+            hsplit(x[q].c[3], xm.c[3], oc0, och);
+            hsplit(nz[q], nzm, nz0, nzh);
+            col_mode(b, p, mode, want_e, use_nz, LL, i, 0, j, ph0, co0, on0, oc0, nz0, a0, b0, g);
+            col_mode(b, p, mode, want_e, use_nz, LL, i, b.M / 2, j, phh, coh, onh, och, nzh, ah, bh, g);
+            ph = hjoin(ph0, phh);
+            co = hjoin(co0, coh);
+            a = hjoin(a0, ah);
+            bb = hjoin(b0, bh);
+        }
+        x[q].c[0] = ph;
+        x[q].c[1] = co;
+        x[q].c[2] = a;
+        x[q].c[3] = bb;
+    }
+}
+
+// LEAN: the plain step (mode STEP, no noise, no energy) with the other paths
+// compiled out, which frees registers for occupancy.
+template <typename R, int LL, bool LEAN>
+__global__ void __launch_bounds__((ColCfg<R, LL>::NT))
+__attribute__((amdgpu_waves_per_eu(LEAN ? ColCfg<R, LL>::WPE_LEAN : ColCfg<R, LL>::WPE)))
+k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     using C = ColCfg<R, LL>;
     constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT;
     __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     __shared__ double red[(NT + 63) / 64 + 1];
-    const int tid = threadIdx.x, col = tid % T, t = tid / T;
+    const int tid = threadIdx.x;
+    const int col = C::WAVE ? tid / G : tid % T, t = C::WAVE ? tid % G : tid / T;
     const int inst = blockIdx.y;
     if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
     const int K = b.K;
@@ -565,7 +697,7 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     const size_t tbytes = ((size_t)LL * CS - tile0) * es;    // rest of the instance
     const Buf S = mkbuf((const Vec<R, 2>*)b.S + tbase, tbytes);
     const Cx<R>* twL = (const Cx<R>*)b.twL;
-    const LdsBuf<R, 4, C::CH, C::PS> lds{smem + col * C::CB};
+    const typename C::Lds lds{smem + col * C::CB};
     const int Nh = b.N / 2;
     // Columns kc = 0 of slots j = 0 and N/2 carry two Hermitian-in-l sequences
     // (k_m = 0 and M/2); uniform per workgroup.
@@ -578,7 +710,24 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
         kmi = j < Nh ? 0 : b.M / 2;
         kni = j < Nh ? j : b.N - j;
     }
+    if (LEAN) mode = COL_STEP;
     const bool step = mode == COL_STEP;
+    const bool use_nz = !LEAN && step && b.NZ != nullptr;
+
+    // Plain FP32 steps take the modes' coefficients from cA/cC (same layout
+    // as one instance of S, loaded with the state so their latency overlaps).
+    constexpr bool COEF = LEAN && sizeof(R) == 4 && LL % 3 != 0;  // E = 12 has no registers to spare
+    float4 cf[COEF ? E : 1];
+    float2 cc[COEF ? E : 1];
+    if constexpr (COEF) {
+        const Buf CA = mkbuf((const float4*)b.cA + tile0, 0), CC = mkbuf((const float2*)b.cC + tile0, 0);
+        const uint32_t ca_off = (uint32_t)(col + t * CS) * 16u, cc_off = (uint32_t)(col + t * CS) * 8u;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            cf[q] = bload<float4>(CA, ca_off, (uint32_t)(G * q * CS) * 16u);
+            cc[q] = bload<float2>(CC, cc_off, (uint32_t)(G * q * CS) * 8u);
+        }
+    }
 
     // Forward transform of {psi, con, inn, inc} along l in one pass.
     Vec<R, 4> x[E];
@@ -600,7 +749,7 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
     Cx<R> nz[E];
 #pragma unroll
     for (int q = 0; q < E; q++) nz[q] = Cx<R>{R(0), R(0)};
-    if (step && b.NZ) {
+    if (use_nz) {
         Vec<R, 1> n1[E];
 #pragma unroll
         for (int q = 0; q < E; q++)
@@ -610,60 +759,16 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
 #pragma unroll
         for (int q = 0; q < E; q++) nz[q] = n1[q].c[0];
     }
-    // The packed column needs each mode's mirror -k: exchange it through LDS
-    // (column 0's buffer, and column 1's for the noise).
-    Vec<R, 4>* mx4 = (Vec<R, 4>*)smem;
-    Cx<R>* nzx = (Cx<R>*)(mx4 + C::PB);
-    if (packed_tile) {
-        __syncthreads();
-        if (packed) {
-#pragma unroll
-            for (int q = 0; q < E; q++) {
-                const int i = pidx(t + G * q);
-                mx4[i] = x[q];
-                if (step && b.NZ) nzx[i] = nz[q];
-            }
-        }
-        __syncthreads();
-    }
-
-    const bool want_e = (flags & COL_ENERGY) != 0;
+    const bool want_e = !LEAN && (flags & COL_ENERGY) != 0;
     double g = 0.0;
-    // x[q] becomes {psi', con', C2A psi', C2B psi'} in place.
-#pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int i = t + G * q;
-        Cx<R> ph = x[q].c[0], co = x[q].c[1], a, bb;
-        if (!packed) {
-            col_mode(b, p, mode, want_e, LL, i, kmi, kni, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
-        } else {
-            const int im = pidx((LL - i) % LL);
-            const Vec<R, 4> xm = mx4[im];
-            const Cx<R> nzm = (step && b.NZ) ? nzx[im] : Cx<R>{R(0), R(0)};
-            Cx<R> ph0, phh, co0, coh, on0, onh, oc0, och, nz0, nzh, a0, ah, b0, bh;
-            hsplit(x[q].c[0], xm.c[0], ph0, phh);
-            hsplit(x[q].c[1], xm.c[1], co0, coh);
-            hsplit(x[q].c[2], xm.c[2], on0, onh);
-            hsplit(x[q].c[3], xm.c[3], oc0, och);
-            hsplit(nz[q], nzm, nz0, nzh);
-            col_mode(b, p, mode, want_e, LL, i, 0, j, ph0, co0, on0, oc0, nz0, a0, b0, g);
-            col_mode(b, p, mode, want_e, LL, i, b.M / 2, j, phh, coh, onh, och, nzh, ah, bh, g);
-            ph = hjoin(ph0, phh);
-            co = hjoin(co0, coh);
-            a = hjoin(a0, ah);
-            bb = hjoin(b0, bh);
-        }
-        x[q].c[0] = ph;
-        x[q].c[1] = co;
-        x[q].c[2] = a;
-        x[q].c[3] = bb;
-    }
+    col_tile_modes<R, LL, E, G, COEF>(b, p, mode, want_e, use_nz, x, nz, t, j, kmi, kni, packed_tile, packed, smem, g,
+                                      cf, cc);
     if (step && b.tcount && blockIdx.x == 0 && tid == 0) b.tcount[inst] += 1;
     if (want_e) {
         const double gs = block_sum<NT>(g, red);
         if (tid == 0) b.ecol[(size_t)inst * gridDim.x + blockIdx.x] = gs;
     }
-    if (mode == COL_ENERGY_ONLY) return;
+    if (!LEAN && mode == COL_ENERGY_ONLY) return;
 
     // Inverse transform of the new state and of C2A/C2B times the new psi.
     fft_reg<R, 4, LL, E, +1>(x, lds, t, twL);
@@ -680,6 +785,39 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT)) k_col(DevBufs b, PhysArgs
         bstore(sv, S, toff, qo);
         bstore(av, AB, toff, qo);
     }
+}
+
+// Per-mode coefficients of the plain FP32 step in the column layout: entry
+// (l, n slot j, packed column kc) holds the mode k_col gives that column
+// (kc = 0 of slots other than 0 and N/2 carries k_m = 0 or M/2 with k_n
+// remapped); the packed columns' entries are unused.  Same float expressions
+// as update_mode, so the coefficient path is bit-identical.
+__global__ void __launch_bounds__(256) k_coef(DevBufs b, PhysArgs p) {
+    const long long n = (long long)b.L * b.N * b.K;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int K = b.K, Nh = b.N / 2, Mh = b.M / 2;
+    const int kc = (int)(e % K);
+    const int j = (int)((e / K) % b.N);
+    const int i = (int)(e / ((long long)K * b.N));
+    int kmi = kc, kni = j;
+    if (kc == 0) {
+        kmi = j < Nh ? 0 : Mh;
+        kni = j < Nh ? j : b.N - j;
+    }
+    const float ks = ksq(b.kl[i], b.km[kmi], b.kn[kni]);
+    const int il = i <= b.L / 2 ? i : b.L - i;
+    const int jn = kni <= Nh ? kni : b.N - kni;
+    const size_t ci = 2 * (((size_t)il * (Mh + 1) + kmi) * (Nh + 1) + jn);
+    ((float4*)const_cast<void*>(b.cA))[e] = mode_coef(p, ks);
+    ((float2*)const_cast<void*>(b.cC))[e] = make_float2(b.c2[ci], b.c2[ci + 1]);
+}
+
+hipError_t launch_coef(const DevBufs& b, const PhysArgs& p, hipStream_t s) {
+    if (!b.cA || !b.cC) return hipErrorInvalidValue;
+    const long long n = (long long)b.L * b.N * b.K;
+    hipLaunchKernelGGL(k_coef, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, p);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -919,8 +1057,14 @@ static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags
     const int nb = col_blocks_t<R>(b.L, b.M, b.N);
     if (nb <= 0) return hipErrorInvalidValue;
     dim3 grid(nb, b.B);
+    const bool lean = mode == COL_STEP && !b.NZ && !(flags & COL_ENERGY) &&
+                      (sizeof(R) == 8 || b.L % 3 == 0 || b.cA != nullptr);
     switch (b.L) {
-#define CS(LL) case LL: PFC_LAUNCH((k_col<R, LL>), grid, dim3(ColCfg<R, LL>::NT), s, t0, t1, b, p, mode, flags); break;
+#define CS(LL)                                                                                          \
+    case LL:                                                                                            \
+        if (lean) PFC_LAUNCH((k_col<R, LL, true>), grid, dim3(ColCfg<R, LL>::NT), s, t0, t1, b, p, mode, flags); \
+        else PFC_LAUNCH((k_col<R, LL, false>), grid, dim3(ColCfg<R, LL>::NT), s, t0, t1, b, p, mode, flags);    \
+        break;
         CS(32) CS(64) CS(128) CS(256) CS(384) CS(512) CS(768) CS(1024)
 #undef CS
         default: return hipErrorInvalidValue;
