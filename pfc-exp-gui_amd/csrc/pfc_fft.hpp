@@ -403,6 +403,135 @@ __device__ __forceinline__ void fft_reg(Vec<R, NC> (&x)[E], const L& lds, int t,
     fft_reg_passes<R, NC, N, E, DIR, 1, 0, L>(x, x, lds, t, tw);
 }
 
+// ---------------------------------------------------------------------------
+// Lengths N = 3*2^k with per-pass thread counts.
+//
+// Plan (forward order): radix 6, radix 8 while it divides, then 4 or 2; the
+// reversed plan runs the same passes backwards (a Stockham transform may take
+// its radices in any order).  A group of G = N/6 threads runs the transform;
+// pass p uses GP = the largest divisor of N/R_p not above G threads, each
+// doing N/(R_p*GP) butterflies, the other threads idle in that pass.  So a
+// thread never holds more than 8 elements (E = 12 with a fixed thread count
+// needs 12 and falls to one wave per SIMD).  Thread t enters the first pass
+// with elements t + GP0*q (q < N/GP0) and leaves the last with t + GPL*q.
+// ---------------------------------------------------------------------------
+constexpr int gen_npass(int n) {
+    int r = n / 6, c = 1;
+    while (r % 8 == 0) { r /= 8; c++; }
+    return r > 1 ? c + 1 : c;
+}
+constexpr int gen_radix_fwd(int n, int p) {
+    if (p == 0) return 6;
+    int r = n / 6;
+    for (int i = 1; i < p; i++) r /= 8;
+    return r % 8 == 0 ? 8 : r;
+}
+constexpr int gen_radix(int n, int p, bool rev) { return rev ? gen_radix_fwd(n, gen_npass(n) - 1 - p) : gen_radix_fwd(n, p); }
+constexpr int gen_threads(int n, int rad, int g) {
+    const int m = n / rad;
+    int t = g < m ? g : m;
+    while (m % t) t--;
+    return t;
+}
+// Elements per thread entering (first) or leaving (last) a plan.
+constexpr int gen_e_in(int n, bool rev) { return n / gen_threads(n, gen_radix(n, 0, rev), n / 6); }
+constexpr int gen_g_in(int n, bool rev) { return gen_threads(n, gen_radix(n, 0, rev), n / 6); }
+constexpr int gen_emax(int n) {
+    int e = 0;
+    for (int p = 0; p < gen_npass(n); p++) {
+        const int x = n / gen_threads(n, gen_radix(n, p, false), n / 6);
+        e = x > e ? x : e;
+    }
+    return e;
+}
+
+template <typename R, int NC, int N, int DIR, bool REV, int PI, int NS, int EM, typename L>
+__device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds, int t,
+                                               const Cx<R>* __restrict__ tw) {
+    if constexpr (NS < N) {
+        constexpr int G = N / 6;
+        constexpr int RAD = gen_radix(N, PI, REV);
+        constexpr int GP = gen_threads(N, RAD, G);
+        constexpr int NB = N / (RAD * GP), STEP = N / RAD;
+        constexpr bool LAST = (NS * RAD == N);
+        static_assert(RAD * NB <= EM, "register array too small for the plan");
+        const bool act = GP == G || t < GP;
+        Vec<R, NC> v[RAD * NB];
+        if (act) {
+            if constexpr (PI == 0) {
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+#pragma unroll
+                    for (int r = 0; r < RAD; r++) v[b * RAD + r] = x[b + r * NB];
+            } else {
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+#pragma unroll
+                    for (int r = 0; r < RAD; r++) v[b * RAD + r] = lds.ld(pidx(t + GP * b + r * STEP));
+            }
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int j = t + GP * b;
+                const int k = j % NS;
+                if constexpr (NS > 1) {
+#pragma unroll
+                    for (int r = 1; r < RAD; r++) {
+                        Cx<R> w = tw[r * k * (N / (NS * RAD))];
+                        if constexpr (DIR > 0) w.y = -w.y;
+                        v[b * RAD + r] = vmulw(v[b * RAD + r], w);
+                    }
+                }
+                bfly<RAD, DIR>(&v[b * RAD]);
+            }
+        }
+        if constexpr (LAST) {
+            if (act) {
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+#pragma unroll
+                    for (int r = 0; r < RAD; r++) x[b + r * NB] = v[b * RAD + r];
+            }
+        } else {
+            xsync<L::WSYNC>();
+            if (act) {
+#pragma unroll
+                for (int b = 0; b < NB; b++) {
+                    const int j = t + GP * b;
+                    const int idx = (j / NS) * NS * RAD + j % NS;
+#pragma unroll
+                    for (int r = 0; r < RAD; r++) lds.st(pidx(idx + r * NS), v[b * RAD + r]);
+                }
+            }
+            xsync<L::WSYNC>();
+            fft_gen_passes<R, NC, N, DIR, REV, PI + 1, NS * RAD, EM, L>(x, lds, t, tw);
+        }
+    }
+}
+
+// A transform between the two element mappings of a length: side A (the
+// loaded/stored side) and side B (the other).  Powers of two: fft_reg, the
+// same mapping on both sides (E elements, G = N/E threads).  N = 3*2^k: the
+// general plan, A -> B in forward order, B -> A reversed.
+template <typename R, int NC, int N, int EM, int DIR, bool A2B, bool GEN, typename L>
+__device__ __forceinline__ void fft_ab(Vec<R, NC> (&x)[EM], const L& lds, int t, const Cx<R>* __restrict__ tw) {
+    if constexpr (GEN) {
+        fft_gen_passes<R, NC, N, DIR, !A2B, 0, 1, EM, L>(x, lds, t, tw);
+    } else {
+        fft_reg<R, NC, N, EM, DIR>(x, lds, t, tw);
+    }
+}
+
+// Mappings of a length N on its two sides: thread counts and elements.
+// GEN: the general plan (N = 3*2^k); otherwise fft_reg with EPOW2 elements.
+template <int N, int EPOW2, bool USEGEN>
+struct Sides {
+    static constexpr bool GEN = USEGEN && N % 3 == 0;
+    static constexpr int EM = GEN ? gen_emax(N) : EPOW2;       // register elements
+    static constexpr int G = GEN ? N / 6 : N / EPOW2;          // threads per transform
+    static constexpr int GA = GEN ? gen_g_in(N, false) : G, EA = GEN ? gen_e_in(N, false) : EPOW2;
+    static constexpr int GB = GEN ? gen_g_in(N, true) : G, EB = GEN ? gen_e_in(N, true) : EPOW2;
+};
+
 // AoS exchange buffer (one Vec per slot).
 template <typename R, int NC, int N, int E, int DIR>
 __device__ __forceinline__ void fft_reg(Vec<R, NC> (&x)[E], Vec<R, NC>* lds, int t, const Cx<R>* __restrict__ tw) {

@@ -252,6 +252,16 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #ifndef PFC_LDS_MOD
 #define PFC_LDS_MOD 2
 #endif
+// 3*2^k lengths: the general plan (radix 6, then 8s; at most 8 elements per
+// thread) rather than 12 elements per thread.
+#ifndef PFC_GEN
+#define PFC_GEN 1
+#endif
+// With the general plan, one 384-column per wavefront (16-byte row pieces)
+// instead of four columns interleaved across the workgroup.
+#ifndef PFC_GEN_WAVE
+#define PFC_GEN_WAVE 0
+#endif
 constexpr int col_e(bool f64, int LL) {
     return LL % 3 == 0 ? 12 : (f64 ? (LL >= 1024 ? 8 : 4) : (LL == 512 ? 8 : 4));
 }
@@ -263,8 +273,12 @@ constexpr int col_t(bool f64, int LL) {
 // LDS: one padded exchange buffer of 4-field elements per row.
 template <typename R, int H>
 struct RowCfg {
-    static constexpr int E = elems_for(H, PFC_ROW_E);
-    static constexpr int G = H / E;
+    static constexpr bool GEN = PFC_GEN && H % 3 == 0;
+    using SD = Sides<H, elems_for(H, PFC_ROW_E), GEN>;
+    static constexpr int E = SD::EM, G = SD::G;  // register elements, threads per row
+    // Side A: the row spectrum (loads of S/AB, stores of IN); side B: real space.
+    static constexpr int EA = SD::EA, GA = SD::GA, EB = SD::EB, GB = SD::GB;
+    static_assert(GA == G, "side A uses every thread of the row");
     static constexpr int cap = sizeof(R) == 8 ? 4 : 8;
     static constexpr int want = PFC_ROW_THREADS / G;
     static constexpr int RPW = want < 1 ? 1 : (want > cap ? cap : want);
@@ -275,7 +289,7 @@ struct RowCfg {
     static constexpr int PS = PFC_LDS_MOD ? soa_stride(H, CH, PFC_LDS_MOD) : PB;
     static constexpr size_t RB = (size_t)CH * PS * (sizeof(Vec<R, 4>) / CH);  // bytes per row buffer
     static constexpr size_t LDS = (size_t)RPW * RB;
-    static constexpr int WPE = E == 12 ? PFC_WPE12 : PFC_WPE;  // waves per SIMD the register budget targets
+    static constexpr int WPE = (!GEN && E == 12) ? PFC_WPE12 : PFC_WPE;  // waves per SIMD the register budget targets
     // A row's G threads are consecutive; when they fit one wavefront the
     // row transforms exchange without workgroup barriers.
     static constexpr bool WAVE = PFC_ROW_WAVE != 0 && G <= 64 && 64 % G == 0;
@@ -287,8 +301,12 @@ struct RowCfg {
 template <typename R, int LL>
 struct ColCfg {
     static constexpr bool F64 = sizeof(R) == 8;
-    static constexpr int E = col_e(F64, LL);
-    static constexpr int G = LL / E;
+    static constexpr bool GEN = PFC_GEN && LL % 3 == 0;
+    using SD = Sides<LL, col_e(F64, LL), GEN>;
+    static constexpr int E = SD::EM, G = SD::G;  // register elements, threads per column
+    // Side A: the loaded/stored row spectra along l; side B: the full spectrum.
+    static constexpr int EA = SD::EA, GA = SD::GA, EB = SD::EB, GB = SD::GB;
+    static_assert(GA == G, "side A uses every thread of the column");
     static constexpr int T0 = col_t(F64, LL);
     static constexpr int T1 = T0 / PFC_COL_T_DIV > 0 ? T0 / PFC_COL_T_DIV : 1;
     // Wider tiles (PFC_COL_T_MUL) only while the exchange buffers fit in LDS.
@@ -300,22 +318,23 @@ struct ColCfg {
     static constexpr int PS = PFC_LDS_MOD ? soa_stride(LL, CH, PFC_LDS_MOD) : PB;
     // SPLIT: the exchanges move the 4-field Vec in 16-byte chunks through a
     // buffer of 16 bytes per slot (fewer LDS bytes per wave, more waves per CU).
-    static constexpr bool SPLIT = PFC_COL_SPLIT < 0 ? (!F64 && LL % 3 == 0) : PFC_COL_SPLIT != 0;
+    static constexpr bool SPLIT = GEN ? false : (PFC_COL_SPLIT < 0 ? (!F64 && LL % 3 == 0) : PFC_COL_SPLIT != 0);
     static constexpr int PS1 = soa_stride(LL, 1, PFC_LDS_MOD ? PFC_LDS_MOD : 1);
     static constexpr size_t CB = SPLIT ? (size_t)PS1 * 16
                                        : (size_t)CH * PS * (sizeof(Vec<R, 4>) / CH);  // bytes per column buffer
     // WAVE: each column's G threads sit in one wavefront (thread = col*G + t),
     // so the transform's exchanges need no workgroup barrier and the waves of
     // a workgroup run out of step with each other.
-    static constexpr bool WAVE = (PFC_COL_WAVE < 0 ? (!F64 && (LL % 3 == 0 || LL <= 128)) : PFC_COL_WAVE != 0) &&
+    static constexpr bool WAVE = (GEN ? PFC_GEN_WAVE != 0
+                                      : (PFC_COL_WAVE < 0 ? (!F64 && (LL % 3 == 0 || LL <= 128)) : PFC_COL_WAVE != 0)) &&
                                  G <= 64 && 64 % G == 0;
     using Lds = std::conditional_t<SPLIT, LdsSplit<R, 4, WAVE>, LdsBuf<R, 4, CH, PS, WAVE>>;
     // T column buffers; the packed column's mirror exchange (4-field Vec and
     // noise value per slot) reuses the space from the start.
     static constexpr size_t MIRROR = (size_t)PB * (sizeof(Vec<R, 4>) + sizeof(Cx<R>));
     static constexpr size_t LDS = (size_t)T * CB > MIRROR ? (size_t)T * CB : MIRROR;
-    static constexpr int WPE = E == 12 ? PFC_WPE12 : PFC_WPE;
-    static constexpr int WPE_LEAN = E == 12 ? PFC_WPE12 : PFC_COL_WPE_LEAN;
+    static constexpr int WPE = (!GEN && E == 12) ? PFC_WPE12 : PFC_WPE;
+    static constexpr int WPE_LEAN = (!GEN && E == 12) ? PFC_WPE12 : PFC_COL_WPE_LEAN;
 };
 
 // ---------------------------------------------------------------------------
@@ -347,19 +366,21 @@ __device__ __forceinline__ Cx<R> r2c_merge(Cx<R> wk, Cx<R> wc, Cx<R> tw) {
 // they are stored as one element X[0] + i X[H], so a row holds H elements.
 // r2c of w[q] (= x[2j] + i x[2j+1] per field, j = t + G*q), stored packed
 // to dst[0..H-1].  `lds` is this row's padded buffer viewed as Vec<R, NC>.
-template <typename R, int H, int E, int NC, bool WAVE = false>
-__device__ __forceinline__ void row_r2c_store(Vec<R, NC> (&w)[E], Vec<R, NC>* lds, int t,
+// w arrives in the row configuration C's side-B mapping (real space) and the
+// spectrum leaves in its side-A mapping.
+template <typename R, typename C, int NC>
+__device__ __forceinline__ void row_r2c_store(Vec<R, NC> (&w)[C::E], Vec<R, NC>* lds, int t,
                                               const Cx<R>* __restrict__ twH, const Cx<R>* __restrict__ twM,
                                               Buf dst, uint32_t doff) {
-    constexpr int G = H / E;
-    fft_reg<R, NC, H, E, -1>(w, LdsBuf<R, NC, 1, 0, WAVE>{(char*)lds}, t, twH);
-    xsync<WAVE>();
+    constexpr int H = C::K, EA = C::EA, GA = C::GA;
+    fft_ab<R, NC, H, C::E, -1, false, C::GEN>(w, LdsBuf<R, NC, 1, 0, C::WAVE>{(char*)lds}, t, twH);
+    xsync<C::WAVE>();
 #pragma unroll
-    for (int q = 0; q < E; q++) lds[pidx(t + G * q)] = w[q];
-    xsync<WAVE>();
+    for (int q = 0; q < EA; q++) lds[pidx(t + GA * q)] = w[q];
+    xsync<C::WAVE>();
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int k = t + G * q;
+    for (int q = 0; q < EA; q++) {
+        const int k = t + GA * q;
         Vec<R, NC> X;
         if (k == 0) {
             // X[0] = Re W0 + Im W0, X[H] = Re W0 - Im W0, both real.
@@ -405,9 +426,11 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
     const Cx<R>* twM = (const Cx<R>*)b.twM;
 
     // One 4-field c2r: psi, con (from S) and a, b (from AB).
+    constexpr int EA = C::EA, EB = C::EB, GB = C::GB;
+    const bool actB = GB == G || t < GB;  // holds side-B elements
     Vec<R, 4> z[E];
 #pragma unroll
-    for (int q = 0; q < E; q++) {
+    for (int q = 0; q < EA; q++) {
         const int k = t + G * q;
         const int kc = (H - k) % H;
         constexpr uint32_t es = sizeof(Vec<R, 2>);
@@ -419,7 +442,7 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
         z[q].c[2] = c2r_split_packed(a0.c[0], a1.c[0], k, w);
         z[q].c[3] = c2r_split_packed(a0.c[1], a1.c[1], k, w);
     }
-    fft_reg<R, 4, H, E, +1>(z, LdsBuf<R, 4, C::CH, C::PS, C::WAVE>{smem + rl * C::RB}, t, twH);
+    fft_ab<R, 4, H, E, +1, true, C::GEN>(z, LdsBuf<R, 4, C::CH, C::PS, C::WAVE>{smem + rl * C::RB}, t, twH);
 
     const double* mu = b.means + 4 * inst;
     const R mph = (R)mu[0], mco = (R)mu[1], ma = (R)mu[2], mb = (R)mu[3];
@@ -428,7 +451,7 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
     double esum = 0.0;
     const bool want_e = (flags & ROW_ENERGY) != 0;
 #pragma unroll
-    for (int q = 0; q < E; q++) {
+    for (int q = 0; q < EB; q++) {
         const R ph[2] = {mph + z[q].c[0].x * invM, mph + z[q].c[0].y * invM};
         const R co[2] = {mco + z[q].c[1].x * invM, mco + z[q].c[1].y * invM};
         const R aa[2] = {ma + z[q].c[2].x * invM, ma + z[q].c[2].y * invM};
@@ -437,13 +460,13 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
 #pragma unroll
         for (int s2 = 0; s2 < 2; s2++) {
             nonlinear(p, ph[s2], co[s2], aa[s2], bb[s2], inn[s2], inc[s2], in3[s2]);
-            if (want_e) esum += energy_point<R>(p, ph[s2], co[s2], in3[s2]);
+            if (want_e && actB) esum += energy_point<R>(p, ph[s2], co[s2], in3[s2]);
         }
         w[q].c[0] = Cx<R>{inn[0], inn[1]};
         w[q].c[1] = Cx<R>{inc[0], inc[1]};
     }
     if (flags & ROW_WRITE_IN)
-        row_r2c_store<R, H, E, 2, C::WAVE>(w, (Vec<R, 2>*)(smem + rl * C::RB), t, twH, twM,
+        row_r2c_store<R, C, 2>(w, (Vec<R, 2>*)(smem + rl * C::RB), t, twH, twM,
                                   mkbuf((Vec<R, 2>*)b.IN + bbase, rows_bytes), roff);
     if (want_e) {
         const double se = block_sum<C::NT>(esum, red);
@@ -469,13 +492,15 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_in(DevBufs b, const 
     const size_t ps = b.N;  // real point (l, m, n) of row l*N + n at pbase + m*N
     const size_t pbase = ((size_t)inst * b.L + row / b.N) * M * ps + row % b.N;
     Vec<R, 2> w[E];
+    if (C::GB == G || t < C::GB) {
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int j = t + G * q;
-        w[q].c[0] = Cx<R>{inphi[pbase + 2 * j * ps], inphi[pbase + (2 * j + 1) * ps]};
-        w[q].c[1] = Cx<R>{incon[pbase + 2 * j * ps], incon[pbase + (2 * j + 1) * ps]};
+        for (int q = 0; q < C::EB; q++) {
+            const int j = t + C::GB * q;
+            w[q].c[0] = Cx<R>{inphi[pbase + 2 * j * ps], inphi[pbase + (2 * j + 1) * ps]};
+            w[q].c[1] = Cx<R>{incon[pbase + 2 * j * ps], incon[pbase + (2 * j + 1) * ps]};
+        }
     }
-    row_r2c_store<R, H, E, 2>(w, (Vec<R, 2>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
+    row_r2c_store<R, C, 2>(w, (Vec<R, 2>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
                            mkbuf((Vec<R, 2>*)b.S + rbase, K * sizeof(Vec<R, 2>)), 0u);
 }
 
@@ -494,12 +519,14 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_r2c1(DevBufs b, cons
     const size_t ps = b.N;
     const size_t pbase = ((size_t)inst * b.L + row / b.N) * M * ps + row % b.N;
     Vec<R, 1> w[E];
+    if (C::GB == G || t < C::GB) {
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int j = t + G * q;
-        w[q].c[0] = Cx<R>{in[pbase + 2 * j * ps], in[pbase + (2 * j + 1) * ps]};
+        for (int q = 0; q < C::EB; q++) {
+            const int j = t + C::GB * q;
+            w[q].c[0] = Cx<R>{in[pbase + 2 * j * ps], in[pbase + (2 * j + 1) * ps]};
+        }
     }
-    row_r2c_store<R, H, E, 1>(w, (Vec<R, 1>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
+    row_r2c_store<R, C, 1>(w, (Vec<R, 1>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH, (const Cx<R>*)b.twM,
                            mkbuf(out + rbase, K * sizeof(Vec<R, 1>)), 0u);
 }
 
@@ -520,7 +547,7 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __
     const Cx<R>* twM = (const Cx<R>*)b.twM;
     Vec<R, 2> z[E];
 #pragma unroll
-    for (int q = 0; q < E; q++) {
+    for (int q = 0; q < C::EA; q++) {
         const int k = t + G * q;
         const int kc = (H - k) % H;
         const Vec<R, 2> s0 = S[k], s1 = S[kc];
@@ -528,13 +555,15 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __
         z[q].c[0] = c2r_split_packed(s0.c[0], s1.c[0], k, w);
         z[q].c[1] = c2r_split_packed(s0.c[1], s1.c[1], k, w);
     }
-    fft_reg<R, 2, H, E, +1>(z, (Vec<R, 2>*)smem + rl * C::PB, t, (const Cx<R>*)b.twH);
+    fft_ab<R, 2, H, E, +1, true, C::GEN>(z, LdsBuf<R, 2, 1, 0>{smem + rl * C::PB * sizeof(Vec<R, 2>)}, t,
+                                          (const Cx<R>*)b.twH);
     const double* mu = b.means + 4 * inst;
     const R mph = (R)mu[0], mco = (R)mu[1];
     const R invM = R(1) / R(M);
+    if (!(C::GB == G || t < C::GB)) return;
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int j = t + G * q;
+    for (int q = 0; q < C::EB; q++) {
+        const int j = t + C::GB * q;
         outphi[pbase + 2 * j * ps] = mph + z[q].c[0].x * invM;
         outphi[pbase + (2 * j + 1) * ps] = mph + z[q].c[0].y * invM;
         outcon[pbase + 2 * j * ps] = mco + z[q].c[1].x * invM;
@@ -608,16 +637,19 @@ __device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, in
 // exchange with barriers when the tile is packed).
 // COEF (FP32 plain step): cf/cc hold the precomputed coefficients of the
 // column's modes (plain columns only; the packed column evaluates its own).
-template <typename R, int LL, int E, int G, bool COEF = false>
+// x and nz hold the column's full-spectrum elements t + G*q (q < E) in
+// threads t < G (GF: the column's thread count; the others hold none).
+template <typename R, int LL, int EM, int E, int G, int GF, bool COEF = false>
 __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs& p, int mode, bool want_e,
-                                               bool use_nz, Vec<R, 4> (&x)[E], const Cx<R> (&nz)[E], int t, int j,
+                                               bool use_nz, Vec<R, 4> (&x)[EM], const Cx<R> (&nz)[EM], int t, int j,
                                                int kmi, int kni, bool packed_tile, bool packed, char* xch,
                                                double& g, const float4* cf = nullptr, const float2* cc = nullptr) {
+    const bool act = G == GF || t < G;
     Vec<R, 4>* mx4 = (Vec<R, 4>*)xch;
     Cx<R>* nzx = (Cx<R>*)(mx4 + padded(LL));
     if (packed_tile) {
         __syncthreads();
-        if (packed) {
+        if (packed && act) {
 #pragma unroll
             for (int q = 0; q < E; q++) {
                 const int i = pidx(t + G * q);
@@ -627,6 +659,7 @@ __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs&
         }
         __syncthreads();
     }
+    if (!act) return;
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int i = t + G * q;
@@ -678,7 +711,7 @@ __global__ void __launch_bounds__((ColCfg<R, LL>::NT))
 __attribute__((amdgpu_waves_per_eu(LEAN ? ColCfg<R, LL>::WPE_LEAN : ColCfg<R, LL>::WPE)))
 k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     using C = ColCfg<R, LL>;
-    constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT;
+    constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT, EA = C::EA;
     __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     __shared__ double red[(NT + 63) / 64 + 1];
     const int tid = threadIdx.x;
@@ -732,7 +765,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     // Forward transform of {psi, con, inn, inc} along l in one pass.
     Vec<R, 4> x[E];
 #pragma unroll
-    for (int q = 0; q < E; q++) {
+    for (int q = 0; q < EA; q++) {
         const uint32_t qo = (uint32_t)(G * q * CS) * es;  // uniform
         const Vec<R, 2> sv = bload<Vec<R, 2>>(S, toff, qo);
         x[q].c[0] = sv.c[0];
@@ -745,24 +778,24 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
             x[q].c[2] = x[q].c[3] = Cx<R>{R(0), R(0)};
         }
     }
-    fft_reg<R, 4, LL, E, -1>(x, lds, t, twL);
+    fft_ab<R, 4, LL, E, -1, true, C::GEN>(x, lds, t, twL);
     Cx<R> nz[E];
 #pragma unroll
     for (int q = 0; q < E; q++) nz[q] = Cx<R>{R(0), R(0)};
     if (use_nz) {
         Vec<R, 1> n1[E];
 #pragma unroll
-        for (int q = 0; q < E; q++)
+        for (int q = 0; q < EA; q++)
             n1[q] = bload<Vec<R, 1>>(mkbuf((const Vec<R, 1>*)b.NZ + tbase, tbytes / 2), toff / 2,
                                      (uint32_t)(G * q * CS) * (es / 2));
-        fft_reg<R, 1, LL, E, -1>(n1, (Vec<R, 1>*)(smem + col * C::CB), t, twL);
+        fft_ab<R, 1, LL, E, -1, true, C::GEN>(n1, LdsBuf<R, 1, 1, 0>{smem + col * C::CB}, t, twL);
 #pragma unroll
         for (int q = 0; q < E; q++) nz[q] = n1[q].c[0];
     }
     const bool want_e = !LEAN && (flags & COL_ENERGY) != 0;
     double g = 0.0;
-    col_tile_modes<R, LL, E, G, COEF>(b, p, mode, want_e, use_nz, x, nz, t, j, kmi, kni, packed_tile, packed, smem, g,
-                                      cf, cc);
+    col_tile_modes<R, LL, E, C::EB, C::GB, G, COEF>(b, p, mode, want_e, use_nz, x, nz, t, j, kmi, kni, packed_tile,
+                                                    packed, smem, g, cf, cc);
     if (step && b.tcount && blockIdx.x == 0 && tid == 0) b.tcount[inst] += 1;
     if (want_e) {
         const double gs = block_sum<NT>(g, red);
@@ -771,11 +804,11 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     if (!LEAN && mode == COL_ENERGY_ONLY) return;
 
     // Inverse transform of the new state and of C2A/C2B times the new psi.
-    fft_reg<R, 4, LL, E, +1>(x, lds, t, twL);
+    fft_ab<R, 4, LL, E, +1, false, C::GEN>(x, lds, t, twL);
     const R invL = R(1) / R(LL);
     const Buf AB = mkbuf((const Vec<R, 2>*)b.AB + tbase, tbytes);
 #pragma unroll
-    for (int q = 0; q < E; q++) {
+    for (int q = 0; q < EA; q++) {
         const uint32_t qo = (uint32_t)(G * q * CS) * es;
         Vec<R, 2> sv, av;
         sv.c[0] = Cx<R>{x[q].c[0].x * invL, x[q].c[0].y * invL};

@@ -58,17 +58,22 @@ def test_fp64_narrow(hip, oracle, kind):
     assert err < 1e-6
 
 
-@pytest.mark.parametrize("kind", [0, 1])
-def test_fp32(hip, oracle, kind):
+@pytest.mark.parametrize("kind,L,M,tol", [(0, 64, 64, 2e-4), (1, 64, 64, 2e-4), (0, 384, 128, 2e-4),
+                                          (1, 128, 384, 1e-3)])
+def test_fp32(hip, oracle, kind, L, M, tol):
+    """FP32 against the FP64 oracle after 50 steps; the 3*2^k grids run the
+    general radix-6/8 plan in both passes.  CVD's FP32 error grows with the
+    grid (measured: 3.8e-4 at 128x256, 2.8e-4 at 128x512, 5.2e-4 at 128x384;
+    6.6e-4 at 384^2), hence 1e-3 for its larger case."""
     kw = dict(grainx=4, grainy=4) if kind == 0 else {}
-    sim, osim = _pair(hip, oracle, kind, 64, 64, 1, hip.FP32, **kw)
+    sim, osim = _pair(hip, oracle, kind, L, M, 1, hip.FP32, **kw)
     sim.step(50)
     osim.run(50)
     phi, con = sim.get_fields()
     ophi, ocon = osim.fields()
     err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
-    print(f"kind={kind} fp32 err={err:.3e}", flush=True)
-    assert err < 2e-4
+    print(f"kind={kind} {L}x{M} fp32 err={err:.3e}", flush=True)
+    assert err < tol
     # k=0 modes held in FP64: means conserved exactly as the oracle's
     assert abs(phi.mean() - ophi.mean()) < 1e-6
 
