@@ -257,6 +257,12 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #ifndef PFC_GEN
 #define PFC_GEN 1
 #endif
+// XCD-aware column tiles: workgroup x of an instance takes tile
+// (x % 8) * (nx / 8) + x / 8, so the workgroups one XCD receives (every
+// eighth) cover adjacent columns and share its L2's lines.
+#ifndef PFC_XCD_REMAP
+#define PFC_XCD_REMAP 1
+#endif
 // With the general plan, one 384-column per wavefront (16-byte row pieces)
 // instead of four columns interleaved across the workgroup.
 #ifndef PFC_GEN_WAVE
@@ -720,7 +726,9 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
     const int K = b.K;
     const size_t CS = (size_t)b.N * K;       // stride between consecutive l
-    const int tile0 = blockIdx.x * T;        // first column of the tile; T divides K
+    const int bx = (PFC_XCD_REMAP && gridDim.x % 8 == 0) ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8
+                                                         : blockIdx.x;
+    const int tile0 = bx * T;                // first column of the tile; T divides K
     const int j = tile0 / K;                 // n slot of the whole tile
     const int kc = tile0 - j * K + col;      // packed m column (kc = 0 holds k_m = 0 and M/2)
     // Element i of this column: uniform tile base + 32-bit offset col + i*CS.
