@@ -261,6 +261,8 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     ph.mc_dt_a = m.mc * m.dt * m.alpha;
     ph.narrow = p->narrow ? 1 : 0;
     ph.db = c->noise ? m.dt * m.b0 : 0.0f;
+    ph.lnc0 = (float)std::log((double)m.c0);
+    ph.ln1c0 = (float)std::log(1.0 - (double)m.c0);
     DevBufs& b = c->bufs;
     b.S = c->S; b.AB = c->AB; b.IN = c->IN;
     b.means = c->means_d;

@@ -31,6 +31,8 @@ struct PhysArgs {
     float mc_dt_a;  // dynamic_mc*dt*para_alpha (float, misfit.cpp:1009)
     int narrow;     // reference float narrowing of Rhat/inc/energy terms
     float db;       // dt*b0 (float, misfit.cpp:1002); 0 = no noise term
+    float lnc0;     // ln(c0), ln(1 - c0): the FP32 nonlinear term's log offsets
+    float ln1c0;
 };
 
 struct DevBufs {

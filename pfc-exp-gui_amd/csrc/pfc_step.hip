@@ -81,11 +81,13 @@ __device__ __forceinline__ void nonlinear(const PhysArgs& p, double ph, double c
         inc = t1 - t2 - t3 - t4;
 }
 
+// FP32: log(c/c0) = ln2 * log2(c) - ln(c0) with the hardware log2
+// (v_log_f32, about 1 ulp) and ln(c0), ln(1 - c0) precomputed on the host.
 __device__ __forceinline__ void nonlinear(const PhysArgs& p, float ph, float c, float a, float b, float& inn,
                                           float& inc, float& in3) {
-    const float c0 = p.c0;
-    const float lc = logf(c / c0);
-    const float lc1 = logf((1.0f - c) / (1.0f - c0));
+    constexpr float LN2 = 0.693147180559945309f;
+    const float lc = __builtin_fmaf(LN2, __builtin_amdgcn_logf(c), -p.lnc0);
+    const float lc1 = __builtin_fmaf(LN2, __builtin_amdgcn_logf(1.0f - c), -p.ln1c0);
     const float ph2 = ph * ph;
     const float in1 = -p.niu * ph2 * 0.5f + p.ksi * (ph2 * ph) * (1.0f / 3.0f);
     const float in2 = p.w * (c * lc + (1.0f - c) * lc1);
@@ -215,7 +217,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // FP32) and the single-instance FP64 path; the PFC_* macros override them
 // for timing sweeps.
 #ifndef PFC_ROW_THREADS
-#define PFC_ROW_THREADS 256
+#define PFC_ROW_THREADS 128
 #endif
 #ifndef PFC_COL_T_DIV
 #define PFC_COL_T_DIV 1
