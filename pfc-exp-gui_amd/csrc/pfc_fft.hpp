@@ -314,12 +314,25 @@ constexpr int soa_stride(int n, int CH, int MOD) {
     return ps;
 }
 
+// Twiddles of the pass after one with stride NS and radix RAD, per
+// butterfly b < NB2 and r = 1 .. RAD2-1: loaded before the exchange barrier,
+// so their latency overlaps the exchange instead of following it.
+template <typename R, int N, int NS2, int RAD2, int NB2, int GP2>
+__device__ __forceinline__ void tw_fetch(Cx<R>* w, int t, const Cx<R>* __restrict__ tw) {
+#pragma unroll
+    for (int b = 0; b < NB2; b++) {
+        const int k = (t + GP2 * b) % NS2;
+#pragma unroll
+        for (int r = 1; r < RAD2; r++) w[b * (RAD2 - 1) + (r - 1)] = tw[r * k * (N / (NS2 * RAD2))];
+    }
+}
+
 // SRC: where the pass's inputs are: 0 = x in natural order (first pass),
 // 1 = the LDS buffer, 2 = `in`, already in this pass's butterfly order (after
-// a chunked exchange).
+// a chunked exchange).  wp: this pass's twiddles (passes after the first).
 template <typename R, int NC, int N, int E, int DIR, int NS, int SRC, typename L>
 __device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], Vec<R, NC> (&in)[E], const L& lds, int t,
-                                               const Cx<R>* __restrict__ tw) {
+                                               const Cx<R>* __restrict__ tw, const Cx<R>* wp) {
     if constexpr (NS < N) {
         constexpr int RAD = pick_radix(N / NS, E);
         static_assert(RAD > 0, "unsupported FFT length / elements-per-thread");
@@ -342,12 +355,10 @@ __device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], Vec<R, NC> (&
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
-            const int j = t + G * b;
-            const int k = j % NS;
             if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < RAD; r++) {
-                    Cx<R> w = tw[r * k * (N / (NS * RAD))];
+                    Cx<R> w = wp[b * (RAD - 1) + (r - 1)];
                     if constexpr (DIR > 0) w.y = -w.y;
                     v[b * RAD + r] = vmulw(v[b * RAD + r], w);
                 }
@@ -359,39 +370,43 @@ __device__ __forceinline__ void fft_reg_passes(Vec<R, NC> (&x)[E], Vec<R, NC> (&
             for (int b = 0; b < NB; b++)
 #pragma unroll
                 for (int r = 0; r < RAD; r++) x[b + r * NB] = v[b * RAD + r];
-        } else if constexpr (L::SPLIT) {
+        } else {
             constexpr int NS2 = NS * RAD;
             constexpr int RAD2 = pick_radix(N / NS2, E);
             constexpr int NB2 = E / RAD2, STEP2 = N / RAD2;
-            Vec<R, NC> w[E];
+            Cx<R> wn[NB2 * (RAD2 - 1)];
+            tw_fetch<R, N, NS2, RAD2, NB2, G>(wn, t, tw);
+            if constexpr (L::SPLIT) {
+                Vec<R, NC> w[E];
 #pragma unroll
-            for (int c = 0; c < L::NCH; c++) {
+                for (int c = 0; c < L::NCH; c++) {
+                    xsync<L::WSYNC>();
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+                        const int j = t + G * b;
+                        const int idx = (j / NS) * NS * RAD + j % NS;
+#pragma unroll
+                        for (int r = 0; r < RAD; r++) lds.stc(c, pidx(idx + r * NS), v[b * RAD + r]);
+                    }
+                    xsync<L::WSYNC>();
+#pragma unroll
+                    for (int b = 0; b < NB2; b++)
+#pragma unroll
+                        for (int r = 0; r < RAD2; r++) lds.ldc(c, pidx(t + G * b + r * STEP2), w[b * RAD2 + r]);
+                }
+                fft_reg_passes<R, NC, N, E, DIR, NS2, 2, L>(x, w, lds, t, tw, wn);
+            } else {
                 xsync<L::WSYNC>();
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
                     const int j = t + G * b;
                     const int idx = (j / NS) * NS * RAD + j % NS;
 #pragma unroll
-                    for (int r = 0; r < RAD; r++) lds.stc(c, pidx(idx + r * NS), v[b * RAD + r]);
+                    for (int r = 0; r < RAD; r++) lds.st(pidx(idx + r * NS), v[b * RAD + r]);
                 }
                 xsync<L::WSYNC>();
-#pragma unroll
-                for (int b = 0; b < NB2; b++)
-#pragma unroll
-                    for (int r = 0; r < RAD2; r++) lds.ldc(c, pidx(t + G * b + r * STEP2), w[b * RAD2 + r]);
+                fft_reg_passes<R, NC, N, E, DIR, NS2, 1, L>(x, in, lds, t, tw, wn);
             }
-            fft_reg_passes<R, NC, N, E, DIR, NS2, 2, L>(x, w, lds, t, tw);
-        } else {
-            xsync<L::WSYNC>();
-#pragma unroll
-            for (int b = 0; b < NB; b++) {
-                const int j = t + G * b;
-                const int idx = (j / NS) * NS * RAD + j % NS;
-#pragma unroll
-                for (int r = 0; r < RAD; r++) lds.st(pidx(idx + r * NS), v[b * RAD + r]);
-            }
-            xsync<L::WSYNC>();
-            fft_reg_passes<R, NC, N, E, DIR, NS * RAD, 1, L>(x, in, lds, t, tw);
         }
     }
 }
@@ -400,7 +415,7 @@ template <typename R, int NC, int N, int E, int DIR, typename L>
 __device__ __forceinline__ void fft_reg(Vec<R, NC> (&x)[E], const L& lds, int t, const Cx<R>* __restrict__ tw) {
     static_assert(N % E == 0, "length must be a multiple of elements per thread");
     static_assert(plan_ok(N, E), "no radix plan");
-    fft_reg_passes<R, NC, N, E, DIR, 1, 0, L>(x, x, lds, t, tw);
+    fft_reg_passes<R, NC, N, E, DIR, 1, 0, L>(x, x, lds, t, tw, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -447,7 +462,7 @@ constexpr int gen_emax(int n) {
 
 template <typename R, int NC, int N, int DIR, bool REV, int PI, int NS, int EM, typename L>
 __device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds, int t,
-                                               const Cx<R>* __restrict__ tw) {
+                                               const Cx<R>* __restrict__ tw, const Cx<R>* wp) {
     if constexpr (NS < N) {
         constexpr int G = N / 6;
         constexpr int RAD = gen_radix(N, PI, REV);
@@ -471,12 +486,10 @@ __device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds
             }
 #pragma unroll
             for (int b = 0; b < NB; b++) {
-                const int j = t + GP * b;
-                const int k = j % NS;
                 if constexpr (NS > 1) {
 #pragma unroll
                     for (int r = 1; r < RAD; r++) {
-                        Cx<R> w = tw[r * k * (N / (NS * RAD))];
+                        Cx<R> w = wp[b * (RAD - 1) + (r - 1)];
                         if constexpr (DIR > 0) w.y = -w.y;
                         v[b * RAD + r] = vmulw(v[b * RAD + r], w);
                     }
@@ -492,6 +505,12 @@ __device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds
                     for (int r = 0; r < RAD; r++) x[b + r * NB] = v[b * RAD + r];
             }
         } else {
+            constexpr int NS2 = NS * RAD;
+            constexpr int RAD2 = gen_radix(N, PI + 1, REV);
+            constexpr int GP2 = gen_threads(N, RAD2, G);
+            constexpr int NB2 = N / (RAD2 * GP2);
+            Cx<R> wn[NB2 * (RAD2 - 1)];
+            if (GP2 == G || t < GP2) tw_fetch<R, N, NS2, RAD2, NB2, GP2>(wn, t, tw);
             xsync<L::WSYNC>();
             if (act) {
 #pragma unroll
@@ -503,7 +522,7 @@ __device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds
                 }
             }
             xsync<L::WSYNC>();
-            fft_gen_passes<R, NC, N, DIR, REV, PI + 1, NS * RAD, EM, L>(x, lds, t, tw);
+            fft_gen_passes<R, NC, N, DIR, REV, PI + 1, NS2, EM, L>(x, lds, t, tw, wn);
         }
     }
 }
@@ -515,7 +534,7 @@ __device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds
 template <typename R, int NC, int N, int EM, int DIR, bool A2B, bool GEN, typename L>
 __device__ __forceinline__ void fft_ab(Vec<R, NC> (&x)[EM], const L& lds, int t, const Cx<R>* __restrict__ tw) {
     if constexpr (GEN) {
-        fft_gen_passes<R, NC, N, DIR, !A2B, 0, 1, EM, L>(x, lds, t, tw);
+        fft_gen_passes<R, NC, N, DIR, !A2B, 0, 1, EM, L>(x, lds, t, tw, nullptr);
     } else {
         fft_reg<R, NC, N, EM, DIR>(x, lds, t, tw);
     }
