@@ -310,18 +310,18 @@ COPY_PROBE_BYTES = 256 * 1024 * 1024
 
 def pmc_drive(hip, steps):
     """A few sweep steps and plain copy kernels of known size (256 MiB read
-    and written; linear, and in the column pass's 64-byte pieces; each
-    launched twice) for hardware-counter runs: the copies calibrate the byte
-    counters."""
+    and written; linear, and in the column pass's 64-byte pieces in plain and
+    in its XCD-aware tile order; each launched twice) for hardware-counter
+    runs: the copies calibrate the byte counters."""
     sims = build_sweep(hip, 0)
     for s in sims.values():
         s.step(steps)
         s.sync()
-    ms = [hip.copy_probe(COPY_PROBE_BYTES, pat) for pat in (0, 1)]
+    names = ("linear", "columns_64B", "columns_64B_xcd")
+    ms = [hip.copy_probe(COPY_PROBE_BYTES, pat) for pat in (0, 1, 2)]
     print(json.dumps({"pmc_drive": True, "steps": steps, "copy_bytes_each_way": COPY_PROBE_BYTES,
-                      "copy_ms": {"linear": ms[0], "columns_64B": ms[1]},
-                      "copy_GBps": {"linear": 2 * COPY_PROBE_BYTES / ms[0] / 1e6,
-                                    "columns_64B": 2 * COPY_PROBE_BYTES / ms[1] / 1e6}}))
+                      "copy_ms": dict(zip(names, ms)),
+                      "copy_GBps": {k: 2 * COPY_PROBE_BYTES / m / 1e6 for k, m in zip(names, ms)}}))
 
 
 def main():

@@ -154,8 +154,9 @@ int pfc_kernel_bytes(pfc_ctx* ctx, int kernel, double* bytes);
 /* Measurement aid, no counterpart in the reference: two launches of a plain
  * device copy of `nbytes` (a multiple of 4096; reads and writes nbytes each),
  * the second timed.  pattern 0: linear; 1: 64-byte pieces, column-wise over
- * 4096-byte rows (the column pass's access granularity).  Calibrates hardware
- * byte counters against kernels of known traffic. */
+ * 4096-byte rows (the column pass's access granularity); 2: as 1 with the
+ * column pass's XCD-aware tile order.  Calibrates hardware byte counters
+ * against kernels of known traffic. */
 int pfc_copy_probe(size_t nbytes, int pattern, double* ms);
 
 /* Free and total device memory of the current GPU (hipMemGetInfo). */

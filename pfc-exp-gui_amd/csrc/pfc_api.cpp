@@ -90,7 +90,7 @@ size_t ctx_bytes(const pfc_params& p, int batch) {
 }  // namespace pfc
 
 extern "C" int pfc_copy_probe(size_t nbytes, int pattern, double* ms) {
-    if (nbytes < 4096 || !ms || pattern < 0 || pattern > 1) return fail(PFC_E_ARG, "bad arguments");
+    if (nbytes < 4096 || !ms || pattern < 0 || pattern > 2) return fail(PFC_E_ARG, "bad arguments");
     const size_t n = nbytes / 4096 * 256;  // whole rows of 256 words
     void *a = nullptr, *b = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;

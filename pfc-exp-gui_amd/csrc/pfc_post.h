@@ -37,7 +37,8 @@ hipError_t post_cvd_q6(int L, int M, int N, float dx, int n, const float* x, con
                        unsigned* nb, float* ql, float* qglobal, unsigned long long* histq);
 
 // Plain device copy of n 16-byte words (counter calibration); pattern 0:
-// linear, 1: 64-byte pieces of a [n/256][256] array copied column-wise.
+// linear, 1: 64-byte pieces of a [n/256][256] array copied column-wise,
+// 2: the same with the column pass's XCD-aware tile order.
 hipError_t launch_copy16(const void* src, void* dst, size_t n, int pattern, hipStream_t s);
 
 }  // namespace pfc

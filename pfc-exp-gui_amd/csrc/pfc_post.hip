@@ -1113,10 +1113,14 @@ __global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, u
 
 // The same words viewed as [n/256][256]: each workgroup copies 4 adjacent
 // columns down all rows, so every row contributes one 64-byte piece per
-// workgroup -- the access granularity of the column pass.
+// workgroup -- the access granularity of the column pass.  XCD: column tiles
+// mapped as the column pass maps them (the workgroups of one XCD take
+// adjacent tiles).
+template <bool XCD>
 __global__ void __launch_bounds__(256) k_copy16_cols(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                      size_t rows) {
-    const size_t col = (size_t)blockIdx.x * 4 + threadIdx.x % 4;
+    const unsigned bx = XCD ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
+    const size_t col = (size_t)bx * 4 + threadIdx.x % 4;
     for (size_t r = (size_t)blockIdx.y * 64 + threadIdx.x / 4; r < rows; r += (size_t)gridDim.y * 64)
         dst[r * 256 + col] = src[r * 256 + col];
 }
@@ -1126,7 +1130,12 @@ hipError_t launch_copy16(const void* src, void* dst, size_t n, int pattern, hipS
         hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
     } else {
         if (n % 256) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_copy16_cols, dim3(64, 256), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n / 256);
+        if (pattern == 1)
+            hipLaunchKernelGGL(k_copy16_cols<false>, dim3(64, 256), dim3(256), 0, s, (const uint4*)src, (uint4*)dst,
+                               n / 256);
+        else
+            hipLaunchKernelGGL(k_copy16_cols<true>, dim3(64, 256), dim3(256), 0, s, (const uint4*)src, (uint4*)dst,
+                               n / 256);
     }
     return hipGetLastError();
 }

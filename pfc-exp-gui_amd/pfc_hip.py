@@ -152,7 +152,8 @@ def _fp(a):
 
 def copy_probe(nbytes, pattern=0):
     """Milliseconds of one plain device copy of nbytes (counter calibration);
-    pattern 0 linear, 1 column-wise 64-byte pieces."""
+    pattern 0 linear, 1 column-wise 64-byte pieces, 2 the same in the column
+    pass's XCD-aware tile order."""
     ms = ctypes.c_double()
     _check(load().pfc_copy_probe(nbytes, pattern, ctypes.byref(ms)))
     return ms.value
