@@ -262,6 +262,10 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // XCD-aware column tiles: workgroup x of an instance takes tile
 // (x % 8) * (nx / 8) + x / 8, so the workgroups one XCD receives (every
 // eighth) cover adjacent columns and share its L2's lines.
+// Batched FP32 plain steps read per-mode coefficient tables (k_coef).
+#ifndef PFC_COEF
+#define PFC_COEF 1
+#endif
 #ifndef PFC_XCD_REMAP
 #define PFC_XCD_REMAP 1
 #endif
@@ -759,7 +763,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
 
     // Plain FP32 steps take the modes' coefficients from cA/cC (same layout
     // as one instance of S, loaded with the state so their latency overlaps).
-    constexpr bool COEF = LEAN && sizeof(R) == 4 && LL % 3 != 0;  // E = 12 has no registers to spare
+    constexpr bool COEF = PFC_COEF && LEAN && sizeof(R) == 4 && LL % 3 != 0;
     float4 cf[COEF ? E : 1];
     float2 cc[COEF ? E : 1];
     if constexpr (COEF) {
