@@ -277,8 +277,18 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 constexpr int col_e(bool f64, int LL) {
     return LL % 3 == 0 ? 12 : (f64 ? (LL >= 1024 ? 8 : 4) : (LL == 512 ? 8 : 4));
 }
+// FP64 columns up to 512: one per workgroup, so that a single instance (the
+// drop-in path) launches enough workgroups to fill the GPU; 32-byte row
+// pieces, whose lines the XCD-aware tile order keeps in one L2.
+#ifndef PFC_COL_T_F64_MID
+#define PFC_COL_T_F64_MID 1
+#endif
+#ifndef PFC_COL_T_F64_SMALL
+#define PFC_COL_T_F64_SMALL 1
+#endif
 constexpr int col_t(bool f64, int LL) {
-    return f64 ? (LL <= 256 ? 4 : 2) : (LL <= 128 ? 8 : (LL <= 768 ? 4 : 2));
+    return f64 ? (LL <= 256 ? PFC_COL_T_F64_SMALL : (LL <= 512 ? PFC_COL_T_F64_MID : 2))
+               : (LL <= 128 ? 8 : (LL <= 768 ? 4 : 2));
 }
 
 // Row pass: RPW rows of length M = 2H per workgroup, G = H/E threads per row.
