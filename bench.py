@@ -32,6 +32,7 @@ import importlib.util
 import json
 import os
 import shutil
+import subprocess
 import sys
 import tempfile
 import time
@@ -96,6 +97,28 @@ def cpu_baseline(budget_s=15.0):
             break
     return {"value": 256 * 256 * steps / el / 1e6, "unit": "Mpoint-steps/s", "cores": 1, "kind": "port",
             "sample": f"oracle Misfit 256x256, {steps} steps (faithful: 14 c2c FFTs/step), {el:.1f} s, 1 thread"}
+
+
+def cpu_baseline_concurrent(nproc, steps=150):
+    """The reference runs one single-threaded job at a time; the best case
+    for a CPU host is one such job per core.  Runs `nproc` oracle processes
+    (the same 256^2 job) at once and sums their rates over the timed steps."""
+    code = ("import importlib.util, sys, time\n"
+            "spec = importlib.util.spec_from_file_location('o', sys.argv[1])\n"
+            "o = importlib.util.module_from_spec(spec); spec.loader.exec_module(o)\n"
+            "s = o.Sim(o.model(o.MISFIT, 256, 256, 1)); s.run(2, faithful=1)\n"
+            "t = time.perf_counter(); s.run(%d, faithful=1); print(time.perf_counter() - t)\n" % steps)
+    path = os.path.join(ROOT, "oracle", "oracle.py")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen([sys.executable, "-c", code, path], stdout=subprocess.PIPE, env=env)
+             for _ in range(nproc)]
+    secs = [float(p.communicate()[0].decode().strip().splitlines()[-1]) for p in procs]
+    wall = time.perf_counter() - t0
+    rates = [256 * 256 * steps / t / 1e6 for t in secs]
+    return {"procs": nproc, "value": sum(rates), "unit": "Mpoint-steps/s", "per_proc": sum(rates) / nproc,
+            "sample": f"{nproc} oracle processes at once, Misfit 256x256, {steps} steps each, "
+                      f"{min(secs):.1f}-{max(secs):.1f} s timed, {wall:.1f} s wall"}
 
 
 def _run_pair(hip, L, steps, every, perturb=0.0, seed=11):
@@ -453,6 +476,8 @@ def main():
         print(f"serving done: {out['serving']['fp32_batched']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        # one job per host core, 16 at most (the host share of one GPU)
+        out["cpu_baseline"]["concurrent"] = cpu_baseline_concurrent(min(16, os.cpu_count() or 1))
     print(json.dumps(out))
 
 
