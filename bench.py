@@ -438,6 +438,7 @@ def main():
             "workload": "Misfit sweep, 64 instances (4 u0 x 4 con0 x 4 seeds), 16 each of 128^2/256^2/384^2/512^2, "
                         "FP32 + FP64 k=0/reductions, hipGraph chunks of %d steps" % args.graph_chunk,
             "points_per_step": pts,
+            "col_tiles_autotuned": {str(g): s.col_tiles() for g, s in sims.items()},
         },
         "fp64_plain": {"value": round(fp64, 1), "unit": "Mpoint-steps/s",
                        "workload": "the sweep's 64 instances one at a time, FP64 (reference narrowing), eager",

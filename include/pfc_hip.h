@@ -137,6 +137,15 @@ int pfc_cvd_gr(int L, int M, int N, float dx, int n, const float* pc_x, const fl
 int pfc_cvd_q6(int L, int M, int N, float dx, int n, const float* pc_x, const float* pc_y, const float* pc_z,
                unsigned int* nb, float* ql, float* qglobal, unsigned long long* histq);
 
+/* Columns per workgroup of the plain-step column pass, as chosen at
+ * pfc_create by the library's autotuner (no counterpart in the reference:
+ * the tile width has no effect on results; PFC_AUTOTUNE=0 keeps the default). */
+int pfc_col_tiles(pfc_ctx* ctx, int* columns);
+/* Force tile-width candidate 0 (default), 1 (twice) or 2 (half as many
+ * columns) for the plain-step column pass (no counterpart in the reference;
+ * for comparisons; results do not depend on it). */
+int pfc_set_col_tiles(pfc_ctx* ctx, int candidate);
+
 /* Capture steps in HIP graphs of `chunk` steps (0 disables). */
 int pfc_set_graph(pfc_ctx* ctx, int chunk);
 

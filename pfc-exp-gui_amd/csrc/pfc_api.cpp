@@ -2,10 +2,14 @@
 // energy, graphs and kernel timing.  See include/pfc_hip.h for the contract.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -164,6 +168,75 @@ static void free_ctx(pfc_ctx* c) {
     delete c;
 }
 
+// Column-tile autotuner.  The plain-step column pass is compiled for three
+// tile widths (the default, twice and half as many columns per workgroup);
+// all give bit-identical results, and which is fastest depends on the
+// length, the precision and the batch (how many workgroups one launch has).
+// Each candidate is timed on a scratch copy of the context's state; the
+// choice is kept per (precision, grid, batch) for the process.
+// PFC_AUTOTUNE=0 in the environment keeps the default.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::array<int, 5>, int> g_tune;
+}  // namespace
+
+static int tune_cols(pfc_ctx* c) {
+    const char* env = std::getenv("PFC_AUTOTUNE");
+    if (env && env[0] == '0') return 0;
+    if (c->noise) return 0;  // steps with the noise term run the general kernel
+    if (c->prec == PREC_FP32 && c->L % 3 != 0 && !c->bufs.cA) return 0;  // no plain-step kernel in use
+    const std::array<int, 5> key{c->prec, c->L, c->M, c->N, c->B};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) return it->second;
+    }
+    int cand[3], nc = 0;
+    for (int ts = 0; ts < 3; ts++) {
+        const int t = col_tiles(c->prec, c->L, ts);
+        bool dup = t <= 0 || c->K % t != 0;
+        for (int i = 0; i < nc; i++) dup = dup || col_tiles(c->prec, c->L, cand[i]) == t;
+        if (!dup) cand[nc++] = ts;
+    }
+    int best = 0;
+    if (nc > 1) {
+        const size_t bytes = (size_t)c->B * c->L * c->N * c->K * c->vsize;
+        void *s2 = nullptr, *a2 = nullptr, *i2 = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipMalloc(&s2, bytes);
+        if (e == hipSuccess) e = hipMalloc(&a2, bytes);
+        if (e == hipSuccess) e = hipMalloc(&i2, bytes);
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess) e = hipMemcpyAsync(s2, c->S, bytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(i2, c->IN, bytes, hipMemcpyDeviceToDevice, c->stream);
+        DevBufs b = c->bufs;
+        b.S = s2; b.AB = a2; b.IN = i2;
+        b.tcount = nullptr;
+        float best_ms = 1e30f;
+        for (int k = 0; k < nc && e == hipSuccess; k++) {
+            b.col_ts = cand[k];
+            for (int w = 0; w < 2 && e == hipSuccess; w++) e = launch_col(c->prec, b, c->phys, COL_STEP, 0, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+            for (int r = 0; r < 5 && e == hipSuccess; r++) e = launch_col(c->prec, b, c->phys, COL_STEP, 0, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && ms < best_ms) { best_ms = ms; best = cand[k]; }
+        }
+        if (e != hipSuccess) best = 0;
+        hipStreamSynchronize(c->stream);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        for (void* q : {s2, a2, i2})
+            if (q) hipFree(q);
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best;
+    return best;
+}
+
 extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     if (!p || !out || batch <= 0) return fail(PFC_E_ARG, "bad arguments");
     *out = nullptr;
@@ -291,8 +364,25 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
             return fail(PFC_E_NOMEM, "coefficient tables: %s", hipGetErrorString(e));
         }
     }
+    b.col_ts = 0;
+    b.col_ts = tune_cols(c);
     *out = c;
     return PFC_OK;
+}
+
+extern "C" int pfc_col_tiles(pfc_ctx* ctx, int* columns) {
+    if (!ctx || !columns) return fail(PFC_E_ARG, "bad arguments");
+    *columns = col_tiles(ctx->prec, ctx->L, ctx->bufs.col_ts);
+    return PFC_OK;
+}
+
+extern "C" int pfc_set_col_tiles(pfc_ctx* ctx, int candidate) {
+    if (!ctx || candidate < 0 || candidate > 2) return fail(PFC_E_ARG, "bad arguments");
+    const int t = col_tiles(ctx->prec, ctx->L, candidate);
+    if (t <= 0 || ctx->K % t) return fail(PFC_E_UNSUPPORTED, "tile candidate %d does not divide the columns", candidate);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->bufs.col_ts = candidate;
+    return pfc_set_graph(ctx, ctx->graph_chunk);  // recapture with the new width
 }
 
 extern "C" int pfc_destroy(pfc_ctx* ctx) {

@@ -58,6 +58,7 @@ struct DevBufs {
     // 1/(1 + mn dt k^2), 1/(1 + mc dt alpha k^4)}, cC = {C2A, C2B}.
     const void* cA;
     const void* cC;
+    int col_ts;  // plain-step column tile width chosen by the autotuner (0 default, 1 wider, 2 narrower)
     int L, M, N, K, B;
 };
 
@@ -71,7 +72,8 @@ hipError_t launch_coef(const DevBufs& b, const PhysArgs& p, hipStream_t s);
 bool size_supported(int n);
 bool depth_supported(int n);                    // N > 1 lengths of the depth pass
 int row_blocks(int prec, int M, int rows);      // row-pass workgroups per instance (rows = L*N)
-int col_blocks(int prec, int L, int M, int N);  // column-pass workgroups per instance
+int col_blocks(int prec, int L, int M, int N);  // column-pass workgroups per instance (default tiles)
+int col_tiles(int prec, int L, int ts);         // columns per workgroup of tile-width candidate ts
 
 // t0/t1 (optional): events that take the kernel's own start and end times.
 hipError_t launch_row_step(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s,

@@ -319,8 +319,10 @@ struct RowCfg {
 
 // Column pass: T columns of length LL per workgroup, G = LL/E threads per
 // column; consecutive threads take consecutive columns (coalesced T*16 or
-// T*32 byte segments per row).
-template <typename R, int LL>
+// T*32 byte segments per row).  TS selects the tile width among the
+// autotuner's candidates: 0 the default, 1 twice, 2 half as many columns
+// (a candidate that would not fit the LDS or 1024 threads is the default).
+template <typename R, int LL, int TS = 0>
 struct ColCfg {
     static constexpr bool F64 = sizeof(R) == 8;
     static constexpr bool GEN = PFC_GEN && LL % 3 == 0;
@@ -332,8 +334,11 @@ struct ColCfg {
     static constexpr int T0 = col_t(F64, LL);
     static constexpr int T1 = T0 / PFC_COL_T_DIV > 0 ? T0 / PFC_COL_T_DIV : 1;
     // Wider tiles (PFC_COL_T_MUL) only while the exchange buffers fit in LDS.
-    static constexpr int T = (size_t)T1 * PFC_COL_T_MUL * padded(LL) * sizeof(Vec<R, 4>) <= 160 * 1024
-                                 ? T1 * PFC_COL_T_MUL : T1;
+    static constexpr int TD = (size_t)T1 * PFC_COL_T_MUL * padded(LL) * sizeof(Vec<R, 4>) <= 160 * 1024
+                                  ? T1 * PFC_COL_T_MUL : T1;
+    static constexpr int TW = TS == 1 ? 2 * TD : (TS == 2 ? (TD / 2 > 0 ? TD / 2 : 1) : TD);
+    static constexpr bool TW_OK = (size_t)TW * padded(LL) * sizeof(Vec<R, 4>) <= 150 * 1024 && TW * G <= 1024;
+    static constexpr int T = TW_OK ? TW : TD;
     static constexpr int NT = T * G;
     static constexpr int PB = padded(LL);
     static constexpr int CH = PFC_LDS_MOD ? (int)(sizeof(Vec<R, 4>) / 16) : 1;
@@ -728,11 +733,11 @@ __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs&
 
 // LEAN: the plain step (mode STEP, no noise, no energy) with the other paths
 // compiled out, which frees registers for occupancy.
-template <typename R, int LL, bool LEAN>
-__global__ void __launch_bounds__((ColCfg<R, LL>::NT))
-__attribute__((amdgpu_waves_per_eu(LEAN ? ColCfg<R, LL>::WPE_LEAN : ColCfg<R, LL>::WPE)))
+template <typename R, int LL, bool LEAN, int TS>
+__global__ void __launch_bounds__((ColCfg<R, LL, TS>::NT))
+__attribute__((amdgpu_waves_per_eu(LEAN ? ColCfg<R, LL, TS>::WPE_LEAN : ColCfg<R, LL, TS>::WPE)))
 k_col(DevBufs b, PhysArgs p, int mode, int flags) {
-    using C = ColCfg<R, LL>;
+    using C = ColCfg<R, LL, TS>;
     constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT, EA = C::EA;
     __shared__ __attribute__((aligned(16))) char smem[C::LDS];
     __shared__ double red[(NT + 63) / 64 + 1];
@@ -1072,18 +1077,29 @@ static int row_blocks_t(int M, int L) {
     return L / rpw;
 }
 
-template <typename R>
-static int col_blocks_t(int L, int M, int N) {
-    int T = 1;
+template <typename R, int TS>
+static int col_tiles_t(int L) {
     switch (L) {
-#define CB(LL) case LL: T = ColCfg<R, LL>::T; break;
+#define CB(LL) case LL: return ColCfg<R, LL, TS>::T;
         CB(32) CB(64) CB(128) CB(256) CB(384) CB(512) CB(768) CB(1024)
 #undef CB
         default: return -1;
     }
+}
+
+template <typename R>
+static int col_blocks_t(int L, int M, int N, int ts = 0) {
+    const int T = ts == 1 ? col_tiles_t<R, 1>(L) : (ts == 2 ? col_tiles_t<R, 2>(L) : col_tiles_t<R, 0>(L));
+    if (T <= 0) return -1;
     const int K = M / 2;
     if (K % T) return -1;  // k_col handles whole tiles of T columns only
     return N * (K / T);
+}
+
+int col_tiles(int prec, int L, int ts) {
+    if (prec == PREC_FP64)
+        return ts == 1 ? col_tiles_t<double, 1>(L) : (ts == 2 ? col_tiles_t<double, 2>(L) : col_tiles_t<double, 0>(L));
+    return ts == 1 ? col_tiles_t<float, 1>(L) : (ts == 2 ? col_tiles_t<float, 2>(L) : col_tiles_t<float, 0>(L));
 }
 
 int row_blocks(int prec, int M, int rows) {
@@ -1111,16 +1127,22 @@ static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hip
 template <typename R>
 static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s, hipEvent_t t0,
                         hipEvent_t t1) {
-    const int nb = col_blocks_t<R>(b.L, b.M, b.N);
-    if (nb <= 0) return hipErrorInvalidValue;
-    dim3 grid(nb, b.B);
     const bool lean = mode == COL_STEP && !b.NZ && !(flags & COL_ENERGY) &&
                       (sizeof(R) == 8 || b.L % 3 == 0 || b.cA != nullptr);
+    // The plain step runs the tile width the autotuner chose; the other
+    // modes (and a width whose tiles do not divide K) the default.
+    int ts = lean ? b.col_ts : 0;
+    int nb = col_blocks_t<R>(b.L, b.M, b.N, ts);
+    if (nb <= 0 && ts) nb = col_blocks_t<R>(b.L, b.M, b.N, ts = 0);
+    if (nb <= 0) return hipErrorInvalidValue;
+    dim3 grid(nb, b.B);
     switch (b.L) {
-#define CS(LL)                                                                                          \
-    case LL:                                                                                            \
-        if (lean) PFC_LAUNCH((k_col<R, LL, true>), grid, dim3(ColCfg<R, LL>::NT), s, t0, t1, b, p, mode, flags); \
-        else PFC_LAUNCH((k_col<R, LL, false>), grid, dim3(ColCfg<R, LL>::NT), s, t0, t1, b, p, mode, flags);    \
+#define CS(LL)                                                                                                  \
+    case LL:                                                                                                    \
+        if (!lean) PFC_LAUNCH((k_col<R, LL, false, 0>), grid, dim3(ColCfg<R, LL, 0>::NT), s, t0, t1, b, p, mode, flags); \
+        else if (ts == 1) PFC_LAUNCH((k_col<R, LL, true, 1>), grid, dim3(ColCfg<R, LL, 1>::NT), s, t0, t1, b, p, mode, flags); \
+        else if (ts == 2) PFC_LAUNCH((k_col<R, LL, true, 2>), grid, dim3(ColCfg<R, LL, 2>::NT), s, t0, t1, b, p, mode, flags); \
+        else PFC_LAUNCH((k_col<R, LL, true, 0>), grid, dim3(ColCfg<R, LL, 0>::NT), s, t0, t1, b, p, mode, flags); \
         break;
         CS(32) CS(64) CS(128) CS(256) CS(384) CS(512) CS(768) CS(1024)
 #undef CS

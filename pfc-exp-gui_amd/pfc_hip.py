@@ -24,7 +24,7 @@ SYMBOLS = [
     "pfc_set_active",
     "pfc_step", "pfc_sync", "pfc_get_fields", "pfc_get_fields_f32", "pfc_energy",
     "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic",
-    "pfc_strain", "pfc_cryst_order", "pfc_cvd_gr", "pfc_cvd_q6", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
+    "pfc_strain", "pfc_cryst_order", "pfc_cvd_gr", "pfc_cvd_q6", "pfc_col_tiles", "pfc_set_col_tiles", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
     "pfc_device_memory", "pfc_copy_probe", "pfc_last_error",
     "pfc_server_create", "pfc_server_submit", "pfc_server_poll", "pfc_server_result", "pfc_server_error",
     "pfc_server_cancel", "pfc_server_drain", "pfc_server_stats", "pfc_server_destroy",
@@ -111,6 +111,8 @@ def load():
         "pfc_cvd_q6": [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int, fp, fp, fp,
                                             ctypes.POINTER(ctypes.c_uint), fp, fp,
                                             ctypes.POINTER(ctypes.c_ulonglong)],
+        "pfc_col_tiles": [vp, ctypes.POINTER(ctypes.c_int)],
+        "pfc_set_col_tiles": [vp, ctypes.c_int],
         "pfc_set_graph": [vp, ctypes.c_int],
         "pfc_time_kernel": [vp, ctypes.c_int, ctypes.c_int],
         "pfc_kernel_time": [vp, dp, ip],
@@ -237,6 +239,16 @@ class Sim:
 
     def set_graph(self, chunk):
         _check(self.lib.pfc_set_graph(self.h, chunk))
+
+    def col_tiles(self):
+        """Columns per workgroup the autotuner chose for the plain step."""
+        t = ctypes.c_int()
+        _check(self.lib.pfc_col_tiles(self.h, ctypes.byref(t)))
+        return t.value
+
+    def set_col_tiles(self, candidate):
+        """Force tile-width candidate 0 (default), 1 (twice) or 2 (half)."""
+        _check(self.lib.pfc_set_col_tiles(self.h, candidate))
 
     def time_kernel(self, kid, enable=True):
         _check(self.lib.pfc_time_kernel(self.h, kid, 1 if enable else 0))

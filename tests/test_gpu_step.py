@@ -156,3 +156,29 @@ def test_misfit_noise_term(hip, oracle, prec, tol):
     quiet = np.abs(q.get_fields()[0] - ophi).max()
     print(f"prec={prec} noise err={err:.3e} (no-noise distance {quiet:.3e})", flush=True)
     assert err < tol and quiet > 100 * tol
+
+
+@pytest.mark.parametrize("prec,L,B", [(1, 128, 4), (1, 384, 4), (1, 512, 16), (0, 256, 1), (0, 512, 1)])
+def test_column_tile_widths_bit_identical(hip, prec, L, B):
+    """Every tile width the autotuner may pick gives the same bits (each
+    column is computed by the same code whatever the workgroup holds)."""
+    p = hip.default_params(model=0, L=L, M=L, precision=prec, grainx=4, grainy=4)
+    phi, con = hip.init_fields(p)
+    out = {}
+    for cand in (0, 1, 2):
+        s = hip.Sim(p, batch=B)
+        for i in range(B):
+            s.set_fields(phi, con, inst=i)
+        try:
+            s.set_col_tiles(cand)
+        except hip.PfcError:
+            s.close()
+            continue
+        s.step(6)
+        out[(cand, s.col_tiles())] = s.get_fields(B - 1)
+        s.close()
+    print(f"prec={prec} L={L} B={B} widths tried: {sorted(k[1] for k in out)}", flush=True)
+    assert len(out) >= 2
+    ref = next(iter(out.values()))
+    for k, v in out.items():
+        assert np.array_equal(v[0], ref[0]) and np.array_equal(v[1], ref[1]), k
