@@ -370,25 +370,47 @@ def main():
         return
 
     # ---- batched FP32 sweep ------------------------------------------------
+    # The four grid classes are branches of one hipGraph (chunk steps of each
+    # per launch, pfc_group_*); steps beyond whole chunks run per context.
     sims = build_sweep(hip, args.graph_chunk)
     pts = sum(s.batch * s.P for s in sims.values())
-    for s in sims.values():
-        s.step(args.warmup)
+    chunk = max(1, args.graph_chunk)
+    group = hip.Group(list(sims.values()), chunk)
+
+    def sweep(nsteps, grouped):
+        if grouped:
+            group.step(nsteps // chunk)
+            rest = nsteps % chunk
+            if rest:
+                for s in sims.values():
+                    s.step(rest)
+            return
+        done = 0
+        while done < nsteps:
+            n = min(chunk, nsteps - done)
+            for s in sims.values():
+                s.step(n)
+            done += n
+
+    sweep(args.warmup, True)
     for s in sims.values():
         s.sync()
-    big = sims[512]
-    big.time_kernel(hip.KID_COL, True)
     t0 = time.perf_counter()
-    done = 0
-    chunk = max(1, args.graph_chunk)
-    while done < args.steps:
-        n = min(chunk, args.steps - done)
-        for s in sims.values():
-            s.step(n)
-        done += n
+    sweep(args.steps, True)
     for s in sims.values():
         s.sync()
     el = time.perf_counter() - t0
+    value = pts * args.steps / el / 1e6
+    # Roofline timing: graph-captured launches carry no per-launch events, so
+    # the same sweep runs again with per-class graphs on four streams and the
+    # 512^2 class's column launches event-timed (that class eager).
+    big = sims[512]
+    big.time_kernel(hip.KID_COL, True)
+    t1 = time.perf_counter()
+    sweep(args.steps, False)
+    for s in sims.values():
+        s.sync()
+    el_streams = time.perf_counter() - t1
     col_ms, col_n = big.kernel_time()
     big.time_kernel(hip.KID_COL, False)
     # The same kernel with the GPU to itself (the 512^2 class alone), for
@@ -397,8 +419,8 @@ def main():
     big.step(100)
     iso_ms, iso_n = big.kernel_time()
     big.time_kernel(hip.KID_COL, False)
+    group.close()
     col_bytes = big.kernel_bytes(hip.KID_COL)
-    value = pts * args.steps / el / 1e6
     achieved = col_bytes / (col_ms * 1e-3) / 1e9 if col_ms > 0 else 0.0
 
     # ---- plain FP64 path (one instance at a time, eager) --------------------
@@ -436,10 +458,14 @@ def main():
         "data": "synthetic: reference initialisers (INITIAL_dis), seeded",
         "config": {
             "workload": "Misfit sweep, 64 instances (4 u0 x 4 con0 x 4 seeds), 16 each of 128^2/256^2/384^2/512^2, "
-                        "FP32 + FP64 k=0/reductions, hipGraph chunks of %d steps" % args.graph_chunk,
+                        "FP32 + FP64 k=0/reductions; the four grid classes as branches of one hipGraph, "
+                        "%d steps per launch" % args.graph_chunk,
             "points_per_step": pts,
             "col_tiles_autotuned": {str(g): s.col_tiles() for g, s in sims.items()},
         },
+        "sweep_per_class_streams": {"value": round(pts * args.steps / el_streams / 1e6, 1), "unit": "Mpoint-steps/s",
+                                    "note": "the same sweep with one graph per class on four streams and the 512^2 "
+                                            "class eager with event-timed column launches (the roofline run)"},
         "fp64_plain": {"value": round(fp64, 1), "unit": "Mpoint-steps/s",
                        "workload": "the sweep's 64 instances one at a time, FP64 (reference narrowing), eager",
                        "ms_per_sweep_step": round(el64 * 1e3, 4),
@@ -456,7 +482,8 @@ def main():
             "traffic": None,
             "launches_timed": col_n,
             "mean_launch_ms": round(col_ms, 5),
-            "timing": "hipExtLaunchKernel start/stop events on every launch of the kernel in the timed region",
+            "timing": "hipExtLaunchKernel start/stop events on every launch of the kernel in a second timed run "
+                      "of the same sweep (per-class streams; graph-captured launches carry no per-launch events)",
             "mean_launch_ms_alone": round(iso_ms, 5),
             "launches_alone": iso_n,
             "bytes_per_launch": col_bytes,

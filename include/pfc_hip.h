@@ -146,6 +146,17 @@ int pfc_col_tiles(pfc_ctx* ctx, int* columns);
  * for comparisons; results do not depend on it). */
 int pfc_set_col_tiles(pfc_ctx* ctx, int candidate);
 
+/* Contexts stepped as one graph (no counterpart in the reference, which runs
+ * one job per process; for a sweep's grid classes, SURVEY cfg 3): the graph
+ * holds `chunk` steps of every context as parallel branches, so one launch
+ * advances all of them.  pfc_group_step runs nchunks launches, ordered after
+ * work already queued on the member contexts and before work queued on them
+ * afterwards.  Kernel timing must be off for the members. */
+typedef struct pfc_group pfc_group;
+int pfc_group_create(pfc_ctx** ctxs, int n, int chunk, pfc_group** out);
+int pfc_group_step(pfc_group* group, int nchunks);
+int pfc_group_destroy(pfc_group* group);
+
 /* Capture steps in HIP graphs of `chunk` steps (0 disables). */
 int pfc_set_graph(pfc_ctx* ctx, int chunk);
 

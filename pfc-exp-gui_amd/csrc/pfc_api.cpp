@@ -513,6 +513,96 @@ extern "C" int pfc_set_graph(pfc_ctx* c, int chunk) {
     return PFC_OK;
 }
 
+// Several contexts (the grid classes of an uneven sweep) as one graph: each
+// context's `chunk` steps are a branch forked from and joined to the first
+// context's stream, so one launch advances all of them and the device
+// interleaves their kernels.
+struct pfc_group {
+    std::vector<pfc_ctx*> ctxs;
+    int chunk = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<hipEvent_t> ev;  // fork, one join per branch, order events
+};
+
+static void free_group(pfc_group* g) {
+    if (!g) return;
+    if (g->gexec) hipGraphExecDestroy(g->gexec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    for (auto e : g->ev)
+        if (e) hipEventDestroy(e);
+    delete g;
+}
+
+extern "C" int pfc_group_create(pfc_ctx** ctxs, int n, int chunk, pfc_group** out) {
+    if (!ctxs || n <= 0 || chunk <= 0 || !out) return fail(PFC_E_ARG, "bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i]) return fail(PFC_E_ARG, "null context %d", i);
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return fail(PFC_E_ARG, "context %d given twice", i);
+        if (ctxs[i]->time_kid >= 0) return fail(PFC_E_ARG, "kernel timing is on for context %d", i);
+    }
+    pfc_group* g = new pfc_group();
+    g->ctxs.assign(ctxs, ctxs + n);
+    g->chunk = chunk;
+    g->ev.assign(2 * (size_t)n + 1, nullptr);
+    hipError_t e = hipSuccess;
+    for (auto& ev : g->ev)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    for (int i = 0; i < n && e == hipSuccess; i++) e = hipStreamSynchronize(ctxs[i]->stream);
+    if (e != hipSuccess) {
+        free_group(g);
+        return fail(PFC_E_HIP, "group set-up: %s", hipGetErrorString(e));
+    }
+    hipStream_t s0 = ctxs[0]->stream;
+    e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t ec = hipEventRecord(g->ev[0], s0);
+        for (int i = 1; i < n && ec == hipSuccess; i++) ec = hipStreamWaitEvent(ctxs[i]->stream, g->ev[0], 0);
+        for (int i = 0; i < n && ec == hipSuccess; i++)
+            for (int k = 0; k < chunk && ec == hipSuccess; k++) ec = enqueue_step(ctxs[i], ROW_WRITE_IN, 0, 0);
+        for (int i = 1; i < n && ec == hipSuccess; i++) {
+            ec = hipEventRecord(g->ev[i], ctxs[i]->stream);
+            if (ec == hipSuccess) ec = hipStreamWaitEvent(s0, g->ev[i], 0);
+        }
+        hipError_t e2 = hipStreamEndCapture(s0, &g->graph);
+        e = ec != hipSuccess ? ec : e2;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        free_group(g);
+        return fail(PFC_E_HIP, "group graph capture: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return PFC_OK;
+}
+
+// Advances every context of the group by nchunks * chunk steps.  Work queued
+// on the member contexts' streams before the call runs first; work queued on
+// them afterwards runs after the group's steps.
+extern "C" int pfc_group_step(pfc_group* g, int nchunks) {
+    if (!g || nchunks < 0) return fail(PFC_E_ARG, "bad arguments");
+    if (nchunks == 0) return PFC_OK;
+    const int n = (int)g->ctxs.size();
+    hipStream_t s0 = g->ctxs[0]->stream;
+    for (int i = 1; i < n; i++) {
+        HIP_TRY(hipEventRecord(g->ev[n + i], g->ctxs[i]->stream));
+        HIP_TRY(hipStreamWaitEvent(s0, g->ev[n + i], 0));
+    }
+    for (int k = 0; k < nchunks; k++) HIP_TRY(hipGraphLaunch(g->gexec, s0));
+    HIP_TRY(hipEventRecord(g->ev[n], s0));
+    for (int i = 1; i < n; i++) HIP_TRY(hipStreamWaitEvent(g->ctxs[i]->stream, g->ev[n], 0));
+    return PFC_OK;
+}
+
+extern "C" int pfc_group_destroy(pfc_group* g) {
+    if (!g) return PFC_OK;
+    if (!g->ctxs.empty()) hipStreamSynchronize(g->ctxs[0]->stream);
+    free_group(g);
+    return PFC_OK;
+}
+
 extern "C" int pfc_step(pfc_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return fail(PFC_E_ARG, "bad arguments");
     int done = 0;

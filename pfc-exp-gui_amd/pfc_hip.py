@@ -24,7 +24,8 @@ SYMBOLS = [
     "pfc_set_active",
     "pfc_step", "pfc_sync", "pfc_get_fields", "pfc_get_fields_f32", "pfc_energy",
     "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic",
-    "pfc_strain", "pfc_cryst_order", "pfc_cvd_gr", "pfc_cvd_q6", "pfc_col_tiles", "pfc_set_col_tiles", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
+    "pfc_strain", "pfc_cryst_order", "pfc_cvd_gr", "pfc_cvd_q6", "pfc_col_tiles", "pfc_set_col_tiles", "pfc_group_create", "pfc_group_step",
+    "pfc_group_destroy", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
     "pfc_device_memory", "pfc_copy_probe", "pfc_last_error",
     "pfc_server_create", "pfc_server_submit", "pfc_server_poll", "pfc_server_result", "pfc_server_error",
     "pfc_server_cancel", "pfc_server_drain", "pfc_server_stats", "pfc_server_destroy",
@@ -113,6 +114,9 @@ def load():
                                             ctypes.POINTER(ctypes.c_ulonglong)],
         "pfc_col_tiles": [vp, ctypes.POINTER(ctypes.c_int)],
         "pfc_set_col_tiles": [vp, ctypes.c_int],
+        "pfc_group_create": [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)],
+        "pfc_group_step": [vp, ctypes.c_int],
+        "pfc_group_destroy": [vp],
         "pfc_set_graph": [vp, ctypes.c_int],
         "pfc_time_kernel": [vp, ctypes.c_int, ctypes.c_int],
         "pfc_kernel_time": [vp, dp, ip],
@@ -348,6 +352,30 @@ def cvd_q6(L, M, N, dx, x, y, z):
                           nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), _fp(ql), _fp(qg),
                           hq.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong))))
     return nb, ql, float(qg[0]), hq
+
+
+class Group:
+    """Several Sims stepped as one graph (chunk steps of each per launch)."""
+
+    def __init__(self, sims, chunk):
+        self.lib = load()
+        self.sims = list(sims)
+        self.chunk = chunk
+        arr = (ctypes.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
+        self.h = ctypes.c_void_p()
+        _check(self.lib.pfc_group_create(arr, len(self.sims), chunk, ctypes.byref(self.h)))
+
+    def step(self, nchunks=1):
+        _check(self.lib.pfc_group_step(self.h, nchunks))
+
+    def sync(self):
+        for s in self.sims:
+            s.sync()
+
+    def close(self):
+        if self.h:
+            self.lib.pfc_group_destroy(self.h)
+            self.h = ctypes.c_void_p()
 
 
 class Server:

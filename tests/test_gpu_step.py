@@ -182,3 +182,45 @@ def test_column_tile_widths_bit_identical(hip, prec, L, B):
     ref = next(iter(out.values()))
     for k, v in out.items():
         assert np.array_equal(v[0], ref[0]) and np.array_equal(v[1], ref[1]), k
+
+
+def test_group_graph_equals_separate(hip):
+    """Uneven contexts (2-D FP32 batch, 3-D FP64, noise FP32) stepped as one
+    group graph give the same bits as each stepped on its own, and work
+    queued on a member before and after the group launch stays in order."""
+    specs = [dict(model=0, L=128, M=128, precision=hip.FP32, grainx=4, grainy=4, seed=5),
+             dict(model=1, L=32, M=32, N=16, precision=hip.FP64),
+             dict(model=0, L=64, M=64, precision=hip.FP32, grainx=2, grainy=2, axx=0.001, seed=9)]
+    batches = (3, 1, 2)
+
+    def make():
+        out = []
+        for kw, B in zip(specs, batches):
+            p = hip.default_params(**kw)
+            s = hip.Sim(p, batch=B)
+            phi, con = hip.init_fields(p)
+            for i in range(B):
+                s.set_fields(phi, con, inst=i)
+            s.set_seed(p.seed)
+            out.append(s)
+        return out
+
+    ref = make()
+    for s in ref:
+        s.step(12)
+        s.sync()
+    want = [[s.get_fields(i) for i in range(s.batch)] for s in ref]
+    grp_sims = make()
+    g = hip.Group(grp_sims, chunk=4)
+    g.step(2)
+    for s in grp_sims:
+        s.step(4)  # queued on the member's own stream after the group launches
+    g.step(0)
+    g.sync()
+    for s, w in zip(grp_sims, want):
+        for i in range(s.batch):
+            x = s.get_fields(i)
+            assert np.array_equal(x[0], w[i][0]) and np.array_equal(x[1], w[i][1])
+    g.close()
+    for s in ref + grp_sims:
+        s.close()
