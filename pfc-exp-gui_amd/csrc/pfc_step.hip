@@ -212,80 +212,71 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // Launch shapes
 // ---------------------------------------------------------------------------
 // Launch shapes (elements per thread E, rows per workgroup RPW, columns per
-// workgroup T) per precision and length.  The values are the fastest of the
-// shapes timed on MI355X for the sweep grids (16 instances of 128^2..512^2,
-// FP32) and the single-instance FP64 path; the PFC_* macros override them
-// for timing sweeps.
+// workgroup T) per precision and length, and the code-shape switches below.
+// The defaults are the fastest of the variants timed on MI355X for the sweep
+// grids (16 instances of 128^2..512^2, FP32) and the single-instance FP64
+// path; each PFC_* macro overrides one of them for timing experiments.
+//
+// Row pass
 #ifndef PFC_ROW_THREADS
-#define PFC_ROW_THREADS 128
-#endif
-#ifndef PFC_COL_T_DIV
-#define PFC_COL_T_DIV 1
+#define PFC_ROW_THREADS 128  // threads per row-pass workgroup (RPW = this / G)
 #endif
 #ifndef PFC_ROW_E
-#define PFC_ROW_E 4
+#define PFC_ROW_E 4  // elements per thread of power-of-two rows
+#endif
+#ifndef PFC_ROW_WAVE
+#define PFC_ROW_WAVE 1  // row exchanges with wavefront fences when a row is one wave
+#endif
+// Column pass
+#ifndef PFC_COL_T_DIV
+#define PFC_COL_T_DIV 1  // divide / multiply the default columns per workgroup
 #endif
 #ifndef PFC_COL_T_MUL
 #define PFC_COL_T_MUL 1
 #endif
-// Exchange buffers of 4-field transforms as 16-byte chunk arrays; successive
-// buffers start PFC_LDS_MOD 16-byte bank groups apart (0: plain AoS buffers).
-// Chunked column exchanges: -1 where measured faster (FP32 3*2^k), 0/1.
+#ifndef PFC_COL_T_F64_SMALL
+#define PFC_COL_T_F64_SMALL 1  // FP64 columns per workgroup, L <= 256
+#endif
+#ifndef PFC_COL_T_F64_MID
+#define PFC_COL_T_F64_MID 1  // FP64 columns per workgroup, L = 384, 512 (fills the GPU from one instance)
+#endif
 #ifndef PFC_COL_SPLIT
-#define PFC_COL_SPLIT -1
+#define PFC_COL_SPLIT -1  // exchanges in 16-byte chunks: -1 where faster (FP32 3*2^k, fixed plan), 0/1
 #endif
-// Column exchanges inside one wavefront: -1 where measured faster (FP32
-// 3*2^k lengths and L <= 128), 0 never, 1 wherever a column fits a wave.
 #ifndef PFC_COL_WAVE
-#define PFC_COL_WAVE -1
-#endif
-#ifndef PFC_ROW_WAVE
-#define PFC_ROW_WAVE 1
-#endif
-#ifndef PFC_COL_WPE_LEAN
-#define PFC_COL_WPE_LEAN 1
-#endif
-#ifndef PFC_WPE12
-#define PFC_WPE12 1
-#endif
-#ifndef PFC_WPE
-#define PFC_WPE 1
-#endif
-#ifndef PFC_LDS_MOD
-#define PFC_LDS_MOD 2
-#endif
-// 3*2^k lengths: the general plan (radix 6, then 8s; at most 8 elements per
-// thread) rather than 12 elements per thread.
-#ifndef PFC_GEN
-#define PFC_GEN 1
-#endif
-// XCD-aware column tiles: workgroup x of an instance takes tile
-// (x % 8) * (nx / 8) + x / 8, so the workgroups one XCD receives (every
-// eighth) cover adjacent columns and share its L2's lines.
-// Batched FP32 plain steps read per-mode coefficient tables (k_coef).
-#ifndef PFC_COEF
-#define PFC_COEF 1
+#define PFC_COL_WAVE -1  // one column per wavefront: -1 where faster (FP32 L <= 128), 0/1
 #endif
 #ifndef PFC_XCD_REMAP
-#define PFC_XCD_REMAP 1
+#define PFC_XCD_REMAP 1  // workgroup x takes tile (x % 8)*(nx/8) + x/8: one XCD's tiles are adjacent
 #endif
-// With the general plan, one 384-column per wavefront (16-byte row pieces)
-// instead of four columns interleaved across the workgroup.
+#ifndef PFC_COEF
+#define PFC_COEF 1  // batched FP32 plain steps read per-mode coefficient tables (k_coef)
+#endif
+// Both passes
+#ifndef PFC_GEN
+#define PFC_GEN 1  // 3*2^k lengths on the general radix-6/8 plan (<= 8 elements per thread)
+#endif
 #ifndef PFC_GEN_WAVE
-#define PFC_GEN_WAVE 0
+#define PFC_GEN_WAVE 0  // with the general plan, one column per wavefront
+#endif
+#ifndef PFC_LDS_MOD
+#define PFC_LDS_MOD 2  // 4-field exchange buffers as 16-byte chunk arrays, staggered by this many bank groups
+#endif
+#ifndef PFC_WPE
+#define PFC_WPE 1  // waves per SIMD the register allocator targets (amdgpu_waves_per_eu)
+#endif
+#ifndef PFC_WPE12
+#define PFC_WPE12 1  // the same for the fixed 12-element plan
+#endif
+#ifndef PFC_COL_WPE_LEAN
+#define PFC_COL_WPE_LEAN 1  // the same for the plain-step column kernel
 #endif
 constexpr int col_e(bool f64, int LL) {
     return LL % 3 == 0 ? 12 : (f64 ? (LL >= 1024 ? 8 : 4) : (LL == 512 ? 8 : 4));
 }
-// FP64 columns up to 512: one per workgroup, so that a single instance (the
-// drop-in path) launches enough workgroups to fill the GPU; 32-byte row
-// pieces, whose lines the XCD-aware tile order keeps in one L2.
-#ifndef PFC_COL_T_F64_MID
-#define PFC_COL_T_F64_MID 1
-#endif
-#ifndef PFC_COL_T_F64_SMALL
-#define PFC_COL_T_F64_SMALL 1
-#endif
+// FP64 columns up to 512 default to one per workgroup, so that a single
+// instance (the drop-in path) launches enough workgroups to fill the GPU;
+// the 32-byte row pieces' lines stay in one L2 through the XCD tile order.
 constexpr int col_t(bool f64, int LL) {
     return f64 ? (LL <= 256 ? PFC_COL_T_F64_SMALL : (LL <= 512 ? PFC_COL_T_F64_MID : 2))
                : (LL <= 128 ? 8 : (LL <= 768 ? 4 : 2));
