@@ -17,6 +17,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -118,11 +119,23 @@ struct pfc_server {
     double pt_steps = 0; // grid-point steps of finished jobs
     long long done = 0;
     std::string fatal;
+    // Busy contexts step the whole graph chunks they share as one group graph
+    // (scheduler thread only; rebuilt when the set of stepping contexts
+    // changes, dropped before any member is destroyed).
+    bool use_group = true;
+    pfc_group* grp = nullptr;
+    std::vector<pfc_ctx*> grp_key;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
 
 namespace {
+
+void drop_group(pfc_server* s) {
+    if (s->grp) pfc_group_destroy(s->grp);
+    s->grp = nullptr;
+    s->grp_key.clear();
+}
 
 void finish(pfc_server* s, Job* j, int state, double t) {
     // caller holds s->mu
@@ -291,6 +304,7 @@ void admit(pfc_server* s, std::unique_lock<std::mutex>& lk) {
             // Free contexts with no running job (oldest first) until it fits.
             for (size_t bi = 0; bi < s->batches.size() && (double)(s->used + one) > s->budget;) {
                 if (s->batches[bi]->running() == 0) {
+                    drop_group(s);
                     pfc_destroy(s->batches[bi]->ctx);
                     s->used -= s->batches[bi]->bytes;
                     s->batches.erase(s->batches.begin() + bi);
@@ -372,8 +386,26 @@ void sched_loop(pfc_server* s) {
         }
         lk.unlock();
         std::string err;
+        // Whole graph chunks every stepping context takes this round run as
+        // one group graph; the rest of each context's steps on its own stream.
+        const int chunk = s->o.graph_chunk;
+        int common = -1, nstep = 0;
+        std::vector<pfc_ctx*> key;
         for (auto& pl : plan)
-            if (pl.n > 0 && pfc_step(pl.b->ctx, pl.n) && err.empty()) err = pfc_last_error();
+            if (pl.n > 0) {
+                common = common < 0 ? pl.n : std::min(common, pl.n);
+                nstep++;
+                key.push_back(pl.b->ctx);
+            }
+        int k = (s->use_group && chunk > 0 && nstep >= 2) ? common / chunk : 0;
+        if (k > 0 && (!s->grp || key != s->grp_key)) {
+            drop_group(s);
+            if (pfc_group_create(key.data(), (int)key.size(), chunk, &s->grp) == 0) s->grp_key = key;
+            else { s->grp = nullptr; k = 0; }
+        }
+        if (k > 0 && pfc_group_step(s->grp, k) && err.empty()) err = pfc_last_error();
+        for (auto& pl : plan)
+            if (pl.n - k * chunk > 0 && pfc_step(pl.b->ctx, pl.n - k * chunk) && err.empty()) err = pfc_last_error();
         for (auto& pl : plan)
             if (pl.n > 0 && pfc_sync(pl.b->ctx) && err.empty()) err = pfc_last_error();
         lk.lock();
@@ -470,6 +502,8 @@ extern "C" int pfc_server_create(const pfc_server_opts* o, pfc_server** out) {
     if (d.max_slots <= 0 || d.graph_chunk < 0 || d.quantum <= 0) return pfc::api_fail(PFC_E_ARG, "bad server options");
     pfc_server* s = new pfc_server();
     s->o = d;
+    const char* ge = std::getenv("PFC_SERVE_GROUP");  // "0": each context on its own stream only
+    s->use_group = !(ge && ge[0] == '0');
     if (d.hbm_budget > 0) {
         s->budget = d.hbm_budget;
     } else {
@@ -603,6 +637,7 @@ extern "C" int pfc_server_destroy(pfc_server* s) {
         s->cv_writer.notify_all();
     }
     s->writer.join();
+    drop_group(s);
     for (auto& b : s->batches) pfc_destroy(b->ctx);
     delete s;
     return PFC_OK;
