@@ -15,6 +15,9 @@ Workload (the headline metric "Mpoint-steps/s batched FP32 vs plain FP64
     continuous-batching server in FP32, final files written per job; the first
     jobs of the same stream also run one at a time on the plain FP64 path.
     -> serving (p50/p99 job latency, aggregate Mpoint-steps/s)
+  * CVD 1024^2 in FP32, 2x10^4 steps with the energy traced every step and a
+    checkpoint every 500 steps, then the elastic post-process of every
+    checkpoint, all on the GPU through the product CLI.  -> cvd_checkpoints
 Initial fields are the reference initialisers (synthetic, seeded).
 
 The dominant kernel (column pass of the 512^2 class) is timed with HIP
@@ -328,6 +331,75 @@ def serve_leg(hip, seconds, seed=20240611):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def _checkpoint_times(path):
+    """(step, ms since epoch) pairs of a checkpoint_timestamps.txt."""
+    out = []
+    for line in open(path).read().splitlines()[1:]:
+        t = line.split()
+        out.append((int(t[1]), int(t[3])))
+    return out
+
+
+def cvd_checkpoint_leg(hip, steps, mod, L=1024):
+    """CVD L x L in FP32 (FP64 k=0 and reductions), u0 0.05, con0 0.25:
+    `steps` steps with the energy pair traced every step and the checkpoint
+    outputs (VTI, phiball/Phimax, GR, Q6) every `mod`, through the product
+    CLI; then the elastic post-processor (Phimax, strain, q4q6) on the GPU over
+    every checkpoint's VTI.  The same stepping without checkpoints, through
+    the library, separates the checkpoint cost."""
+    cli = os.path.join(ROOT, "pfc-exp-gui_amd", "pfc-exp-cli")
+    p = hip.default_params(model=hip.CVD, L=L, M=L, N=1, precision=hip.FP32, u0=0.05, con0=0.25)
+    phi, con = hip.init_fields(p)
+    s = hip.Sim(p)
+    s.set_fields(phi, con)
+    s.step_energy(mod)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step_energy(mod)
+    s.sync()
+    per_step = (time.perf_counter() - t0) / mod
+    s.close()
+    tmp = tempfile.mkdtemp(prefix="pfc_cvd_")
+    try:
+        run_dir, el_dir = os.path.join(tmp, "run"), os.path.join(tmp, "elastic")
+        t0 = time.perf_counter()
+        subprocess.run([cli, "--model", "cvd", "--precision", "fp32", "--L", str(L), "--M", str(L), "--u0", "0.05",
+                        "--con0", "0.25", "--steps", str(steps), "--mod", str(mod), "--outdir", run_dir],
+                       check=True, capture_output=True, timeout=900)
+        wall = time.perf_counter() - t0
+        ck = _checkpoint_times(os.path.join(run_dir, "checkpoint_timestamps.txt"))
+        span_steps = ck[-1][0] - ck[0][0]
+        span_s = (ck[-1][1] - ck[0][1]) / 1e3
+        n_ck = len(ck)
+        t0 = time.perf_counter()
+        subprocess.run([cli, "--model", "elastic", "--L", str(L), "--M", str(L), "--input-dir", run_dir,
+                        "--outdir", el_dir], check=True, capture_output=True, timeout=900)
+        el_wall = time.perf_counter() - t0
+        n_el = len([f for f in os.listdir(el_dir) if f.startswith("strain_")])
+        atoms = int(open(os.path.join(run_dir, f"Phimax_{ck[-1][0]}.txt")).readline())
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    pts = L * L
+    return {
+        "workload": f"CVD {L}^2 FP32 + FP64 k=0/reductions, u0 0.05, con0 0.25, {steps} steps, energy every step, "
+                    f"checkpoint every {mod} (VTI, phiball/Phimax, GR, Q6), then the elastic post-process (Phimax, "
+                    "strain, q4q6) of every checkpoint on the GPU; product CLI",
+        "checkpoints": n_ck,
+        "atoms_last_checkpoint": atoms,
+        "job_wall_s": round(wall, 3),
+        "job_mpoint_steps_per_s": round(pts * steps / wall / 1e6, 1),
+        "steady_mpoint_steps_per_s": round(pts * span_steps / span_s / 1e6, 1),
+        "stepping_only_mpoint_steps_per_s": round(pts / per_step / 1e6, 1),
+        "ms_per_step_stepping_only": round(per_step * 1e3, 5),
+        "ms_per_checkpoint_outputs": round((span_s - span_steps * per_step) / max(n_ck - 1, 1) * 1e3, 2),
+        "elastic_wall_s": round(el_wall, 3),
+        "elastic_ms_per_checkpoint": round(el_wall / max(n_el, 1) * 1e3, 2),
+        "elastic_checkpoints": n_el,
+        "note": "job wall includes process start-up, context creation and tile autotuning; steady = first to last "
+                "checkpoint timestamp; checkpoint cost = steady interval minus the same steps without outputs",
+    }
+
+
 COPY_PROBE_BYTES = 256 * 1024 * 1024
 
 
@@ -357,6 +429,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=10000)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--cvd-steps", type=int, default=20000, help="CVD 1024^2 checkpoint run (0: skip)")
+    ap.add_argument("--cvd-mod", type=int, default=500)
     ap.add_argument("--pmc-drive", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -502,6 +576,9 @@ def main():
     if args.serve_seconds > 0:
         out["serving"] = serve_leg(hip, args.serve_seconds)
         print(f"serving done: {out['serving']['fp32_batched']}", file=sys.stderr, flush=True)
+    if args.cvd_steps > 0:
+        out["cvd_checkpoints"] = cvd_checkpoint_leg(hip, args.cvd_steps, args.cvd_mod)
+        print(f"cvd checkpoints done: {out['cvd_checkpoints']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         # one job per host core, 16 at most (the host share of one GPU)

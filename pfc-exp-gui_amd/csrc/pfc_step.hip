@@ -1021,14 +1021,32 @@ __global__ void __launch_bounds__((DepCfg<R, NN>::NT)) k_depth(DevBufs b, const 
 }
 
 // Per-instance energy: e_out = {(sum e + alpha/P sum_k w|c|^2)/P, mean psi}, P = L*M*N.
-__global__ void k_energy_reduce(DevBufs b, int nrow, int ncol, double alpha, double* e_out) {
-    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
-    if (inst >= b.B) return;
+// One workgroup per instance: strided FP64 partial sums, then the wavefront
+// and the four waves combined in a fixed order (the result does not depend
+// on the batch size or on timing).
+constexpr int ENERGY_NT = 256;
+__global__ __launch_bounds__(ENERGY_NT) void k_energy_reduce(DevBufs b, int nrow, int ncol, double alpha,
+                                                             double* e_out) {
+    const int inst = blockIdx.x;
+    const int t = threadIdx.x;
+    const double* er = b.erow + (size_t)inst * nrow;
+    const double* ec = b.ecol + (size_t)inst * ncol;
     double se = 0.0, sg = 0.0;
-    for (int i = 0; i < nrow; i++) {
-        se += b.erow[(size_t)inst * nrow + i];
+    for (int i = t; i < nrow; i += ENERGY_NT) se += er[i];
+    for (int i = t; i < ncol; i += ENERGY_NT) sg += ec[i];
+    for (int o = 32; o > 0; o >>= 1) {
+        se += __shfl_xor(se, o);
+        sg += __shfl_xor(sg, o);
     }
-    for (int i = 0; i < ncol; i++) sg += b.ecol[(size_t)inst * ncol + i];
+    __shared__ double part[2][ENERGY_NT / 64];
+    if ((t & 63) == 0) {
+        part[0][t >> 6] = se;
+        part[1][t >> 6] = sg;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    se = (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]);
+    sg = (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]);
     const double P = (double)b.L * (double)b.M * (double)b.N;
     const double tot = se + alpha * sg / P;
     e_out[2 * inst] = tot / b.L / b.M / b.N;
@@ -1238,7 +1256,7 @@ hipError_t launch_row_out(int prec, const DevBufs& b, void* outphi, void* outcon
     return prec == PREC_FP64 ? row_out_t<double>(b, outphi, outcon, s) : row_out_t<float>(b, outphi, outcon, s);
 }
 hipError_t launch_energy_reduce(const DevBufs& b, int nrow, int ncol, double alpha, double* e_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_energy_reduce, dim3((b.B + 63) / 64), dim3(64), 0, s, b, nrow, ncol, alpha, e_out);
+    hipLaunchKernelGGL(k_energy_reduce, dim3(b.B), dim3(ENERGY_NT), 0, s, b, nrow, ncol, alpha, e_out);
     return hipGetLastError();
 }
 

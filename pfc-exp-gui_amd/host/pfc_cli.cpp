@@ -242,34 +242,52 @@ void write_run_config(Kind k, const Opts& o, float b0) {
     std::fclose(f);
 }
 
-void write_fields(pfc_ctx* ctx, int kd, const Opts& o, std::vector<float>& a, std::vector<float>& b) {
-    const int L = o.p.L, M = o.p.M, N = o.p.N;
-    pfc_get_fields_f32(ctx, 0, a.data(), b.data());
-    pfcio::write_vti("phi_" + std::to_string(kd) + "_0.vti", "phi", L, M, N, a.data());
+// VTI + PVTI pair of both fields at checkpoint kd (misfit.cpp:1588-1698).
+void write_field_files(int kd, int L, int M, int N, const float* a, const float* b) {
+    pfcio::write_vti("phi_" + std::to_string(kd) + "_0.vti", "phi", L, M, N, a);
     pfcio::write_pvti("phi_" + std::to_string(kd) + ".pvti", "phi", "phi_", kd, L, M, N);
-    pfcio::write_vti("con_" + std::to_string(kd) + "_0.vti", "con", L, M, N, b.data());
+    pfcio::write_vti("con_" + std::to_string(kd) + "_0.vti", "con", L, M, N, b);
     pfcio::write_pvti("con_" + std::to_string(kd) + ".pvti", "con", "con_", kd, L, M, N);
 }
 
-// CVD order outputs of a checkpoint: GR every time, Q6_GLO when kd % 20 == 0
-// (cvd.cpp:1253-1254), over the atom fractions of pfc_phimax_cvd.
-int cvd_order_files(int kd, const pfc_params& p, int na, const float* px, const float* py, const float* pz,
-                    const std::string& dir) {
-    int nbin = 0;
-    if (pfc_cvd_gr(p.L, p.M, p.N, p.dx, na, px, py, pz, nullptr, &nbin)) return 1;
-    std::vector<unsigned> hist(static_cast<size_t>(nbin));
-    if (pfc_cvd_gr(p.L, p.M, p.N, p.dx, na, px, py, pz, hist.data(), &nbin)) return 1;
-    pfcio::write_cvd_gr(kd, p.L, p.M, p.N, p.dx, na, px, hist.data(), nbin, dir);
-    if (kd >= 0 && kd % 20 == 0 && na > 0) {
-        std::vector<unsigned> nb(static_cast<size_t>(na));
-        std::vector<float> ql(static_cast<size_t>(na));
-        float qg = 0.f;
-        unsigned long long hq[30];
-        if (pfc_cvd_q6(p.L, p.M, p.N, p.dx, na, px, py, pz, nb.data(), ql.data(), &qg, hq)) return 1;
-        pfcio::write_cvd_q6(kd, na, nb.data(), ql.data(), qg, hq, dir);
-    }
-    return 0;
+// Both fields of the context's instance 0 as float (what the VTI holds).
+int fetch_fields(pfc_ctx* ctx, size_t P, std::vector<float>& a, std::vector<float>& b) {
+    a.resize(P);
+    b.resize(P);
+    return pfc_get_fields_f32(ctx, 0, a.data(), b.data());
 }
+
+// CVD order results of a checkpoint: GR every time, Q6_GLO when kd % 20 == 0
+// (cvd.cpp:1253-1254), over the atom fractions of pfc_phimax_cvd.  Computed
+// on the GPU here; write() puts them into files.
+struct CvdOrder {
+    int kd = 0;
+    std::vector<unsigned> hist;
+    bool has_q6 = false;
+    std::vector<unsigned> nb;
+    std::vector<float> ql;
+    float qg = 0.f;
+    unsigned long long hq[30] = {};
+
+    int compute(int k, const pfc_params& p, int na, const float* px, const float* py, const float* pz) {
+        kd = k;
+        int nbin = 0;
+        if (pfc_cvd_gr(p.L, p.M, p.N, p.dx, na, px, py, pz, nullptr, &nbin)) return 1;
+        hist.assign(static_cast<size_t>(nbin), 0u);
+        if (pfc_cvd_gr(p.L, p.M, p.N, p.dx, na, px, py, pz, hist.data(), &nbin)) return 1;
+        has_q6 = kd >= 0 && kd % 20 == 0 && na > 0;
+        if (has_q6) {
+            nb.resize(static_cast<size_t>(na));
+            ql.resize(static_cast<size_t>(na));
+            if (pfc_cvd_q6(p.L, p.M, p.N, p.dx, na, px, py, pz, nb.data(), ql.data(), &qg, hq)) return 1;
+        }
+        return 0;
+    }
+    void write(const pfc_params& p, int na, const float* px, const std::string& dir) const {
+        pfcio::write_cvd_gr(kd, p.L, p.M, p.N, p.dx, na, px, hist.data(), static_cast<int>(hist.size()), dir);
+        if (has_q6) pfcio::write_cvd_q6(kd, na, nb.data(), ql.data(), qg, hq, dir);
+    }
+};
 
 float b0_of(const pfc_params& p) {
     if (p.dx > 0.0f && p.dt > 0.0f && p.axx >= 0.0f)
@@ -301,15 +319,15 @@ int run_sim(Kind k, int argc, char** argv) {
     pfc_ctx* ctx = nullptr;
     if (pfc_create(&o.p, 1, &ctx)) return gpu_fail("pfc_create");
     if (pfc_set_fields(ctx, 0, phi.data(), con.data())) return gpu_fail("pfc_set_fields");
-    std::vector<float> fa(P), fb(P);
+    const int L = o.p.L, M = o.p.M, N = o.p.N;
     if (k == MISFIT) {
+        std::vector<float> fa(P), fb(P);
         for (size_t i = 0; i < P; i++) { fa[i] = static_cast<float>(phi[i]); fb[i] = static_cast<float>(con[i]); }
-        const int L = o.p.L, M = o.p.M, N = o.p.N;
-        pfcio::write_vti("phi_0_0.vti", "phi", L, M, N, fa.data());
-        pfcio::write_pvti("phi_0.pvti", "phi", "phi_", 0, L, M, N);
-        pfcio::write_vti("con_0_0.vti", "con", L, M, N, fb.data());
-        pfcio::write_pvti("con_0.pvti", "con", "con_", 0, L, M, N);
+        write_field_files(0, L, M, N, fa.data(), fb.data());
     }
+    // Checkpoint files are formatted and written by `out` while the GPU
+    // steps on; each task owns copies of what it writes.
+    pfcio::Writer out;
     std::vector<double> ax(P), ay(P), az(P), aphi(P);
     std::vector<float> acon(P), px(P), py(P), pz(P), pp(P);
     if (k == MISFIT) {
@@ -336,20 +354,25 @@ int run_sim(Kind k, int argc, char** argv) {
                 // The 2-D extraction is repeated for each n (misfit.cpp:3345).
                 std::vector<double> X, Y, Z, V;
                 std::vector<float> C;
-                for (int nn = 0; nn < o.p.N; nn++)
+                for (int nn = 0; nn < N; nn++)
                     for (int i = 0; i < n; i++) {
                         X.push_back(ax[i]); Y.push_back(ay[i]);
                         Z.push_back(nn == 0 ? az[i] : static_cast<double>(static_cast<float>(nn + 1) * o.p.dx));
                         V.push_back(aphi[i]); C.push_back(acon[i]);
                     }
-                pfcio::write_phimax_misfit("Phimax_" + std::to_string(kd) + ".txt", static_cast<int>(X.size()),
-                                           X.data(), Y.data(), Z.data(), V.data(), C.data());
-                write_fields(ctx, kd, o, fa, fb);
+                std::vector<float> fa, fb;
+                if (fetch_fields(ctx, P, fa, fb)) return gpu_fail("pfc_get_fields_f32");
                 double e[2];
                 if (pfc_energy(ctx, e)) return gpu_fail("pfc_energy");
-                FILE* ef = std::fopen("energy.txt", "a");
-                if (ef) { std::fprintf(ef, "%d %s %s\n", kd, pfcio::g6(e[0]).c_str(), pfcio::g6(e[1]).c_str()); std::fclose(ef); }
-                if (ck) { std::fprintf(ck, "step %d timestamp_ms %lld\n", os, pfcio::now_ms()); std::fflush(ck); }
+                out.post([=, X = std::move(X), Y = std::move(Y), Z = std::move(Z), V = std::move(V),
+                          C = std::move(C), fa = std::move(fa), fb = std::move(fb)] {
+                    pfcio::write_phimax_misfit("Phimax_" + std::to_string(kd) + ".txt", static_cast<int>(X.size()),
+                                               X.data(), Y.data(), Z.data(), V.data(), C.data());
+                    write_field_files(kd, L, M, N, fa.data(), fb.data());
+                    FILE* ef = std::fopen("energy.txt", "a");
+                    if (ef) { std::fprintf(ef, "%d %s %s\n", kd, pfcio::g6(e[0]).c_str(), pfcio::g6(e[1]).c_str()); std::fclose(ef); }
+                    if (ck) { std::fprintf(ck, "step %d timestamp_ms %lld\n", kd, pfcio::now_ms()); std::fflush(ck); }
+                });
             }
         }
     } else {
@@ -370,21 +393,33 @@ int run_sim(Kind k, int argc, char** argv) {
                 if (pfc_phimax_cvd(ctx, 0, px.data(), py.data(), pz.data(), pp.data(), static_cast<int>(P), &na, &phic,
                                    &pmax))
                     return gpu_fail("pfc_phimax_cvd");
-                pfcio::write_cvd_atoms(last, o.p.L, o.p.M, o.p.N, o.p.dx, phic, pmax, na, px.data(), py.data(), pz.data(),
-                                       pp.data());
-                if (cvd_order_files(last, o.p, na, px.data(), py.data(), pz.data(), "")) return gpu_fail("pfc_cvd_gr/q6");
-                write_fields(ctx, last, o, fa, fb);
-                if (ck) { std::fprintf(ck, "step %d timestamp_ms %lld\n", last, pfcio::now_ms()); std::fflush(ck); }
+                CvdOrder ord;
+                if (ord.compute(last, o.p, na, px.data(), py.data(), pz.data())) return gpu_fail("pfc_cvd_gr/q6");
+                std::vector<float> fa, fb;
+                if (fetch_fields(ctx, P, fa, fb)) return gpu_fail("pfc_get_fields_f32");
+                std::vector<float> qx(px.begin(), px.begin() + na), qy(py.begin(), py.begin() + na),
+                    qz(pz.begin(), pz.begin() + na), qp(pp.begin(), pp.begin() + na);
+                const pfc_params prm = o.p;
+                out.post([=, ord = std::move(ord), qx = std::move(qx), qy = std::move(qy), qz = std::move(qz),
+                          qp = std::move(qp), fa = std::move(fa), fb = std::move(fb)] {
+                    pfcio::write_cvd_atoms(last, L, M, N, prm.dx, phic, pmax, na, qx.data(), qy.data(), qz.data(),
+                                           qp.data());
+                    ord.write(prm, na, qx.data(), "");
+                    write_field_files(last, L, M, N, fa.data(), fb.data());
+                    if (ck) { std::fprintf(ck, "step %d timestamp_ms %lld\n", last, pfcio::now_ms()); std::fflush(ck); }
+                });
             }
-            FILE* ef = std::fopen("energy.txt", "a");
-            if (ef) {
+            out.post([t, n, tr = std::move(tr)] {
+                FILE* ef = std::fopen("energy.txt", "a");
+                if (!ef) return;
                 for (int s = 0; s < n; s++)
                     std::fprintf(ef, "%d %s %s\n", t + s, pfcio::g6(tr[2 * s]).c_str(), pfcio::g6(tr[2 * s + 1]).c_str());
                 std::fclose(ef);
-            }
+            });
             t = last + 1;
         }
     }
+    out.drain();
     if (ck) std::fclose(ck);
     pfc_destroy(ctx);
     return 0;
@@ -421,6 +456,7 @@ int run_elastic(int argc, char** argv) {
     std::vector<float> phi(P);
     std::vector<double> x(P), y(P), z(P), pm(P), st(P), q4(P), q6(P);
     std::vector<unsigned> nb(P);
+    pfcio::Writer out;
     for (size_t idx = 0; idx < steps.size(); idx++) {
         const int kd = steps[idx];
         std::fprintf(stderr, "PFC_PROGRESS step=%zu total=%zu\n", idx + 1, steps.size());
@@ -438,15 +474,24 @@ int run_elastic(int argc, char** argv) {
         if (pfc_phimax_elastic(L, M, N, o.p.dx, phi.data(), x.data(), y.data(), z.data(), pm.data(),
                                static_cast<int>(P), &n))
             return gpu_fail("pfc_phimax_elastic");
-        pfcio::write_phimax_elastic("Phimax_" + std::to_string(kd) + ".txt", n, x.data(), y.data(), z.data(), pm.data());
         if (pfc_strain(L, M, N, o.p.dx, o.benchel, n, x.data(), y.data(), z.data(), st.data(), nb.data()))
             return gpu_fail("pfc_strain");
-        pfcio::write_strain("strain_" + std::to_string(kd) + ".txt", n, x.data(), y.data(), z.data(), st.data(), nb.data());
         if (pfc_cryst_order(L, M, N, o.p.dx, n, x.data(), y.data(), z.data(), q4.data(), q6.data(), nullptr))
             return gpu_fail("pfc_cryst_order");
-        pfcio::write_q4q6("q4q6_" + std::to_string(kd) + ".txt", kd, n, x.data(), y.data(), z.data(), q4.data(),
-                          q6.data());
+        // The three files of this checkpoint are written while the next one
+        // is read and processed.
+        auto head = [n](const std::vector<double>& v) { return std::vector<double>(v.begin(), v.begin() + n); };
+        out.post([kd, n, X = head(x), Y = head(y), Z = head(z), PM = head(pm), ST = head(st), Q4 = head(q4),
+                  Q6 = head(q6), NB = std::vector<unsigned>(nb.begin(), nb.begin() + n)] {
+            pfcio::write_phimax_elastic("Phimax_" + std::to_string(kd) + ".txt", n, X.data(), Y.data(), Z.data(),
+                                        PM.data());
+            pfcio::write_strain("strain_" + std::to_string(kd) + ".txt", n, X.data(), Y.data(), Z.data(), ST.data(),
+                                NB.data());
+            pfcio::write_q4q6("q4q6_" + std::to_string(kd) + ".txt", kd, n, X.data(), Y.data(), Z.data(), Q4.data(),
+                              Q6.data());
+        });
     }
+    out.drain();
     return 0;
 }
 

@@ -5,11 +5,83 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <filesystem>
 #include <set>
+#include <thread>
 
 namespace pfcio {
+
+// printf-style append; the same bytes as fprintf with the same format.
+static void appf(std::string& s, const char* fmt, ...) {
+    char b[256];
+    va_list a;
+    va_start(a, fmt);
+    const int k = std::vsnprintf(b, sizeof b, fmt, a);
+    va_end(a);
+    if (k < 0) return;
+    if (k < static_cast<int>(sizeof b)) {
+        s.append(b, static_cast<size_t>(k));
+        return;
+    }
+    std::string big(static_cast<size_t>(k) + 1, '\0');
+    va_start(a, fmt);
+    std::vsnprintf(&big[0], big.size(), fmt, a);
+    va_end(a);
+    s.append(big.data(), static_cast<size_t>(k));
+}
+
+// Rows [0, n) of a table, formatted by row(i, out) (appending nothing skips
+// the row) in contiguous chunks on up to 8 threads, then written in order:
+// the file is byte-for-byte what a sequential loop of the same formats writes.
+template <class Row>
+static void put_rows(FILE* f, int n, const Row& row) {
+    constexpr int kMinRows = 4096;
+    const int hw = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+    const int T = std::max(1, std::min({8, hw, n / kMinRows}));
+    std::vector<std::string> out(static_cast<size_t>(T));
+    auto chunk = [&](int t) {
+        const int a = static_cast<int>(static_cast<long long>(n) * t / T);
+        const int b = static_cast<int>(static_cast<long long>(n) * (t + 1) / T);
+        out[t].reserve(static_cast<size_t>(b - a) * 48);
+        for (int i = a; i < b; i++) row(i, out[t]);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; t++) th.emplace_back(chunk, t);
+    chunk(0);
+    for (auto& x : th) x.join();
+    for (const auto& o : out) std::fwrite(o.data(), 1, o.size(), f);
+}
+
+// A VTI payload (n, m, l with l fastest) from / to the n + N*(m + M*l)
+// layout, in 32 x 32 (m, l) blocks so both sides stay in cache lines.
+static void to_payload(int L, int M, int N, const float* field, float* pay) {
+    constexpr int B = 32;
+    for (int n = 0; n < N; n++)
+        for (int m0 = 0; m0 < M; m0 += B)
+            for (int l0 = 0; l0 < L; l0 += B) {
+                const int m1 = std::min(M, m0 + B), l1 = std::min(L, l0 + B);
+                for (int m = m0; m < m1; m++) {
+                    float* dst = pay + (static_cast<size_t>(n) * M + m) * L;
+                    for (int l = l0; l < l1; l++) dst[l] = field[n + static_cast<size_t>(N) * (m + static_cast<size_t>(M) * l)];
+                }
+            }
+}
+
+static void from_payload(int L, int M, int N, const float* pay, size_t count, float* field) {
+    constexpr int B = 32;
+    for (int n = 0; n < N; n++)
+        for (int m0 = 0; m0 < M; m0 += B)
+            for (int l0 = 0; l0 < L; l0 += B) {
+                const int m1 = std::min(M, m0 + B), l1 = std::min(L, l0 + B);
+                for (int m = m0; m < m1; m++) {
+                    const size_t row = (static_cast<size_t>(n) * M + m) * L;
+                    for (int l = l0; l < l1 && row + l < count; l++)
+                        field[n + static_cast<size_t>(N) * (m + static_cast<size_t>(M) * l)] = pay[row + l];
+                }
+            }
+}
 
 std::string g6(double v) {
     char b[64];
@@ -32,12 +104,9 @@ bool write_vti(const std::string& path, const char* name, int L, int M, int N, c
     const int bytes = static_cast<int>(static_cast<size_t>(N) * M * L * sizeof(float)) + static_cast<int>(sizeof(int));
     std::fwrite(&bytes, sizeof(int), 1, f);
     // Payload order n, m, l (l fastest): a transpose of the n+N*(m+M*l) layout.
-    std::vector<float> buf(static_cast<size_t>(L));
-    for (int n = 0; n < N; n++)
-        for (int m = 0; m < M; m++) {
-            for (int l = 0; l < L; l++) buf[l] = field[n + static_cast<size_t>(N) * (m + static_cast<size_t>(M) * l)];
-            std::fwrite(buf.data(), sizeof(float), L, f);
-        }
+    std::vector<float> pay(static_cast<size_t>(N) * M * L);
+    to_payload(L, M, N, field, pay.data());
+    std::fwrite(pay.data(), sizeof(float), pay.size(), f);
     std::fprintf(f, "\n</AppendedData>\n</VTKFile>\n");
     return std::fclose(f) == 0;
 }
@@ -65,16 +134,12 @@ long read_vti(const std::string& path, int L, int M, int N, float* field) {
     (void)std::fgetc(f);
     int bytes = 0;
     if (std::fread(&bytes, sizeof(int), 1, f) != 1) bytes = 0;
-    long got = 0;
-    std::vector<float> row(static_cast<size_t>(L));
-    for (int n = 0; n < N; n++)
-        for (int m = 0; m < M; m++) {
-            const size_t r = std::fread(row.data(), sizeof(float), L, f);
-            for (size_t l = 0; l < r; l++) field[n + static_cast<size_t>(N) * (m + static_cast<size_t>(M) * l)] = row[l];
-            got += static_cast<long>(r);
-        }
+    // Values in payload order; a short file fills the first `got` of them.
+    std::vector<float> pay(static_cast<size_t>(N) * M * L);
+    const size_t got = std::fread(pay.data(), sizeof(float), pay.size(), f);
     std::fclose(f);
-    return got;
+    from_payload(L, M, N, pay.data(), got, field);
+    return static_cast<long>(got);
 }
 
 static bool finite_atom(double x, double y, double z, double v, double c) {
@@ -89,12 +154,12 @@ bool write_phimax_misfit(const std::string& path, int n, const double* x, const 
     for (int i = 0; i < n; i++)
         if (finite_atom(x[i], y[i], z[i], phi[i], con[i]) && !(x[i] < 0)) cnt++;
     std::fprintf(f, "%d\nKommentar\n", 2 * cnt);
-    for (int i = 0; i < n; i++) {
+    put_rows(f, n, [&](int i, std::string& o) {
         const double c = con[i];
-        if (!finite_atom(x[i], y[i], z[i], phi[i], c) || x[i] < 0) continue;
-        std::fprintf(f, "%.4f %.4f %.4f %.4f %.4f\n", x[i], y[i], z[i], phi[i], c);
-        std::fprintf(f, "%.4f %.4f %.4f %.4f %.4f\n", x[i], y[i], z[i] + 20, phi[i], c);
-    }
+        if (!finite_atom(x[i], y[i], z[i], phi[i], c) || x[i] < 0) return;
+        appf(o, "%.4f %.4f %.4f %.4f %.4f\n", x[i], y[i], z[i], phi[i], c);
+        appf(o, "%.4f %.4f %.4f %.4f %.4f\n", x[i], y[i], z[i] + 20, phi[i], c);
+    });
     return std::fclose(f) == 0;
 }
 
@@ -103,8 +168,9 @@ bool write_phimax_elastic(const std::string& path, int n, const double* x, const
     FILE* f = std::fopen(path.c_str(), "w");
     if (!f) return false;
     std::fprintf(f, "%d\nKommentar\n", n);
-    for (int i = 0; i < n; i++)
-        if (x[i] >= 0) std::fprintf(f, "%.4f %.4f %.4f %.4f\n", x[i], y[i], z[i], phi[i]);
+    put_rows(f, n, [&](int i, std::string& o) {
+        if (x[i] >= 0) appf(o, "%.4f %.4f %.4f %.4f\n", x[i], y[i], z[i], phi[i]);
+    });
     return std::fclose(f) == 0;
 }
 
@@ -113,8 +179,9 @@ bool write_strain(const std::string& path, int n, const double* x, const double*
     FILE* f = std::fopen(path.c_str(), "w");
     if (!f) return false;
     std::fprintf(f, "%d\nKommentar\n", n);
-    for (int i = 0; i < n; i++)
-        if (x[i] >= 0) std::fprintf(f, "st %.4f %.4f %.4f %.4f %u\n", x[i], y[i], z[i], strain[i], nb[i]);
+    put_rows(f, n, [&](int i, std::string& o) {
+        if (x[i] >= 0) appf(o, "st %.4f %.4f %.4f %.4f %u\n", x[i], y[i], z[i], strain[i], nb[i]);
+    });
     return std::fclose(f) == 0;
 }
 
@@ -128,12 +195,14 @@ bool write_q4q6(const std::string& path, int kd, int n, const double* x, const d
     double x0 = 1e30, x1 = -1e30, y0 = 1e30, y1 = -1e30;
     for (int i = 0; i < n; i++) {
         if (!(x[i] >= 0)) continue;
-        std::fprintf(f, "qc %.4f %.4f %.4f %.4f %.4f\n", x[i], y[i], z[i], q4[i], q6[i]);
         if (x[i] < x0) x0 = x[i];
         if (x[i] > x1) x1 = x[i];
         if (y[i] < y0) y0 = y[i];
         if (y[i] > y1) y1 = y[i];
     }
+    put_rows(f, n, [&](int i, std::string& o) {
+        if (x[i] >= 0) appf(o, "qc %.4f %.4f %.4f %.4f %.4f\n", x[i], y[i], z[i], q4[i], q6[i]);
+    });
     std::fprintf(stderr, "q4q6_%d.txt: atoms=%d x=[%s,%s] y=[%s,%s]\n", kd, cnt, g6(x0).c_str(), g6(x1).c_str(),
                  g6(y0).c_str(), g6(y1).c_str());
     return std::fclose(f) == 0;
@@ -161,14 +230,15 @@ bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float ph
     std::fprintf(f, "H0(2,3) = 0 A \nH0(3,1) = 0 A \nH0(3,2) = 0 A \n");
     std::fprintf(f, "H0(3,3) = %.4f A \n", frame_x);
     std::fprintf(f, ".NO_VELOCITY.\nentry_count = 3\n63.546\nCu\n");
-    for (int i = 0; i < n; i++) std::fprintf(f, "%.4f %.4f %.4f\n", px[i], py[i], pz[i]);
+    put_rows(f, n, [&](int i, std::string& o) { appf(o, "%.4f %.4f %.4f\n", px[i], py[i], pz[i]); });
     std::fclose(f);
     std::snprintf(name, sizeof name, "Phimax_%d.txt", kd);
     f = std::fopen((pre + name).c_str(), "w");
     if (!f) return false;
     std::fprintf(f, "%d\nKommentar\n", n);
-    for (int i = 0; i < n; i++)
-        std::fprintf(f, "%.4f %.4f %.4f %.4f\n", pz[i] * frame_x, px[i] * frame_y, py[i] * frame_z, pphi[i]);
+    put_rows(f, n, [&](int i, std::string& o) {
+        appf(o, "%.4f %.4f %.4f %.4f\n", pz[i] * frame_x, px[i] * frame_y, py[i] * frame_z, pphi[i]);
+    });
     return std::fclose(f) == 0;
 }
 
@@ -178,6 +248,11 @@ bool write_cvd_atoms(int kd, int L, int M, int N, float dx, float phic, float ph
 static void put_fixed(FILE* f, double v, int prec) {
     if (std::isnan(v)) std::fputs("-nan", f);
     else std::fprintf(f, "%.*f", prec, v);
+}
+
+static void app_fixed(std::string& o, double v, int prec) {
+    if (std::isnan(v)) o += "-nan";
+    else appf(o, "%.*f", prec, v);
 }
 
 bool write_cvd_gr(int kd, int L, int M, int N, float dx, int n, const float* px, const unsigned* hist, int maxbin,
@@ -218,13 +293,13 @@ bool write_cvd_q6(int kd, int n, const unsigned* nb, const float* ql, float qglo
     std::snprintf(name, sizeof name, "Ql_%d.txt", kd);
     FILE* f = std::fopen((pre + name).c_str(), "w");
     if (!f) return false;
-    for (int i = 0; i < n; i++) {
-        std::fprintf(f, "%d %u ", i, nb[i]);
-        put_fixed(f, ql[i], 6);
-        std::fputc(' ', f);
-        put_fixed(f, qglobal, 6);
-        std::fputc('\n', f);
-    }
+    put_rows(f, n, [&](int i, std::string& o) {
+        appf(o, "%d %u ", i, nb[i]);
+        app_fixed(o, ql[i], 6);
+        o += ' ';
+        app_fixed(o, qglobal, 6);
+        o += '\n';
+    });
     std::fclose(f);
     std::snprintf(name, sizeof name, "HistQ_%d.txt", kd);
     f = std::fopen((pre + name).c_str(), "w");
@@ -263,6 +338,46 @@ bool make_dirs(const std::string& path) {
     std::error_code ec;
     std::filesystem::create_directories(path, ec);
     return !ec;
+}
+
+Writer::Writer(int depth) : depth_(depth < 1 ? 1 : depth), th_([this] { loop(); }) {}
+
+Writer::~Writer() {
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    th_.join();
+}
+
+void Writer::post(std::function<void()> task) {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [this] { return static_cast<int>(q_.size()) < depth_; });
+    q_.push_back(std::move(task));
+    cv_.notify_all();
+}
+
+void Writer::drain() {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [this] { return q_.empty() && !busy_; });
+}
+
+void Writer::loop() {
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+        if (q_.empty()) return;  // stop_ and nothing left
+        std::function<void()> task = std::move(q_.front());
+        q_.pop_front();
+        busy_ = true;
+        cv_.notify_all();
+        lk.unlock();
+        task();
+        lk.lock();
+        busy_ = false;
+        cv_.notify_all();
+    }
 }
 
 long long now_ms() {

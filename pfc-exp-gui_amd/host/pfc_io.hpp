@@ -1,7 +1,12 @@
 // pfc_io.hpp - file formats of pfc-exp-cli (what the Qt GUI reads).
 #pragma once
+#include <condition_variable>
 #include <cstdio>
+#include <deque>
+#include <functional>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace pfcio {
@@ -49,5 +54,28 @@ std::vector<int> checkpoint_steps(const std::string& dir);
 
 bool make_dirs(const std::string& path);
 long long now_ms();
+
+// Checkpoint files written behind the stepping: a background thread runs the
+// posted tasks in order (so appends to one file keep their order); post()
+// blocks while `depth` tasks wait, which bounds the host copies in flight.
+// drain() returns when every posted task has run; the destructor drains.
+class Writer {
+public:
+    explicit Writer(int depth = 2);
+    ~Writer();
+    Writer(const Writer&) = delete;
+    Writer& operator=(const Writer&) = delete;
+    void post(std::function<void()> task);
+    void drain();
+
+private:
+    void loop();
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> q_;
+    int depth_;
+    bool busy_ = false, stop_ = false;
+    std::thread th_;
+};
 
 }  // namespace pfcio

@@ -156,3 +156,34 @@ def test_3d_cli_matches_oracle(tmp_path, model, extra):
     eo = np.loadtxt(o / "energy.txt")
     assert np.array_equal(eg[:, 0], eo[:, 0])
     assert np.allclose(eg[:, 1:], eo[:, 1:], rtol=2e-5, atol=1e-9)
+
+
+def test_cvd_1024_checkpoints_elastic(tmp_path):
+    """CVD 1024^2 in FP32 (energy every step, checkpoints every 500 steps),
+    then the GPU elastic post-processor and the oracle's on the same VTI
+    files: Phimax and strain byte-identical, q4q6 within its last decimal."""
+    src = tmp_path / "src"
+    r = run(CLI, ["--model", "cvd", "--precision", "fp32", "--L", "1024", "--M", "1024", "--u0", "0.05",
+                  "--con0", "0.25", "--steps", "1001", "--mod", "500", "--outdir", str(src)], tmp_path)
+    assert len([l for l in r.stderr.splitlines() if PROGRESS.match(l)]) == 1001
+    e = np.loadtxt(src / "energy.txt")
+    assert e.shape == (1001, 3) and np.array_equal(e[:, 0], np.arange(1001)) and np.all(np.isfinite(e))
+    ck = [int(l.split()[1]) for l in (src / "checkpoint_timestamps.txt").read_text().splitlines()[1:]]
+    assert ck == [0, 500, 1000]
+    for kd in ck:
+        for f in (f"phi_{kd}_0.vti", f"Phimax_{kd}.txt", f"phiball_{kd}.cfg", f"hist_{kd}.txt", f"Ql_{kd}.txt"):
+            assert (src / f).exists(), f
+    g, o = tmp_path / "g", tmp_path / "o"
+    common = ["--model", "elastic", "--L", "1024", "--M", "1024", "--input-dir", str(src)]
+    run(CLI, common + ["--outdir", str(g)], tmp_path)
+    run(OCLI, common + ["--outdir", str(o)], tmp_path)
+    for kd in (500, 1000):
+        for f in (f"Phimax_{kd}.txt", f"strain_{kd}.txt"):
+            assert (g / f).read_bytes() == (o / f).read_bytes(), f
+        assert int((g / f"Phimax_{kd}.txt").read_text().split("\n", 1)[0]) > 10000
+        a, b = (g / f"q4q6_{kd}.txt").read_text().splitlines(), (o / f"q4q6_{kd}.txt").read_text().splitlines()
+        assert a[:2] == b[:2] and len(a) == len(b) > 2
+        ta = np.array([[float(t) for t in l.split()[1:]] for l in a[2:]])
+        tb = np.array([[float(t) for t in l.split()[1:]] for l in b[2:]])
+        assert np.array_equal(ta[:, :3], tb[:, :3])
+        assert np.abs(ta[:, 3:] - tb[:, 3:]).max() <= 1.5e-4
