@@ -54,6 +54,7 @@ struct pfc_ctx {
     double *erow = nullptr, *ecol[2] = {nullptr, nullptr}, *eout = nullptr;
     int nrow = 0, ncol = 0;
     void *stage_a = nullptr, *stage_b = nullptr;
+    void* host_stage = nullptr;  // pinned [2][L*M*N] reals for read-back, made at the first read
     long long* tcount = nullptr;  // [B] step index per instance (the noise RNG's t)
     // noise term (misfit --axx > 0)
     bool noise = false;
@@ -164,6 +165,7 @@ static void free_ctx(pfc_ctx* c) {
                     c->tcount, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr, c->coefA, c->coefC};
     for (void* q : ptrs)
         if (q) hipFree(q);
+    if (c->host_stage) hipHostFree(c->host_stage);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -658,34 +660,53 @@ static int fields_to_stage(pfc_ctx* c, int inst) {
     return PFC_OK;
 }
 
-extern "C" int pfc_get_fields(pfc_ctx* c, int inst, double* phi, double* con) {
-    if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
+// Both real fields of instance `inst` into the pinned read-back buffer (one
+// synchronisation); *a and *b point at them there.
+static int fields_to_host(pfc_ctx* c, int inst, const void** a, const void** b) {
     int rc = fields_to_stage(c, inst);
     if (rc) return rc;
+    const size_t bytes = (size_t)c->L * c->M * c->N * c->rsize;
+    if (!c->host_stage) HIP_TRY(hipHostMalloc(&c->host_stage, 2 * bytes, hipHostMallocDefault));
+    char* h = (char*)c->host_stage;
+    const size_t poff = (size_t)inst * bytes;
+    HIP_TRY(hipMemcpyAsync(h, (char*)c->stage_a + poff, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(h + bytes, (char*)c->stage_b + poff, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *a = h;
+    *b = h + bytes;
+    return PFC_OK;
+}
+
+extern "C" int pfc_get_fields(pfc_ctx* c, int inst, double* phi, double* con) {
+    if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
+    const void *ha = nullptr, *hb = nullptr;
+    int rc = fields_to_host(c, inst, &ha, &hb);
+    if (rc) return rc;
     const size_t P = (size_t)c->L * c->M * c->N;
-    const size_t poff = (size_t)inst * P * c->rsize;
-    std::vector<char> h(P * c->rsize);
-    if (phi) {
-        HIP_TRY(hipMemcpyAsync(h.data(), (char*)c->stage_a + poff, P * c->rsize, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        if (c->prec == PREC_FP64) widen<double>(h.data(), phi, P); else widen<float>(h.data(), phi, P);
-    }
-    if (con) {
-        HIP_TRY(hipMemcpyAsync(h.data(), (char*)c->stage_b + poff, P * c->rsize, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        if (c->prec == PREC_FP64) widen<double>(h.data(), con, P); else widen<float>(h.data(), con, P);
+    if (c->prec == PREC_FP64) {
+        if (phi) std::memcpy(phi, ha, P * sizeof(double));
+        if (con) std::memcpy(con, hb, P * sizeof(double));
+    } else {
+        if (phi) widen<float>(ha, phi, P);
+        if (con) widen<float>(hb, con, P);
     }
     return PFC_OK;
 }
 
 extern "C" int pfc_get_fields_f32(pfc_ctx* c, int inst, float* phi, float* con) {
     if (!c || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
-    const size_t P = (size_t)c->L * c->M * c->N;
-    std::vector<double> a(phi ? P : 0), b(con ? P : 0);
-    int rc = pfc_get_fields(c, inst, phi ? a.data() : nullptr, con ? b.data() : nullptr);
+    const void *ha = nullptr, *hb = nullptr;
+    int rc = fields_to_host(c, inst, &ha, &hb);
     if (rc) return rc;
-    for (size_t i = 0; phi && i < P; i++) phi[i] = (float)a[i];
-    for (size_t i = 0; con && i < P; i++) con[i] = (float)b[i];
+    const size_t P = (size_t)c->L * c->M * c->N;
+    if (c->prec == PREC_FP64) {
+        const double *a = (const double*)ha, *b = (const double*)hb;
+        for (size_t i = 0; phi && i < P; i++) phi[i] = (float)a[i];
+        for (size_t i = 0; con && i < P; i++) con[i] = (float)b[i];
+    } else {
+        if (phi) std::memcpy(phi, ha, P * sizeof(float));
+        if (con) std::memcpy(con, hb, P * sizeof(float));
+    }
     return PFC_OK;
 }
 

@@ -461,10 +461,16 @@ hipError_t post_phimax_elastic(int L, int M, int N, float dx, const float* phi, 
 }
 
 // ---------------------------------------------------------------------------
-// elastic3D (elastic.cpp:3652-3729): one thread per atom i, all j in order,
-// j tiles staged through LDS.  Float distances with minimum image, at most
-// 16 stored neighbours; terms for 1.3 <= r < benchel (uninitialised reads in
-// the reference) are 0.
+// Ordered neighbour search (elastic3D, cryst_order, Q6_GLO).  The reference
+// walks every j in index order for each atom i, counts every j that passes a
+// distance test and works on the first CAP of them, each step using values
+// the previous accepted neighbour left (rjx/rjy/rjz).  Here the j range is
+// cut into S slices of whole LDS tiles: k_nb_scan runs one thread per
+// (i, slice) and records the slice's count and first CAP hits in j order;
+// a finishing kernel per atom walks the slices in order, which yields the
+// same first CAP neighbours and total count, and applies the reference's
+// per-neighbour steps to them in order.  S * (n / 256) workgroups fill the
+// chip where one thread per atom gave fewer workgroups than CUs.
 // ---------------------------------------------------------------------------
 constexpr int SB = 256;
 
@@ -473,80 +479,207 @@ __device__ __forceinline__ float min3f(float a, float b, float c) {
     return ((aa < bb ? aa : bb) < cc) ? (aa < bb ? aa : bb) : cc;
 }
 
-__global__ void __launch_bounds__(SB) k_strain(int n, const double* __restrict__ X, const double* __restrict__ Y,
-                                               const double* __restrict__ Z, float fx, float fy, float fz,
-                                               float benchel, double* strain, unsigned* nbo) {
+// Elastic pair (double positions): float differences, their +-frame shifts,
+// the smallest |.| per axis and the float distance.
+__device__ __forceinline__ float el_pair(double xi, double yi, double zi, double xj, double yj, double zj, float fx,
+                                         float fy, float fz, float* r3, float& drx, float& dry, float& drz) {
 #pragma clang fp contract(off)
-    __shared__ double sx[SB], sy[SB], sz[SB];
+    r3[0] = (float)(xi - xj);
+    r3[1] = (float)(yi - yj);
+    r3[2] = (float)(zi - zj);
+    r3[3] = r3[0] + fx; r3[4] = r3[1] + fy; r3[5] = r3[2] + fz;
+    r3[6] = r3[0] - fx; r3[7] = r3[1] - fy; r3[8] = r3[2] - fz;
+    drx = min3f(fabsf(r3[0]), fabsf(r3[3]), fabsf(r3[6]));
+    dry = min3f(fabsf(r3[1]), fabsf(r3[4]), fabsf(r3[7]));
+    drz = min3f(fabsf(r3[2]), fabsf(r3[5]), fabsf(r3[8]));
+    return sqrtf(drx * drx + dry * dry + drz * drz);
+}
+
+// The stored neighbour position of the elastic kernels: per axis, the last
+// shift (in 0, +, - order) whose |.| is within 1e-3 of the minimum.
+__device__ __forceinline__ void el_rj(double xi, double yi, double zi, const float* r3, float drx, float dry,
+                                      float drz, float& rjx, float& rjy, float& rjz) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int q = 0; q < 7; q += 3)
+        if ((double)fabsf(fabsf(r3[q]) - drx) < 1e-3) rjx = (float)(xi - (double)r3[q]);
+#pragma unroll
+    for (int q = 1; q < 8; q += 3)
+        if ((double)fabsf(fabsf(r3[q]) - dry) < 1e-3) rjy = (float)(yi - (double)r3[q]);
+#pragma unroll
+    for (int q = 2; q < 9; q += 3)
+        if ((double)fabsf(fabsf(r3[q]) - drz) < 1e-3) rjz = (float)(zi - (double)r3[q]);
+}
+
+// CVD pair (float fractions): fraction differences times the frame per
+// axis, the +-frame shifts, the smallest |.| per axis and the float distance
+// plus 1e-6 (cvd.cpp:1297-1309).
+__device__ __forceinline__ float cvd_pair(float xi, float yi, float zi, float xj, float yj, float zj, float fx,
+                                          float fy, float fz, float* r3, float& drx, float& dry, float& drz) {
+#pragma clang fp contract(off)
+    r3[0] = (xi - xj) * fx;
+    r3[1] = (yi - yj) * fy;
+    r3[2] = (zi - zj) * fz;
+    r3[3] = r3[0] + fx; r3[4] = r3[1] + fy; r3[5] = r3[2] + fz;
+    r3[6] = r3[0] - fx; r3[7] = r3[1] - fy; r3[8] = r3[2] - fz;
+    drx = min3f(r3[0], r3[3], r3[6]);
+    dry = min3f(r3[1], r3[4], r3[7]);
+    drz = min3f(r3[2], r3[5], r3[8]);
+    return (float)((double)sqrtf(drx * drx + dry * dry + drz * drz) + 1e-6);
+}
+
+// Neighbour tests.  Each gives the position type, the number of neighbours
+// the reference stores and the acceptance test of one pair.
+struct StrainTest {  // benchel > r > 0.7 (elastic.cpp:3671), 16 stored
+    using T = double;
+    static constexpr int CAP = 16;
+    float fx, fy, fz, benchel;
+    __device__ bool operator()(double xi, double yi, double zi, double xj, double yj, double zj) const {
+        float r3[9], drx, dry, drz;
+        const float rr = el_pair(xi, yi, zi, xj, yj, zj, fx, fy, fz, r3, drx, dry, drz);
+        return rr < benchel && (double)rr > 0.7;
+    }
+};
+
+struct CrystTest {  // r < bench100_h = 1.2 (elastic.cpp:105,3066), 16 stored
+    using T = double;
+    static constexpr int CAP = 16;
+    float fx, fy, fz;
+    __device__ bool operator()(double xi, double yi, double zi, double xj, double yj, double zj) const {
+        float r3[9], drx, dry, drz;
+        return el_pair(xi, yi, zi, xj, yj, zj, fx, fy, fz, r3, drx, dry, drz) < 1.2f;
+    }
+};
+
+struct Q6Test {  // a*sqrt(3)/2 * [0.9, 1.2] -+ 1e-3 (cvd.cpp:1393), 24 stored
+    using T = float;
+    static constexpr int CAP = 24;
+    float fx, fy, fz;
+    double rlo, rhi;
+    __device__ bool operator()(float xi, float yi, float zi, float xj, float yj, float zj) const {
+        float r3[9], drx, dry, drz;
+        const float rr = cvd_pair(xi, yi, zi, xj, yj, zj, fx, fy, fz, r3, drx, dry, drz);
+        return (double)rr <= rhi && (double)rr >= rlo;
+    }
+};
+
+// cnt[i*S + s] = hits of atom i in slice s; lst[(i*S + s)*CAP + k] its k-th
+// hit (k < CAP) in j order.  Slice s covers j in [s*slice, (s+1)*slice).
+template <class Test>
+__global__ void __launch_bounds__(SB) k_nb_scan(Test test, int n, const typename Test::T* __restrict__ X,
+                                                const typename Test::T* __restrict__ Y,
+                                                const typename Test::T* __restrict__ Z, int slice, int* cnt,
+                                                int* lst) {
+    using T = typename Test::T;
+    __shared__ T sx[SB], sy[SB], sz[SB];
+    const int S = gridDim.y, s = blockIdx.y;
     const int i = blockIdx.x * SB + threadIdx.x;
     const bool act = i < n;
-    const double xi = act ? X[i] : 0.0, yi = act ? Y[i] : 0.0, zi = act ? Z[i] : 0.0;
-    unsigned Nb = 0;
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    float rjx = 0.f, rjy = 0.f, rjz = 0.f;
-    for (int j0 = 0; j0 < n; j0 += SB) {
+    const T xi = act ? X[i] : T(0), yi = act ? Y[i] : T(0), zi = act ? Z[i] : T(0);
+    const int jb = s * slice, je = min(n, jb + slice);
+    int* out = lst + ((size_t)(act ? i : 0) * S + s) * Test::CAP;
+    int c = 0;
+    for (int j0 = jb; j0 < je; j0 += SB) {
         __syncthreads();
         const int jl = j0 + threadIdx.x;
-        if (jl < n) { sx[threadIdx.x] = X[jl]; sy[threadIdx.x] = Y[jl]; sz[threadIdx.x] = Z[jl]; }
+        if (jl < je) { sx[threadIdx.x] = X[jl]; sy[threadIdx.x] = Y[jl]; sz[threadIdx.x] = Z[jl]; }
         __syncthreads();
-        const int cnt = n - j0 < SB ? n - j0 : SB;
         if (!act) continue;
-        for (int jj = 0; jj < cnt; jj++) {
+        const int m = min(SB, je - j0);
+        for (int jj = 0; jj < m; jj++) {
             const int j = j0 + jj;
-            if (j == i) continue;
-            float r3[9];
-            r3[0] = (float)(xi - sx[jj]);
-            r3[1] = (float)(yi - sy[jj]);
-            r3[2] = (float)(zi - sz[jj]);
-            r3[3] = r3[0] + fx; r3[4] = r3[1] + fy; r3[5] = r3[2] + fz;
-            r3[6] = r3[0] - fx; r3[7] = r3[1] - fy; r3[8] = r3[2] - fz;
-            const float drx = min3f(fabsf(r3[0]), fabsf(r3[3]), fabsf(r3[6]));
-            const float dry = min3f(fabsf(r3[1]), fabsf(r3[4]), fabsf(r3[7]));
-            const float drz = min3f(fabsf(r3[2]), fabsf(r3[5]), fabsf(r3[8]));
-            const float rr = sqrtf(drx * drx + dry * dry + drz * drz);
-            if (!(rr < benchel && (double)rr > 0.7)) continue;
-            Nb = Nb + 1;
-            if (Nb > 16u) continue;
-            const double rij = rr;
-#pragma unroll
-            for (int q = 0; q < 7; q += 3)
-                if ((double)fabsf(fabsf(r3[q]) - drx) < 1e-3) rjx = (float)(xi - (double)r3[q]);
-#pragma unroll
-            for (int q = 1; q < 8; q += 3)
-                if ((double)fabsf(fabsf(r3[q]) - dry) < 1e-3) rjy = (float)(yi - (double)r3[q]);
-#pragma unroll
-            for (int q = 2; q < 9; q += 3)
-                if ((double)fabsf(fabsf(r3[q]) - drz) < 1e-3) rjz = (float)(zi - (double)r3[q]);
-            if (rij < 1.3) {
-                ax = ax + 1.0 * (rij - 1) * (((double)rjx - xi) / rij);
-                ay = ay + 1.0 * (rij - 1) * (((double)rjy - yi) / rij);
-                az = az + 1.0 * (rij - 1) * (((double)rjz - zi) / rij);
-            } else {
-                ax = ax + 0.0; ay = ay + 0.0; az = az + 0.0;
-            }
+            if (j == i || !test(xi, yi, zi, sx[jj], sy[jj], sz[jj])) continue;
+            if (c < Test::CAP) out[c] = j;
+            c++;
         }
     }
-    if (act) {
-        strain[i] = sqrt(ax * ax + ay * ay + az * az);
-        nbo[i] = Nb;
+    if (act) cnt[(size_t)i * S + s] = c;
+}
+
+// Calls f(k, j) for the first CAP neighbours of atom i in j order (k = 0,
+// 1, ...); returns the total count.
+template <int CAP, class F>
+__device__ __forceinline__ unsigned nb_walk(int i, int S, const int* __restrict__ cnt, const int* __restrict__ lst,
+                                            F&& f) {
+    unsigned tot = 0;
+    int k = 0;
+    for (int s = 0; s < S; s++) {
+        const int c = cnt[(size_t)i * S + s];
+        tot += (unsigned)c;
+        const int* src = lst + ((size_t)i * S + s) * CAP;
+        for (int q = 0; q < c && q < CAP && k < CAP; q++) f(k++, src[q]);
     }
+    return tot;
+}
+
+struct NbSlices {
+    int S = 1, slice = SB;
+};
+
+// Slices of whole LDS tiles, about 2048 j each, at most 16.
+static NbSlices nb_slices(int n) {
+    NbSlices r;
+    const int tiles = (n + SB - 1) / SB;
+    int S = n / 2048;
+    S = S < 1 ? 1 : (S > 16 ? 16 : S);
+    const int per = (tiles + S - 1) / S;
+    r.slice = (per < 1 ? 1 : per) * SB;
+    r.S = (n + r.slice - 1) / r.slice;
+    if (r.S < 1) r.S = 1;
+    return r;
+}
+
+// elastic3D (elastic.cpp:3652-3729): Nb counts every neighbour, the first 16
+// contribute; terms for 1.3 <= r < benchel (uninitialised reads in the
+// reference) are 0.
+__global__ void __launch_bounds__(SB) k_strain_fin(int n, const double* __restrict__ X, const double* __restrict__ Y,
+                                                   const double* __restrict__ Z, float fx, float fy, float fz, int S,
+                                                   const int* __restrict__ cnt, const int* __restrict__ lst,
+                                                   double* strain, unsigned* nbo) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * SB + threadIdx.x;
+    if (i >= n) return;
+    const double xi = X[i], yi = Y[i], zi = Z[i];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    float rjx = 0.f, rjy = 0.f, rjz = 0.f;
+    const unsigned Nb = nb_walk<StrainTest::CAP>(i, S, cnt, lst, [&](int, int j) {
+        float r3[9], drx, dry, drz;
+        const double rij = el_pair(xi, yi, zi, X[j], Y[j], Z[j], fx, fy, fz, r3, drx, dry, drz);
+        el_rj(xi, yi, zi, r3, drx, dry, drz, rjx, rjy, rjz);
+        if (rij < 1.3) {
+            ax = ax + 1.0 * (rij - 1) * (((double)rjx - xi) / rij);
+            ay = ay + 1.0 * (rij - 1) * (((double)rjy - yi) / rij);
+            az = az + 1.0 * (rij - 1) * (((double)rjz - zi) / rij);
+        } else {
+            ax = ax + 0.0; ay = ay + 0.0; az = az + 0.0;
+        }
+    });
+    strain[i] = sqrt(ax * ax + ay * ay + az * az);
+    nbo[i] = Nb;
 }
 
 hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
                        const double* z, double* strain, unsigned* nb) {
     hipError_t e = hipSuccess;
+    if (n <= 0) return e;
+    const NbSlices ns = nb_slices(n);
     Scratch sc;
     double* dx_ = sc.get<double>(n, e);
     double* dy_ = sc.get<double>(n, e);
     double* dz_ = sc.get<double>(n, e);
     double* ds = sc.get<double>(n, e);
     unsigned* dn = sc.get<unsigned>(n, e);
+    int* cnt = sc.get<int>((size_t)n * ns.S, e);
+    int* lst = sc.get<int>((size_t)n * ns.S * StrainTest::CAP, e);
     if (e != hipSuccess) return e;
     if ((e = hipMemcpy(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(dz_, z, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
     const float fx = L * dx, fy = M * dx, fz = N * dx;
-    hipLaunchKernelGGL(k_strain, dim3((n + SB - 1) / SB), dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, benchel, ds, dn);
+    const StrainTest t{fx, fy, fz, benchel};
+    const int gi = (n + SB - 1) / SB;
+    hipLaunchKernelGGL(k_nb_scan<StrainTest>, dim3(gi, ns.S), dim3(SB), 0, 0, t, n, dx_, dy_, dz_, ns.slice, cnt, lst);
+    hipLaunchKernelGGL(k_strain_fin, dim3(gi), dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, ns.S, cnt, lst, ds, dn);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpy(strain, ds, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
     return hipMemcpy(nb, dn, sizeof(unsigned) * n, hipMemcpyDeviceToHost);
@@ -618,58 +751,28 @@ __device__ __forceinline__ void dv_ylm(int l, int m, float theta, float fai, flo
     else yc = nr * plm * sinf(am * fai);
 }
 
-__global__ void __launch_bounds__(SB) k_cryst_nb(int n, const double* __restrict__ X, const double* __restrict__ Y,
-                                                 const double* __restrict__ Z, float fx, float fy, float fz,
-                                                 unsigned* nbo, int* nbi, double* theta, double* fai) {
+// First 16 neighbours (r < 1.2) in j order: index, theta and fai as the
+// reference forms them from the stored neighbour position; Nb counts all.
+__global__ void __launch_bounds__(SB) k_cryst_fin(int n, const double* __restrict__ X, const double* __restrict__ Y,
+                                                  const double* __restrict__ Z, float fx, float fy, float fz, int S,
+                                                  const int* __restrict__ cnt, const int* __restrict__ lst,
+                                                  unsigned* nbo, int* nbi, double* theta, double* fai) {
 #pragma clang fp contract(off)
-    __shared__ double sx[SB], sy[SB], sz[SB];
     const int i = blockIdx.x * SB + threadIdx.x;
-    const bool act = i < n;
-    const double xi = act ? X[i] : 0.0, yi = act ? Y[i] : 0.0, zi = act ? Z[i] : 0.0;
-    unsigned Nb = 0;
+    if (i >= n) return;
+    const double xi = X[i], yi = Y[i], zi = Z[i];
     float rjx = 0.f, rjy = 0.f, rjz = 0.f;
-    for (int j0 = 0; j0 < n; j0 += SB) {
-        __syncthreads();
-        const int jl = j0 + threadIdx.x;
-        if (jl < n) { sx[threadIdx.x] = X[jl]; sy[threadIdx.x] = Y[jl]; sz[threadIdx.x] = Z[jl]; }
-        __syncthreads();
-        const int cnt = n - j0 < SB ? n - j0 : SB;
-        if (!act) continue;
-        for (int jj = 0; jj < cnt; jj++) {
-            const int j = j0 + jj;
-            if (j == i) continue;
-            float r3[9];
-            r3[0] = (float)(xi - sx[jj]);
-            r3[1] = (float)(yi - sy[jj]);
-            r3[2] = (float)(zi - sz[jj]);
-            r3[3] = r3[0] + fx; r3[4] = r3[1] + fy; r3[5] = r3[2] + fz;
-            r3[6] = r3[0] - fx; r3[7] = r3[1] - fy; r3[8] = r3[2] - fz;
-            const float drx = min3f(fabsf(r3[0]), fabsf(r3[3]), fabsf(r3[6]));
-            const float dry = min3f(fabsf(r3[1]), fabsf(r3[4]), fabsf(r3[7]));
-            const float drz = min3f(fabsf(r3[2]), fabsf(r3[5]), fabsf(r3[8]));
-            const float rr = sqrtf(drx * drx + dry * dry + drz * drz);
-            if (!(rr < 1.2f)) continue;  // bench100_h (elastic.cpp:105)
-            Nb = Nb + 1;
-            if (Nb > (unsigned)CNMAX) continue;
-#pragma unroll
-            for (int q = 0; q < 7; q += 3)
-                if ((double)fabsf(fabsf(r3[q]) - drx) < 1e-3) rjx = (float)(xi - (double)r3[q]);
-#pragma unroll
-            for (int q = 1; q < 8; q += 3)
-                if ((double)fabsf(fabsf(r3[q]) - dry) < 1e-3) rjy = (float)(yi - (double)r3[q]);
-#pragma unroll
-            for (int q = 2; q < 9; q += 3)
-                if ((double)fabsf(fabsf(r3[q]) - drz) < 1e-3) rjz = (float)(zi - (double)r3[q]);
-            const size_t s = (size_t)i * CNMAX + (Nb - 1);
-            const double rij = rr;
-            theta[s] = acos(((double)rjz - zi) / (rij + 1.e-4));
-            double f = atan(((double)rjy - yi) / ((double)rjx - xi + 1.e-4));
-            if ((double)rjx - xi < 0.) f = f + 3.1415926;
-            fai[s] = f;
-            nbi[s] = j;
-        }
-    }
-    if (act) nbo[i] = Nb;
+    nbo[i] = nb_walk<CrystTest::CAP>(i, S, cnt, lst, [&](int k, int j) {
+        float r3[9], drx, dry, drz;
+        const double rij = el_pair(xi, yi, zi, X[j], Y[j], Z[j], fx, fy, fz, r3, drx, dry, drz);
+        el_rj(xi, yi, zi, r3, drx, dry, drz, rjx, rjy, rjz);
+        const size_t s = (size_t)i * CNMAX + k;
+        theta[s] = acos(((double)rjz - zi) / (rij + 1.e-4));
+        double f = atan(((double)rjy - yi) / ((double)rjx - xi + 1.e-4));
+        if ((double)rjx - xi < 0.) f = f + 3.1415926;
+        fai[s] = f;
+        nbi[s] = j;
+    });
 }
 
 // qlm[(i*QM + m+QL)*2 + {0,1}] = sum_j Y_6m(j) / Nb (double sums of float terms).
@@ -741,13 +844,18 @@ hipError_t post_cryst(int L, int M, int N, float dx, int n, const double* x, con
     double* dq = sc.get<double>((size_t)n * QM * 2, e);
     double* d4 = sc.get<double>(n, e);
     double* d6 = sc.get<double>(n, e);
+    const NbSlices ns = nb_slices(n);
+    int* cnt = sc.get<int>((size_t)n * ns.S, e);
+    int* lst = sc.get<int>((size_t)n * ns.S * CrystTest::CAP, e);
     if (e != hipSuccess) return e;
     if ((e = hipMemcpy(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(dz_, z, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
     const float fx = L * dx, fy = M * dx, fz = N * dx;
     const dim3 g((n + SB - 1) / SB);
-    hipLaunchKernelGGL(k_cryst_nb, g, dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, dn, di, dth, dfa);
+    const CrystTest t{fx, fy, fz};
+    hipLaunchKernelGGL(k_nb_scan<CrystTest>, dim3(g.x, ns.S), dim3(SB), 0, 0, t, n, dx_, dy_, dz_, ns.slice, cnt, lst);
+    hipLaunchKernelGGL(k_cryst_fin, g, dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, ns.S, cnt, lst, dn, di, dth, dfa);
     hipLaunchKernelGGL(k_cryst_qlm, g, dim3(SB), 0, 0, n, dn, dth, dfa, dq);
     hipLaunchKernelGGL(k_cryst_q, g, dim3(SB), 0, 0, n, dn, di, dq, d4, d6);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -770,21 +878,8 @@ hipError_t post_cryst(int L, int M, int N, float dx, int n, const double* x, con
 // ---------------------------------------------------------------------------
 constexpr int GRJT = 8;          // j tiles per GR block
 constexpr int GR_LDS_BINS = 12288;  // 48 KB of dynamic LDS
-constexpr int Q6CN = 24, Q6NBIN = 30, QB = 64;
+constexpr int Q6CN = 24, Q6NBIN = 30;
 
-__device__ __forceinline__ float cvd_pair(float xi, float yi, float zi, float xj, float yj, float zj, float fx,
-                                          float fy, float fz, float* r3, float& drx, float& dry, float& drz) {
-#pragma clang fp contract(off)
-    r3[0] = (xi - xj) * fx;
-    r3[1] = (yi - yj) * fy;
-    r3[2] = (zi - zj) * fz;
-    r3[3] = r3[0] + fx; r3[4] = r3[1] + fy; r3[5] = r3[2] + fz;
-    r3[6] = r3[0] - fx; r3[7] = r3[1] - fy; r3[8] = r3[2] - fz;
-    drx = min3f(r3[0], r3[3], r3[6]);
-    dry = min3f(r3[1], r3[4], r3[7]);
-    drz = min3f(r3[2], r3[5], r3[8]);
-    return (float)((double)sqrtf(drx * drx + dry * dry + drz * drz) + 1e-6);
-}
 
 template <bool LDSH>
 __global__ void __launch_bounds__(SB) k_cvd_gr(int n, const float* __restrict__ X, const float* __restrict__ Y,
@@ -825,48 +920,34 @@ __global__ void __launch_bounds__(SB) k_cvd_gr(int n, const float* __restrict__ 
     }
 }
 
-// Neighbour list of entry i in j order with its (theta, fai) as the
-// reference forms them from the stored neighbour position.
-__global__ void __launch_bounds__(QB) k_q6_nb(int n, const float* __restrict__ X, const float* __restrict__ Y,
-                                              const float* __restrict__ Z, float fx, float fy, float fz, double rlo,
-                                              double rhi, unsigned* nbo, float* theta, float* fai) {
+// First 24 neighbours of entry i in j order with their (theta, fai) as the
+// reference forms them from the stored neighbour position; Nb counts all.
+__global__ void __launch_bounds__(SB) k_q6_fin(int n, const float* __restrict__ X, const float* __restrict__ Y,
+                                               const float* __restrict__ Z, float fx, float fy, float fz, int S,
+                                               const int* __restrict__ cnt, const int* __restrict__ lst, unsigned* nbo,
+                                               float* theta, float* fai) {
 #pragma clang fp contract(off)
-    __shared__ float sx[QB], sy[QB], sz[QB];
-    const int i = blockIdx.x * QB + threadIdx.x;
-    const bool act = i < n;
-    const float xi = act ? X[i] : 0.f, yi = act ? Y[i] : 0.f, zi = act ? Z[i] : 0.f;
-    unsigned Nb = 0;
+    const int i = blockIdx.x * SB + threadIdx.x;
+    if (i >= n) return;
+    const float xi = X[i], yi = Y[i], zi = Z[i];
     float rjx = 0.f, rjy = 0.f, rjz = 0.f;
-    for (int j0 = 0; j0 < n; j0 += QB) {
-        __syncthreads();
-        const int jl = j0 + threadIdx.x;
-        if (jl < n) { sx[threadIdx.x] = X[jl]; sy[threadIdx.x] = Y[jl]; sz[threadIdx.x] = Z[jl]; }
-        __syncthreads();
-        if (!act) continue;
-        const int cnt = n - j0 < QB ? n - j0 : QB;
-        for (int jj = 0; jj < cnt; jj++) {
-            if (j0 + jj == i) continue;
-            float r3[9], drx, dry, drz;
-            const float rr = cvd_pair(xi, yi, zi, sx[jj], sy[jj], sz[jj], fx, fy, fz, r3, drx, dry, drz);
-            if (!((double)rr <= rhi && (double)rr >= rlo)) continue;
-            Nb = Nb + 1;
-            if (Nb > (unsigned)Q6CN) continue;
+    nbo[i] = nb_walk<Q6Test::CAP>(i, S, cnt, lst, [&](int k, int j) {
+        float r3[9], drx, dry, drz;
+        const float rr = cvd_pair(xi, yi, zi, X[j], Y[j], Z[j], fx, fy, fz, r3, drx, dry, drz);
 #pragma unroll
-            for (int q = 0; q < 7; q += 3)
-                if ((double)(fabsf(r3[q]) - drx) < 1e-5) rjx = xi * fx - r3[q];
+        for (int q = 0; q < 7; q += 3)
+            if ((double)(fabsf(r3[q]) - drx) < 1e-5) rjx = xi * fx - r3[q];
 #pragma unroll
-            for (int q = 1; q < 8; q += 3)
-                if ((double)(fabsf(r3[q]) - dry) < 1e-5) rjy = yi * fy - r3[q];
+        for (int q = 1; q < 8; q += 3)
+            if ((double)(fabsf(r3[q]) - dry) < 1e-5) rjy = yi * fy - r3[q];
 #pragma unroll
-            for (int q = 2; q < 9; q += 3)
-                if ((double)(fabsf(r3[q]) - drz) < 1e-5) rjz = zi * fz - r3[q];
-            const size_t s = (size_t)i * Q6CN + (Nb - 1);
-            theta[s] = acosf((rjz - zi * fz) / rr);
-            const float ddy = rjy - yi * fy;
-            fai[s] = (ddy == 0.) ? 0.f : atanf(ddy / (rjx - xi * fx));
-        }
-    }
-    if (act) nbo[i] = Nb;
+        for (int q = 2; q < 9; q += 3)
+            if ((double)(fabsf(r3[q]) - drz) < 1e-5) rjz = zi * fz - r3[q];
+        const size_t s = (size_t)i * Q6CN + k;
+        theta[s] = acosf((rjz - zi * fz) / rr);
+        const float ddy = rjy - yi * fy;
+        fai[s] = (ddy == 0.) ? 0.f : atanf(ddy / (rjx - xi * fx));
+    });
 }
 
 // Per entry: qb[m][i] = sum_j Y_6m / Nb (0 without stored neighbours), local
@@ -913,7 +994,25 @@ __global__ void __launch_bounds__(64) k_q6_global(int n, const unsigned* __restr
     if (t < QM) {
         float qr = 0.f, qc = 0.f, sum_nb = 0.f;
         const double2* q = qb + (size_t)t * n;
-        for (int i = 0; i < n; i++) {
+        // Loads of a run of entries are issued before the in-order adds.
+        constexpr int U = 16;
+        int i = 0;
+        for (; i + U <= n; i += U) {
+            unsigned nb[U];
+            double2 v[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                nb[k] = nbo[i + k];
+                v[k] = q[i + k];
+            }
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                qr = (float)((double)qr + (double)nb[k] * v[k].x);
+                qc = (float)((double)qc + (double)nb[k] * v[k].y);
+                sum_nb = sum_nb + (float)nb[k];
+            }
+        }
+        for (; i < n; i++) {
             const unsigned nb = nbo[i];
             const double2 v = q[i];
             qr = (float)((double)qr + (double)nb * v.x);
@@ -983,6 +1082,9 @@ hipError_t post_cvd_q6(int L, int M, int N, float dx, int n, const float* x, con
     float* dql = sc.get<float>(n, e);
     float* dg = sc.get<float>(1, e);
     unsigned long long* dhq = sc.get<unsigned long long>(Q6NBIN, e);
+    const NbSlices ns = nb_slices(n);
+    int* cnt = sc.get<int>((size_t)n * ns.S, e);
+    int* lst = sc.get<int>((size_t)n * ns.S * Q6Test::CAP, e);
     if (e != hipSuccess) return e;
     if ((e = hipMemcpy(dx_, x, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(dy_, y, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
@@ -990,8 +1092,10 @@ hipError_t post_cvd_q6(int L, int M, int N, float dx, int n, const float* x, con
     if ((e = hipMemset(dhq, 0, sizeof(unsigned long long) * Q6NBIN)) != hipSuccess) return e;
     const float fx = L * dx, fy = M * dx, fz = N * dx;
     const double rhi = 1.0f * sqrt(3.) / 2. * 1.2 + 1e-3, rlo = 1.0f * sqrt(3.) / 2. * 0.9 - 1e-3;
-    hipLaunchKernelGGL(k_q6_nb, dim3((n + QB - 1) / QB), dim3(QB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, rlo, rhi, dn,
-                       dth, dfa);
+    const Q6Test t{fx, fy, fz, rlo, rhi};
+    const int gi = (n + SB - 1) / SB;
+    hipLaunchKernelGGL(k_nb_scan<Q6Test>, dim3(gi, ns.S), dim3(SB), 0, 0, t, n, dx_, dy_, dz_, ns.slice, cnt, lst);
+    hipLaunchKernelGGL(k_q6_fin, dim3(gi), dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, ns.S, cnt, lst, dn, dth, dfa);
     hipLaunchKernelGGL(k_q6_local, dim3((n + SB - 1) / SB), dim3(SB), 0, 0, n, dn, dth, dfa, dq, dql, dhq);
     hipLaunchKernelGGL(k_q6_global, dim3(1), dim3(64), 0, 0, n, dn, dq, dg);
     if ((e = hipGetLastError()) != hipSuccess) return e;
