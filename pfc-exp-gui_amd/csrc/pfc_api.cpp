@@ -470,9 +470,17 @@ struct TimedLaunch {
 
 // One step on all active instances: row pass, then column pass; on 3-D
 // grids the column pass sits between the two depth passes.
-static hipError_t enqueue_step(pfc_ctx* c, int row_flags, int col_flags, int ecol_slot) {
+// etrace (energy-traced steps): where the column pass writes the energy pair
+// of the state before this step, from this step's row partials and the other
+// ecol slot (the previous column pass).
+static hipError_t enqueue_step(pfc_ctx* c, int row_flags, int col_flags, int ecol_slot, double* etrace = nullptr) {
     DevBufs b = c->bufs;
     b.ecol = c->ecol[ecol_slot];
+    if (etrace) {
+        b.eprev = c->ecol[ecol_slot ^ 1];
+        b.etrace = etrace;
+        b.nrow = c->nrow;
+    }
     hipError_t e = hipSuccess;
     if (c->noise) {
         e = launch_noise(c->prec, b, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr, c->stream);
@@ -737,14 +745,11 @@ extern "C" int pfc_step_energy(pfc_ctx* c, int nsteps, double* trace) {
     for (int i = 0; i < nsteps && e == hipSuccess; i++) {
         // Row pass of step i also sums the pointwise energy of the state it
         // reads (after step i-1); the column pass of step i-1 left that
-        // state's gradient sum in ecol[(i-1)&1].
+        // state's gradient sum in ecol[(i-1)&1], and the column pass of step
+        // i reduces both into the trace entry of step i-1.
         const int slot = i & 1;
-        e = enqueue_step(c, ROW_WRITE_IN | (i > 0 ? ROW_ENERGY : 0), COL_ENERGY, slot);
-        if (e == hipSuccess && i > 0) {
-            DevBufs b = c->bufs;
-            b.ecol = c->ecol[(i - 1) & 1];
-            e = launch_energy_reduce(b, c->nrow, c->ncol, (double)c->m.alpha, dtr + 2 * c->B * (size_t)(i - 1), c->stream);
-        }
+        e = enqueue_step(c, ROW_WRITE_IN | (i > 0 ? ROW_ENERGY : 0), COL_ENERGY, slot,
+                         i > 0 ? dtr + 2 * c->B * (size_t)(i - 1) : nullptr);
     }
     if (e == hipSuccess) {
         DevBufs b = c->bufs;

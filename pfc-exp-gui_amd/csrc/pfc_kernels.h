@@ -60,6 +60,13 @@ struct DevBufs {
     const void* cC;
     int col_ts;  // plain-step column tile width chosen by the autotuner (0 default, 1 wider, 2 narrower)
     int L, M, N, K, B;
+    // Energy-traced stepping: with etrace set, one workgroup per instance of
+    // the (energy) column pass first reduces the previous state's partial
+    // sums (erow of this step's row pass, eprev of the previous column pass,
+    // which had the same grid) into etrace[inst*2 + {0,1}].
+    const double* eprev = nullptr;
+    double* etrace = nullptr;
+    int nrow = 0;  // row-pass workgroups per instance (erow entries)
 };
 
 enum { ROW_WRITE_IN = 1, ROW_ENERGY = 2 };

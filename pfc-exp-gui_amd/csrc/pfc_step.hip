@@ -54,16 +54,16 @@ __device__ __forceinline__ void bstore(const T& val, const Buf& b, uint32_t voff
 // ---------------------------------------------------------------------------
 
 // Nonlinear terms at one point (misfit.cpp:777, 792, 826, 840, 858-867).
-// Returns inn, inc and in3.
+// Returns inn, inc, in3 and the mixing term in2 = w (c ln(c/c0) + (1-c) ln((1-c)/(1-c0))).
 __device__ __forceinline__ void nonlinear(const PhysArgs& p, double ph, double c, double a, double b,
-                                          double& inn, double& inc, double& in3) {
+                                          double& inn, double& inc, double& in3, double& in2) {
 #pragma clang fp contract(off)
     const double c0 = (double)p.c0;
     const double lc = log(c / c0);
     const double lc1 = log((1.0 - c) / (1.0 - c0));
     const double ph2 = ph * ph;
     const double in1 = -(double)p.niu * ph2 / 2.0 + (double)p.ksi * (ph2 * ph) / 3.0;
-    const double in2 = (double)p.w * (c * lc + (1.0 - c) * lc1);
+    in2 = (double)p.w * (c * lc + (1.0 - c) * lc1);
     const double cc = c * c;
     const double cm = 1.0 - c;
     const double h1 = 1.0 - 3.0 * cc + 2.0 * (cc * c);
@@ -84,13 +84,13 @@ __device__ __forceinline__ void nonlinear(const PhysArgs& p, double ph, double c
 // FP32: log(c/c0) = ln2 * log2(c) - ln(c0) with the hardware log2
 // (v_log_f32, about 1 ulp) and ln(c0), ln(1 - c0) precomputed on the host.
 __device__ __forceinline__ void nonlinear(const PhysArgs& p, float ph, float c, float a, float b, float& inn,
-                                          float& inc, float& in3) {
+                                          float& inc, float& in3, float& in2) {
     constexpr float LN2 = 0.693147180559945309f;
     const float lc = __builtin_fmaf(LN2, __builtin_amdgcn_logf(c), -p.lnc0);
     const float lc1 = __builtin_fmaf(LN2, __builtin_amdgcn_logf(1.0f - c), -p.ln1c0);
     const float ph2 = ph * ph;
     const float in1 = -p.niu * ph2 * 0.5f + p.ksi * (ph2 * ph) * (1.0f / 3.0f);
-    const float in2 = p.w * (c * lc + (1.0f - c) * lc1);
+    in2 = p.w * (c * lc + (1.0f - c) * lc1);
     const float cc = c * c;
     const float cm = 1.0f - c;
     const float h1 = 1.0f - 3.0f * cc + 2.0f * (cc * c);
@@ -103,8 +103,8 @@ __device__ __forceinline__ void nonlinear(const PhysArgs& p, float ph, float c, 
 
 // Per-point energy without the gradient term (ENERGYC2, misfit.cpp:1054-1058);
 // the alpha*grac term is summed in k space by the column pass (Parseval).
-template <typename R>
-__device__ __forceinline__ double energy_point(const PhysArgs& p, R phr, R cr, R in3r) {
+// FP64 (and the reference's float narrowing of the three terms).
+__device__ __forceinline__ double energy_point(const PhysArgs& p, double phr, double cr, double in3r, double) {
 #pragma clang fp contract(off)
     const double ph = (double)phr, c = (double)cr, in3 = (double)in3r;
     const double c0 = (double)p.c0;
@@ -114,6 +114,17 @@ __device__ __forceinline__ double energy_point(const PhysArgs& p, R phr, R cr, R
     const double eex = 0.5 * ph * in3;
     if (p.narrow) return (double)((float)eid + (float)emix + (float)eex);
     return eid + emix + eex;
+}
+
+// FP32 path: the three terms in float, the mixing term as (ph + 1) * in2
+// with in2 = w (c ln(c/c0) + (1-c) ln((1-c)/(1-c0))) from the nonlinear terms
+// (the same logarithms); the caller sums in FP64.
+__device__ __forceinline__ double energy_point(const PhysArgs& p, float ph, float, float in3, float in2) {
+    const float ph2 = ph * ph;
+    const float eid = ph2 * 0.5f - p.niu * (ph2 * ph) * (1.0f / 6.0f) + p.ksi * (ph2 * ph2) * (1.0f / 12.0f);
+    const float emix = (ph + 1.0f) * in2;
+    const float eex = 0.5f * ph * in3;
+    return (double)(eid + emix + eex);
 }
 
 // Noise term of MAINEQUATIONC2_NOI for one mode (misfit.cpp:1002-1003):
@@ -474,11 +485,11 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
         const R co[2] = {mco + z[q].c[1].x * invM, mco + z[q].c[1].y * invM};
         const R aa[2] = {ma + z[q].c[2].x * invM, ma + z[q].c[2].y * invM};
         const R bb[2] = {mb + z[q].c[3].x * invM, mb + z[q].c[3].y * invM};
-        R inn[2], inc[2], in3[2];
+        R inn[2], inc[2], in3[2], in2[2];
 #pragma unroll
         for (int s2 = 0; s2 < 2; s2++) {
-            nonlinear(p, ph[s2], co[s2], aa[s2], bb[s2], inn[s2], inc[s2], in3[s2]);
-            if (want_e && actB) esum += energy_point<R>(p, ph[s2], co[s2], in3[s2]);
+            nonlinear(p, ph[s2], co[s2], aa[s2], bb[s2], inn[s2], inc[s2], in3[s2], in2[s2]);
+            if (want_e && actB) esum += energy_point(p, ph[s2], co[s2], in3[s2], in2[s2]);
         }
         w[q].c[0] = Cx<R>{inn[0], inn[1]};
         w[q].c[1] = Cx<R>{inc[0], inc[1]};
@@ -722,6 +733,32 @@ __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs&
     }
 }
 
+// The energy pair of one instance from its partial sums (ENERGYC2,
+// misfit.cpp:1043-1092): the pointwise terms plus alpha/P times the k-space
+// gradient sum, both over the grid's points; the density is the FP64 k = 0
+// mode, which no step changes.
+__device__ __forceinline__ void energy_pair(const DevBufs& b, double alpha, int inst, double se, double sg,
+                                            double* e_out) {
+    const double P = (double)b.L * (double)b.M * (double)b.N;
+    const double tot = se + alpha * sg / P;
+    e_out[2 * inst] = tot / b.L / b.M / b.N;
+    e_out[2 * inst + 1] = b.means[4 * inst];
+}
+
+// One workgroup of NT threads reduces instance inst's partial sums into
+// b.etrace (energy-traced stepping; ncol = the column grid's width).
+template <int NT>
+__device__ void energy_reduce_block(const DevBufs& b, const PhysArgs& p, int inst, int ncol, double* red) {
+    const double* er = b.erow + (size_t)inst * b.nrow;
+    const double* ec = b.eprev + (size_t)inst * ncol;
+    double se = 0.0, sg = 0.0;
+    for (int i = threadIdx.x; i < b.nrow; i += NT) se += er[i];
+    for (int i = threadIdx.x; i < ncol; i += NT) sg += ec[i];
+    se = block_sum<NT>(se, red);
+    sg = block_sum<NT>(sg, red);
+    if (threadIdx.x == 0) energy_pair(b, (double)p.alpha, inst, se, sg, b.etrace);
+}
+
 // LEAN: the plain step (mode STEP, no noise, no energy) with the other paths
 // compiled out, which frees registers for occupancy.
 template <typename R, int LL, bool LEAN, int TS>
@@ -735,6 +772,9 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     const int tid = threadIdx.x;
     const int col = C::WAVE ? tid / G : tid % T, t = C::WAVE ? tid % G : tid / T;
     const int inst = blockIdx.y;
+    if constexpr (!LEAN) {
+        if (b.etrace && blockIdx.x == gridDim.x - 1) energy_reduce_block<NT>(b, p, inst, gridDim.x, red);
+    }
     if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
     const int K = b.K;
     const size_t CS = (size_t)b.N * K;       // stride between consecutive l
@@ -1047,13 +1087,10 @@ __global__ __launch_bounds__(ENERGY_NT) void k_energy_reduce(DevBufs b, int nrow
     if (t != 0) return;
     se = (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]);
     sg = (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]);
-    const double P = (double)b.L * (double)b.M * (double)b.N;
-    const double tot = se + alpha * sg / P;
-    e_out[2 * inst] = tot / b.L / b.M / b.N;
     // Density: the k=0 mode, held in FP64 and left unchanged by every step
     // (k = 0 makes the update an identity, misfit.cpp:1002-1010); the sum of
     // the reconstructed deviations is zero up to transform round-off.
-    e_out[2 * inst + 1] = b.means[4 * inst];
+    energy_pair(b, alpha, inst, se, sg, e_out);
 }
 
 // ---------------------------------------------------------------------------
