@@ -284,23 +284,93 @@ __global__ void k_max(const void* phi, int prec, long P, double* out) {
     if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-// Device scratch for one compaction; freed at the end of the call.
-struct Scratch {
-    std::vector<void*> p;
-    ~Scratch() { for (void* q : p) hipFree(q); }
-    template <typename T>
-    T* get(size_t n, hipError_t& e) {
-        void* q = nullptr;
-        if (e == hipSuccess) e = hipMalloc(&q, n * sizeof(T) + 16);
-        if (e == hipSuccess) p.push_back(q);
-        return (T*)q;
+// Device scratch of the post-processing calls, kept per host thread: a call
+// takes blocks from its thread's pool (hipMalloc when none fits) and gives
+// them back when it ends, after the stream it used has finished with them.
+// hipFree waits for the whole device, which would tie a checkpoint's
+// post-processing on one thread to the stepping another thread has queued.
+// The pool keeps at most 1 GiB; blocks beyond go back with hipFree.
+struct PoolBlock {
+    void* p;
+    size_t n;
+};
+struct ThreadPool {
+    std::vector<PoolBlock> blocks;
+    size_t bytes = 0;
+    ~ThreadPool() {
+        for (const PoolBlock& b : blocks) hipFree(b.p);
     }
 };
+static thread_local ThreadPool t_pool;
+
+struct Scratch {
+    hipStream_t s;
+    std::vector<PoolBlock> p;
+    explicit Scratch(hipStream_t st = nullptr) : s(st) {}
+    ~Scratch() {
+        if (p.empty()) return;
+        hipStreamSynchronize(s);
+        for (const PoolBlock& b : p) {
+            if (t_pool.bytes + b.n > (size_t(1) << 30)) {
+                hipFree(b.p);
+                continue;
+            }
+            t_pool.blocks.push_back(b);
+            t_pool.bytes += b.n;
+        }
+    }
+    template <typename T>
+    T* get(size_t n, hipError_t& e) {
+        if (e != hipSuccess) return nullptr;
+        const size_t want = n * sizeof(T) + 16;
+        // Smallest pooled block that fits.
+        int best = -1;
+        for (int i = 0; i < (int)t_pool.blocks.size(); i++)
+            if (t_pool.blocks[i].n >= want && (best < 0 || t_pool.blocks[i].n < t_pool.blocks[best].n)) best = i;
+        PoolBlock b{nullptr, want};
+        if (best >= 0) {
+            b = t_pool.blocks[best];
+            t_pool.blocks.erase(t_pool.blocks.begin() + best);
+            t_pool.bytes -= b.n;
+        } else if ((e = hipMalloc(&b.p, want)) != hipSuccess) {
+            return nullptr;
+        }
+        p.push_back(b);
+        return (T*)b.p;
+    }
+};
+
+// The calling thread's stream for post-processing of host arrays (created
+// on first use, non-blocking): it runs beside a context's stepping queued
+// by another thread.
+struct ThreadStream {
+    hipStream_t s = nullptr;
+    ~ThreadStream() {
+        if (s) hipStreamDestroy(s);
+    }
+};
+static thread_local ThreadStream t_stream;
+static hipError_t post_stream(hipStream_t* out) {
+    if (!t_stream.s) {
+        hipError_t e = hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            t_stream.s = nullptr;
+            return e;
+        }
+    }
+    *out = t_stream.s;
+    return hipSuccess;
+}
+
+template <typename T>
+static hipError_t h2d(T* dst, const T* src, size_t n, hipStream_t s) {
+    return hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, s);
+}
 
 template <typename Op>
 static hipError_t compact(Op op, long P, int cap, int* n_out, hipStream_t s) {
     hipError_t e = hipSuccess;
-    Scratch sc;
+    Scratch sc(s);
     const int nb = (int)((P + PB - 1) / PB);
     int* counts = sc.get<int>(nb, e);
     int* offs = sc.get<int>(nb + 1, e);
@@ -328,7 +398,7 @@ static hipError_t misfit_t(int L, int M, float dx, float con0, const void* phi, 
                            double* y, double* z, double* ph, float* co, int cap, PostResult* r, hipStream_t s) {
     const long P = (long)L * M;
     hipError_t e = hipSuccess;
-    Scratch sc;
+    Scratch sc(s);
     MisfitOp<R> op;
     op.phi = (const R*)phi; op.con = (const R*)con; op.L = L; op.M = M; op.dx = dx; op.con0 = con0;
     op.x = sc.get<double>(P, e); op.y = sc.get<double>(P, e); op.z = sc.get<double>(P, e);
@@ -381,7 +451,7 @@ hipError_t post_phimax_misfit(int prec, int L, int M, float dx, float con0, cons
 
 static hipError_t field_max(const void* phi, int prec, long P, double* out, hipStream_t s) {
     hipError_t e = hipSuccess;
-    Scratch sc;
+    Scratch sc(s);
     const int nb = 256;
     double* part = sc.get<double>(nb, e);
     if (e != hipSuccess) return e;
@@ -408,7 +478,7 @@ static hipError_t cvd_t(int L, int M, int N, float dx, float u0, const void* phi
     const float phic = u0 + (pmax - u0) * 0.618f;
     r->phic = phic;
     r->phimax = pmax;
-    Scratch sc;
+    Scratch sc(s);
     CvdOp<R> op;
     op.phi = (const R*)phi; op.L = L; op.M = M; op.N = N; op.dx = dx; op.phic = phic;
     op.fx = L * dx; op.fy = M * dx; op.fz = N * dx;
@@ -434,12 +504,13 @@ hipError_t post_phimax_cvd(int prec, int L, int M, int N, float dx, float u0, co
 hipError_t post_phimax_elastic(int L, int M, int N, float dx, const float* phi, double* x, double* y, double* z,
                                double* ph, int cap, PostResult* r) {
     const long P = (long)L * M * N;
-    hipError_t e = hipSuccess;
-    Scratch sc;
+    hipStream_t s = nullptr;
+    hipError_t e = post_stream(&s);
+    if (e != hipSuccess) return e;
+    Scratch sc(s);
     float* dphi = sc.get<float>(P, e);
     if (e != hipSuccess) return e;
-    hipStream_t s = nullptr;
-    if ((e = hipMemcpy(dphi, phi, sizeof(float) * P, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = h2d(dphi, phi, P, s)) != hipSuccess) return e;
     // elastic.cpp:2809-2816: float running max from 0.
     float pmax = 0.0f;
     for (long i = 0; i < P; i++)
@@ -660,10 +731,12 @@ __global__ void __launch_bounds__(SB) k_strain_fin(int n, const double* __restri
 
 hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
                        const double* z, double* strain, unsigned* nb) {
-    hipError_t e = hipSuccess;
-    if (n <= 0) return e;
+    if (n <= 0) return hipSuccess;
+    hipStream_t s = nullptr;
+    hipError_t e = post_stream(&s);
+    if (e != hipSuccess) return e;
     const NbSlices ns = nb_slices(n);
-    Scratch sc;
+    Scratch sc(s);
     double* dx_ = sc.get<double>(n, e);
     double* dy_ = sc.get<double>(n, e);
     double* dz_ = sc.get<double>(n, e);
@@ -672,17 +745,18 @@ hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, cons
     int* cnt = sc.get<int>((size_t)n * ns.S, e);
     int* lst = sc.get<int>((size_t)n * ns.S * StrainTest::CAP, e);
     if (e != hipSuccess) return e;
-    if ((e = hipMemcpy(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(dz_, z, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = h2d(dx_, x, n, s)) != hipSuccess) return e;
+    if ((e = h2d(dy_, y, n, s)) != hipSuccess) return e;
+    if ((e = h2d(dz_, z, n, s)) != hipSuccess) return e;
     const float fx = L * dx, fy = M * dx, fz = N * dx;
     const StrainTest t{fx, fy, fz, benchel};
     const int gi = (n + SB - 1) / SB;
-    hipLaunchKernelGGL(k_nb_scan<StrainTest>, dim3(gi, ns.S), dim3(SB), 0, 0, t, n, dx_, dy_, dz_, ns.slice, cnt, lst);
-    hipLaunchKernelGGL(k_strain_fin, dim3(gi), dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, ns.S, cnt, lst, ds, dn);
+    hipLaunchKernelGGL(k_nb_scan<StrainTest>, dim3(gi, ns.S), dim3(SB), 0, s, t, n, dx_, dy_, dz_, ns.slice, cnt, lst);
+    hipLaunchKernelGGL(k_strain_fin, dim3(gi), dim3(SB), 0, s, n, dx_, dy_, dz_, fx, fy, fz, ns.S, cnt, lst, ds, dn);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = hipMemcpy(strain, ds, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
-    return hipMemcpy(nb, dn, sizeof(unsigned) * n, hipMemcpyDeviceToHost);
+    if ((e = d2h(strain, ds, n, s)) != hipSuccess) return e;
+    if ((e = d2h(nb, dn, n, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -832,8 +906,10 @@ __global__ void __launch_bounds__(SB) k_cryst_q(int n, const unsigned* __restric
 
 hipError_t post_cryst(int L, int M, int N, float dx, int n, const double* x, const double* y, const double* z,
                       double* q4, double* q6, unsigned* nb) {
-    hipError_t e = hipSuccess;
-    Scratch sc;
+    hipStream_t s = nullptr;
+    hipError_t e = post_stream(&s);
+    if (e != hipSuccess) return e;
+    Scratch sc(s);
     double* dx_ = sc.get<double>(n, e);
     double* dy_ = sc.get<double>(n, e);
     double* dz_ = sc.get<double>(n, e);
@@ -848,21 +924,21 @@ hipError_t post_cryst(int L, int M, int N, float dx, int n, const double* x, con
     int* cnt = sc.get<int>((size_t)n * ns.S, e);
     int* lst = sc.get<int>((size_t)n * ns.S * CrystTest::CAP, e);
     if (e != hipSuccess) return e;
-    if ((e = hipMemcpy(dx_, x, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(dy_, y, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(dz_, z, sizeof(double) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = h2d(dx_, x, n, s)) != hipSuccess) return e;
+    if ((e = h2d(dy_, y, n, s)) != hipSuccess) return e;
+    if ((e = h2d(dz_, z, n, s)) != hipSuccess) return e;
     const float fx = L * dx, fy = M * dx, fz = N * dx;
     const dim3 g((n + SB - 1) / SB);
     const CrystTest t{fx, fy, fz};
-    hipLaunchKernelGGL(k_nb_scan<CrystTest>, dim3(g.x, ns.S), dim3(SB), 0, 0, t, n, dx_, dy_, dz_, ns.slice, cnt, lst);
-    hipLaunchKernelGGL(k_cryst_fin, g, dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, ns.S, cnt, lst, dn, di, dth, dfa);
-    hipLaunchKernelGGL(k_cryst_qlm, g, dim3(SB), 0, 0, n, dn, dth, dfa, dq);
-    hipLaunchKernelGGL(k_cryst_q, g, dim3(SB), 0, 0, n, dn, di, dq, d4, d6);
+    hipLaunchKernelGGL(k_nb_scan<CrystTest>, dim3(g.x, ns.S), dim3(SB), 0, s, t, n, dx_, dy_, dz_, ns.slice, cnt, lst);
+    hipLaunchKernelGGL(k_cryst_fin, g, dim3(SB), 0, s, n, dx_, dy_, dz_, fx, fy, fz, ns.S, cnt, lst, dn, di, dth, dfa);
+    hipLaunchKernelGGL(k_cryst_qlm, g, dim3(SB), 0, s, n, dn, dth, dfa, dq);
+    hipLaunchKernelGGL(k_cryst_q, g, dim3(SB), 0, s, n, dn, di, dq, d4, d6);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = hipMemcpy(q4, d4, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
-    if ((e = hipMemcpy(q6, d6, sizeof(double) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
-    if (nb) e = hipMemcpy(nb, dn, sizeof(unsigned) * n, hipMemcpyDeviceToHost);
-    return e;
+    if ((e = d2h(q4, d4, n, s)) != hipSuccess) return e;
+    if ((e = d2h(q6, d6, n, s)) != hipSuccess) return e;
+    if ((e = d2h(nb, dn, n, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1039,39 +1115,44 @@ int cvd_gr_bins(int L, float dx) {
 
 hipError_t post_cvd_gr(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
                        unsigned* hist) {
-    hipError_t e = hipSuccess;
     const int maxbin = cvd_gr_bins(L, dx);
     if (maxbin <= 0) return hipSuccess;
-    Scratch sc;
+    hipStream_t s = nullptr;
+    hipError_t e = post_stream(&s);
+    if (e != hipSuccess) return e;
+    Scratch sc(s);
     float* dx_ = sc.get<float>(n, e);
     float* dy_ = sc.get<float>(n, e);
     float* dz_ = sc.get<float>(n, e);
     unsigned* dh = sc.get<unsigned>(maxbin, e);
     if (e != hipSuccess) return e;
-    if ((e = hipMemcpy(dx_, x, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(dy_, y, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(dz_, z, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemset(dh, 0, sizeof(unsigned) * maxbin)) != hipSuccess) return e;
+    if ((e = h2d(dx_, x, n, s)) != hipSuccess) return e;
+    if ((e = h2d(dy_, y, n, s)) != hipSuccess) return e;
+    if ((e = h2d(dz_, z, n, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(dh, 0, sizeof(unsigned) * maxbin, s)) != hipSuccess) return e;
     const float fx = L * dx, fy = M * dx, fz = N * dx;
     const float delr = (float)(1.0f / 25.);
     const double rmin = 1.0f / sqrt(2.) * 0.5;
     if (n > 1) {
         const dim3 g((n + SB - 1) / SB, (n + GRJT * SB - 1) / (GRJT * SB));
         if (maxbin <= GR_LDS_BINS)
-            hipLaunchKernelGGL(k_cvd_gr<true>, g, dim3(SB), sizeof(unsigned) * maxbin, 0, n, dx_, dy_, dz_, fx, fy,
+            hipLaunchKernelGGL(k_cvd_gr<true>, g, dim3(SB), sizeof(unsigned) * maxbin, s, n, dx_, dy_, dz_, fx, fy,
                                fz, delr, rmin, maxbin, dh);
         else
-            hipLaunchKernelGGL(k_cvd_gr<false>, g, dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, delr, rmin, maxbin,
+            hipLaunchKernelGGL(k_cvd_gr<false>, g, dim3(SB), 0, s, n, dx_, dy_, dz_, fx, fy, fz, delr, rmin, maxbin,
                                dh);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    return hipMemcpy(hist, dh, sizeof(unsigned) * maxbin, hipMemcpyDeviceToHost);
+    if ((e = d2h(hist, dh, maxbin, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
 }
 
 hipError_t post_cvd_q6(int L, int M, int N, float dx, int n, const float* x, const float* y, const float* z,
                        unsigned* nb, float* ql, float* qglobal, unsigned long long* histq) {
-    hipError_t e = hipSuccess;
-    Scratch sc;
+    hipStream_t s = nullptr;
+    hipError_t e = post_stream(&s);
+    if (e != hipSuccess) return e;
+    Scratch sc(s);
     float* dx_ = sc.get<float>(n, e);
     float* dy_ = sc.get<float>(n, e);
     float* dz_ = sc.get<float>(n, e);
@@ -1086,23 +1167,24 @@ hipError_t post_cvd_q6(int L, int M, int N, float dx, int n, const float* x, con
     int* cnt = sc.get<int>((size_t)n * ns.S, e);
     int* lst = sc.get<int>((size_t)n * ns.S * Q6Test::CAP, e);
     if (e != hipSuccess) return e;
-    if ((e = hipMemcpy(dx_, x, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(dy_, y, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(dz_, z, sizeof(float) * n, hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemset(dhq, 0, sizeof(unsigned long long) * Q6NBIN)) != hipSuccess) return e;
+    if ((e = h2d(dx_, x, n, s)) != hipSuccess) return e;
+    if ((e = h2d(dy_, y, n, s)) != hipSuccess) return e;
+    if ((e = h2d(dz_, z, n, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(dhq, 0, sizeof(unsigned long long) * Q6NBIN, s)) != hipSuccess) return e;
     const float fx = L * dx, fy = M * dx, fz = N * dx;
     const double rhi = 1.0f * sqrt(3.) / 2. * 1.2 + 1e-3, rlo = 1.0f * sqrt(3.) / 2. * 0.9 - 1e-3;
     const Q6Test t{fx, fy, fz, rlo, rhi};
     const int gi = (n + SB - 1) / SB;
-    hipLaunchKernelGGL(k_nb_scan<Q6Test>, dim3(gi, ns.S), dim3(SB), 0, 0, t, n, dx_, dy_, dz_, ns.slice, cnt, lst);
-    hipLaunchKernelGGL(k_q6_fin, dim3(gi), dim3(SB), 0, 0, n, dx_, dy_, dz_, fx, fy, fz, ns.S, cnt, lst, dn, dth, dfa);
-    hipLaunchKernelGGL(k_q6_local, dim3((n + SB - 1) / SB), dim3(SB), 0, 0, n, dn, dth, dfa, dq, dql, dhq);
-    hipLaunchKernelGGL(k_q6_global, dim3(1), dim3(64), 0, 0, n, dn, dq, dg);
+    hipLaunchKernelGGL(k_nb_scan<Q6Test>, dim3(gi, ns.S), dim3(SB), 0, s, t, n, dx_, dy_, dz_, ns.slice, cnt, lst);
+    hipLaunchKernelGGL(k_q6_fin, dim3(gi), dim3(SB), 0, s, n, dx_, dy_, dz_, fx, fy, fz, ns.S, cnt, lst, dn, dth, dfa);
+    hipLaunchKernelGGL(k_q6_local, dim3((n + SB - 1) / SB), dim3(SB), 0, s, n, dn, dth, dfa, dq, dql, dhq);
+    hipLaunchKernelGGL(k_q6_global, dim3(1), dim3(64), 0, s, n, dn, dq, dg);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = hipMemcpy(nb, dn, sizeof(unsigned) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
-    if ((e = hipMemcpy(ql, dql, sizeof(float) * n, hipMemcpyDeviceToHost)) != hipSuccess) return e;
-    if ((e = hipMemcpy(qglobal, dg, sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess) return e;
-    return hipMemcpy(histq, dhq, sizeof(unsigned long long) * Q6NBIN, hipMemcpyDeviceToHost);
+    if ((e = d2h(nb, dn, n, s)) != hipSuccess) return e;
+    if ((e = d2h(ql, dql, n, s)) != hipSuccess) return e;
+    if ((e = d2h(qglobal, dg, 1, s)) != hipSuccess) return e;
+    if ((e = d2h(histq, dhq, Q6NBIN, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
 }
 
 // ---------------------------------------------------------------------------

@@ -6,6 +6,7 @@
 // libpfc_hip.so.  Additions (all optional): --L/--M/--N (runtime grid, the
 // reference's compile-time PFC_GRID_*), --precision fp64|fp32, --narrow 0|1.
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cmath>
 #include <climits>
@@ -326,7 +327,9 @@ int run_sim(Kind k, int argc, char** argv) {
         write_field_files(0, L, M, N, fa.data(), fb.data());
     }
     // Checkpoint files are formatted and written by `out` while the GPU
-    // steps on; each task owns copies of what it writes.
+    // steps on; each task owns copies of what it writes (gr_fail, declared
+    // first, outlives the writer's tasks).
+    std::atomic<bool> gr_fail{false};
     pfcio::Writer out;
     std::vector<double> ax(P), ay(P), az(P), aphi(P);
     std::vector<float> acon(P), px(P), py(P), pz(P), pp(P);
@@ -393,21 +396,28 @@ int run_sim(Kind k, int argc, char** argv) {
                 if (pfc_phimax_cvd(ctx, 0, px.data(), py.data(), pz.data(), pp.data(), static_cast<int>(P), &na, &phic,
                                    &pmax))
                     return gpu_fail("pfc_phimax_cvd");
-                CvdOrder ord;
-                if (ord.compute(last, o.p, na, px.data(), py.data(), pz.data())) return gpu_fail("pfc_cvd_gr/q6");
                 std::vector<float> fa, fb;
                 if (fetch_fields(ctx, P, fa, fb)) return gpu_fail("pfc_get_fields_f32");
                 std::vector<float> qx(px.begin(), px.begin() + na), qy(py.begin(), py.begin() + na),
                     qz(pz.begin(), pz.begin() + na), qp(pp.begin(), pp.begin() + na);
                 const pfc_params prm = o.p;
-                out.post([=, ord = std::move(ord), qx = std::move(qx), qy = std::move(qy), qz = std::move(qz),
-                          qp = std::move(qp), fa = std::move(fa), fb = std::move(fb)] {
+                // GR and Q6 work on the extracted atoms only: the writer
+                // thread runs them (on its own stream) beside the next steps.
+                out.post([=, &gr_fail, qx = std::move(qx), qy = std::move(qy), qz = std::move(qz), qp = std::move(qp),
+                          fa = std::move(fa), fb = std::move(fb)] {
                     pfcio::write_cvd_atoms(last, L, M, N, prm.dx, phic, pmax, na, qx.data(), qy.data(), qz.data(),
                                            qp.data());
+                    CvdOrder ord;
+                    if (ord.compute(last, prm, na, qx.data(), qy.data(), qz.data())) {
+                        gpu_fail("pfc_cvd_gr/q6");  // the message is this thread's
+                        gr_fail = true;
+                        return;
+                    }
                     ord.write(prm, na, qx.data(), "");
                     write_field_files(last, L, M, N, fa.data(), fb.data());
                     if (ck) { std::fprintf(ck, "step %d timestamp_ms %lld\n", last, pfcio::now_ms()); std::fflush(ck); }
                 });
+                if (gr_fail) break;
             }
             out.post([t, n, tr = std::move(tr)] {
                 FILE* ef = std::fopen("energy.txt", "a");
@@ -418,6 +428,8 @@ int run_sim(Kind k, int argc, char** argv) {
             });
             t = last + 1;
         }
+        out.drain();
+        if (gr_fail) return 1;
     }
     out.drain();
     if (ck) std::fclose(ck);
