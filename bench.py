@@ -577,7 +577,11 @@ def main():
         out["serving"] = serve_leg(hip, args.serve_seconds)
         print(f"serving done: {out['serving']['fp32_batched']}", file=sys.stderr, flush=True)
     if args.cvd_steps > 0:
-        out["cvd_checkpoints"] = cvd_checkpoint_leg(hip, args.cvd_steps, args.cvd_mod)
+        try:
+            out["cvd_checkpoints"] = cvd_checkpoint_leg(hip, args.cvd_steps, args.cvd_mod)
+        except (subprocess.SubprocessError, OSError, ValueError, IndexError, hip.PfcError) as e:
+            # the CLI leg is reported, not required for the headline line
+            out["cvd_checkpoints"] = {"error": f"{type(e).__name__}: {e}"}
         print(f"cvd checkpoints done: {out['cvd_checkpoints']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
