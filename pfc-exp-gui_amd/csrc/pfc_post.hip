@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "pfc_kernels.h"
@@ -289,10 +290,12 @@ __global__ void k_max(const void* phi, int prec, long P, double* out) {
 // them back when it ends, after the stream it used has finished with them.
 // hipFree waits for the whole device, which would tie a checkpoint's
 // post-processing on one thread to the stepping another thread has queued.
-// The pool keeps at most 1 GiB; blocks beyond go back with hipFree.
+// A block is reused for requests of half its size or more, on the device it
+// was allocated on; the pool keeps at most 1 GiB (blocks beyond are freed).
 struct PoolBlock {
     void* p;
     size_t n;
+    int dev;
 };
 struct ThreadPool {
     std::vector<PoolBlock> blocks;
@@ -323,11 +326,15 @@ struct Scratch {
     T* get(size_t n, hipError_t& e) {
         if (e != hipSuccess) return nullptr;
         const size_t want = n * sizeof(T) + 16;
-        // Smallest pooled block that fits.
+        int dev = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return nullptr;
+        // Smallest pooled block of this device that fits and is at most twice the request.
         int best = -1;
-        for (int i = 0; i < (int)t_pool.blocks.size(); i++)
-            if (t_pool.blocks[i].n >= want && (best < 0 || t_pool.blocks[i].n < t_pool.blocks[best].n)) best = i;
-        PoolBlock b{nullptr, want};
+        for (int i = 0; i < (int)t_pool.blocks.size(); i++) {
+            const PoolBlock& c = t_pool.blocks[i];
+            if (c.dev == dev && c.n >= want && c.n / 2 <= want && (best < 0 || c.n < t_pool.blocks[best].n)) best = i;
+        }
+        PoolBlock b{nullptr, want, dev};
         if (best >= 0) {
             b = t_pool.blocks[best];
             t_pool.blocks.erase(t_pool.blocks.begin() + best);
@@ -340,25 +347,29 @@ struct Scratch {
     }
 };
 
-// The calling thread's stream for post-processing of host arrays (created
-// on first use, non-blocking): it runs beside a context's stepping queued
-// by another thread.
-struct ThreadStream {
-    hipStream_t s = nullptr;
-    ~ThreadStream() {
-        if (s) hipStreamDestroy(s);
+// The calling thread's stream for post-processing of host arrays, one per
+// device (created on first use, non-blocking): it runs beside a context's
+// stepping queued by another thread.
+struct ThreadStreams {
+    std::vector<std::pair<int, hipStream_t>> s;
+    ~ThreadStreams() {
+        for (auto& d : s) hipStreamDestroy(d.second);
     }
 };
-static thread_local ThreadStream t_stream;
+static thread_local ThreadStreams t_streams;
 static hipError_t post_stream(hipStream_t* out) {
-    if (!t_stream.s) {
-        hipError_t e = hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking);
-        if (e != hipSuccess) {
-            t_stream.s = nullptr;
-            return e;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    for (auto& d : t_streams.s)
+        if (d.first == dev) {
+            *out = d.second;
+            return hipSuccess;
         }
-    }
-    *out = t_stream.s;
+    hipStream_t st = nullptr;
+    if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return e;
+    t_streams.s.emplace_back(dev, st);
+    *out = st;
     return hipSuccess;
 }
 
