@@ -89,7 +89,7 @@ size_t ctx_bytes(const pfc_params& p, int batch) {
         b += nreal * rs + nvec * 2 * rs + noise_scratch_bytes((long)P, batch);
     const size_t tables = ((size_t)(p.L / 2 + 1) * (p.M / 2 + 1) * (p.N / 2 + 1) * 2 + p.L + p.M + p.N) * 4 +
                           (size_t)(p.L + p.M + p.N + p.M / 2) * 2 * rs +
-                          (rs == 4 && batch >= 4 ? (size_t)p.L * p.N * (p.M / 2) * 24 : 0);
+                          (rs == 4 && p.L % 3 != 0 ? (size_t)p.L * p.N * (p.M / 2) * 24 : 0);
     return b + tables + (size_t)batch * 256 + (1u << 20);
 }
 }  // namespace pfc
@@ -217,12 +217,13 @@ static int tune_cols(pfc_ctx* c) {
         b.tcount = nullptr;
         // Rounds over the candidates in turn (so drift hits all of them), the
         // best round of each kept; after the first round the launch count is
-        // raised until a candidate's run is about 1 ms, so that small grids
-        // are not timed by their launch rate.  A candidate replaces the
-        // default only when it is at least 3% faster.
+        // raised until a candidate's run is about 0.3 ms.  A candidate
+        // replaces the default only when it is at least 3% faster, so grids
+        // small enough to be timed by their launch rate keep the default.
+        // About 3 ms per new (precision, grid, batch).
         float tmin[3] = {1e30f, 1e30f, 1e30f};
         int reps = 5;
-        for (int round = 0; round < 4 && e == hipSuccess; round++) {
+        for (int round = 0; round < 3 && e == hipSuccess; round++) {
             for (int k = 0; k < nc && e == hipSuccess; k++) {
                 b.col_ts = cand[k];
                 e = launch_col(c->prec, b, c->phys, COL_STEP, 0, c->stream);
@@ -235,8 +236,8 @@ static int tune_cols(pfc_ctx* c) {
                 if (e == hipSuccess && round > 0 && ms / reps < tmin[k]) tmin[k] = ms / reps;
                 if (e == hipSuccess && round == 0 && k == 0) {
                     const float per = ms / reps;  // the default's launch time
-                    reps = per > 0.f ? (int)(1.0f / per) + 1 : 200;
-                    reps = reps < 5 ? 5 : (reps > 200 ? 200 : reps);
+                    reps = per > 0.f ? (int)(0.3f / per) + 1 : 60;
+                    reps = reps < 5 ? 5 : (reps > 60 ? 60 : reps);
                 }
             }
         }
@@ -366,9 +367,12 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     b.active = c->active_d;
     b.L = c->L; b.M = c->M; b.N = c->N; b.K = c->K; b.B = batch;
     b.cA = b.cC = nullptr;
-    // The tables pay off when several instances share them (a single
-    // instance would read them from HBM beside its state).
-    if (prec == PREC_FP32 && batch >= 4) {
+    // FP32 power-of-two columns read the per-mode coefficients from tables
+    // (coalesced, 24 B per mode, shared by the batch) instead of gathering
+    // C2 and forming k^2 per mode: faster at every batch size measured,
+    // one instance included (512^2: 24.8 -> 19.4 us per step).  The
+    // 3*2^k kernels compute them inline.
+    if (prec == PREC_FP32 && c->L % 3 != 0) {
         const size_t nm = (size_t)c->L * c->N * c->K;
         e = hipMalloc(&c->coefA, nm * 16);
         if (e == hipSuccess) e = hipMalloc(&c->coefC, nm * 8);
