@@ -261,7 +261,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #define PFC_XCD_REMAP 1  // workgroup x takes tile (x % 8)*(nx/8) + x/8: one XCD's tiles are adjacent
 #endif
 #ifndef PFC_COEF
-#define PFC_COEF 1  // batched FP32 plain steps read per-mode coefficient tables (k_coef)
+#define PFC_COEF 1  // FP32 plain steps read per-mode coefficient tables (k_coef)
 #endif
 // Both passes
 #ifndef PFC_GEN
@@ -808,17 +808,22 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     const bool use_nz = !LEAN && step && b.NZ != nullptr;
 
     // Plain FP32 steps take the modes' coefficients from cA/cC (same layout
-    // as one instance of S, loaded with the state so their latency overlaps).
-    constexpr bool COEF = PFC_COEF && LEAN && sizeof(R) == 4 && LL % 3 != 0;
-    float4 cf[COEF ? E : 1];
-    float2 cc[COEF ? E : 1];
+    // as one instance of S, loaded with the state so their latency overlaps),
+    // in the mapping the per-mode work sees (after the forward transform):
+    // rows t + GB*q, q < EB, in threads t < GB.
+    constexpr bool COEF = PFC_COEF && LEAN && sizeof(R) == 4;
+    constexpr int EBc = C::EB, GBc = C::GB;
+    float4 cf[COEF ? EBc : 1];
+    float2 cc[COEF ? EBc : 1];
     if constexpr (COEF) {
-        const Buf CA = mkbuf((const float4*)b.cA + tile0, 0), CC = mkbuf((const float2*)b.cC + tile0, 0);
-        const uint32_t ca_off = (uint32_t)(col + t * CS) * 16u, cc_off = (uint32_t)(col + t * CS) * 8u;
+        if (GBc == G || t < GBc) {
+            const Buf CA = mkbuf((const float4*)b.cA + tile0, 0), CC = mkbuf((const float2*)b.cC + tile0, 0);
+            const uint32_t ca_off = (uint32_t)(col + t * CS) * 16u, cc_off = (uint32_t)(col + t * CS) * 8u;
 #pragma unroll
-        for (int q = 0; q < E; q++) {
-            cf[q] = bload<float4>(CA, ca_off, (uint32_t)(G * q * CS) * 16u);
-            cc[q] = bload<float2>(CC, cc_off, (uint32_t)(G * q * CS) * 8u);
+            for (int q = 0; q < EBc; q++) {
+                cf[q] = bload<float4>(CA, ca_off, (uint32_t)(GBc * q * CS) * 16u);
+                cc[q] = bload<float2>(CC, cc_off, (uint32_t)(GBc * q * CS) * 8u);
+            }
         }
     }
 
@@ -1173,8 +1178,7 @@ static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hip
 template <typename R>
 static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s, hipEvent_t t0,
                         hipEvent_t t1) {
-    const bool lean = mode == COL_STEP && !b.NZ && !(flags & COL_ENERGY) &&
-                      (sizeof(R) == 8 || b.L % 3 == 0 || b.cA != nullptr);
+    const bool lean = mode == COL_STEP && !b.NZ && !(flags & COL_ENERGY) && (sizeof(R) == 8 || b.cA != nullptr);
     // The plain step runs the tile width the autotuner chose; the other
     // modes (and a width whose tiles do not divide K) the default.
     int ts = lean ? b.col_ts : 0;

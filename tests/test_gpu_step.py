@@ -138,6 +138,27 @@ def test_batch_equals_single(hip):
         assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
 
 
+@pytest.mark.parametrize("model,L,M", [(0, 128, 64), (1, 384, 128), (0, 384, 384), (1, 1024, 64)])
+def test_table_and_inline_coefficients_bit_identical(hip, model, L, M):
+    """FP32 plain steps read each mode's update coefficients and C2 values
+    from per-context tables; energy-traced steps (the general column kernel)
+    form them inline.  Both must advance the state identically, or a CVD run
+    would depend on whether its energy is traced."""
+    p = hip.default_params(model=model, L=L, M=M, precision=hip.FP32, grainx=2, grainy=2)
+    phi, con = hip.init_fields(p)
+    a, b = hip.Sim(p, batch=2), hip.Sim(p, batch=2)
+    for s in (a, b):
+        s.set_fields(phi, con, inst=0)
+        s.set_fields(phi[::-1].copy(), con, inst=1)
+    a.step(12)
+    b.step_energy(12)
+    for i in range(2):
+        x, y = a.get_fields(i), b.get_fields(i)
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), (model, L, M, i)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-7), (1, 5e-4)])
 def test_misfit_noise_term(hip, oracle, prec, tol):
     """--axx > 0: the Marsaglia noise stream (misfit.cpp:951-981) regenerated
