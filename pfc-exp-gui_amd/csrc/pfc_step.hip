@@ -617,6 +617,21 @@ __device__ __forceinline__ Cx<R> hjoin(Cx<R> x0, Cx<R> xh) {
     return Cx<R>{x0.x - xh.y, x0.y + xh.x};
 }
 
+// Gradient-energy term of one (k_l = i, k_m = kmi, k_n = kni) mode of the
+// updated state: (kl'^2 + 2 km'^2) |con_hat|^2, own-direction Nyquist
+// removed (Re of the reference's i k con_hat transform).  A mode whose
+// conjugate is not stored (0 < km < M/2, or km in {0, M/2} with
+// 0 < kn < N/2) counts twice.
+template <typename R>
+__device__ __forceinline__ double mode_energy(const DevBufs& b, int LL, int i, int kmi, int kni, Cx<R> co) {
+    const int Mh = b.M / 2, Nh = b.N / 2;
+    const bool mself = kmi == 0 || kmi == Mh;
+    const double kmd = (kmi == Mh) ? 0.0 : (double)b.km[kmi];
+    const double mult = (mself && (kni == 0 || kni == Nh)) ? 1.0 : 2.0;
+    const double kld = (i == LL / 2) ? 0.0 : (double)b.kl[i];
+    return mult * (kld * kld + 2.0 * kmd * kmd) * ((double)co.x * co.x + (double)co.y * co.y);
+}
+
 // Per-mode work of the column pass on one (k_l = i, k_m = kmi, k_n = kni)
 // mode: update (mode STEP), DC removal, energy term, C2 products.
 template <typename R>
@@ -641,16 +656,7 @@ __device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, in
         ph = Cx<R>{R(0), R(0)};
         co = Cx<R>{R(0), R(0)};
     }
-    if (want_e) {
-        // (kl'^2 + 2 km'^2) |con_hat|^2: own-direction Nyquist removed (Re of
-        // the reference's i k con_hat transform).  A mode whose conjugate is
-        // not stored (0 < km < M/2, or km in {0, M/2} with 0 < kn < N/2)
-        // counts twice.
-        const double kmd = (kmi == Mh) ? 0.0 : (double)kmv;
-        const double mult = (mself && (kni == 0 || kni == Nh)) ? 1.0 : 2.0;
-        const double kld = (i == LL / 2) ? 0.0 : (double)b.kl[i];
-        g += mult * (kld * kld + 2.0 * kmd * kmd) * ((double)co.x * co.x + (double)co.y * co.y);
-    }
+    if (want_e) g += mode_energy(b, LL, i, kmi, kni, co);
     const int il = i <= LL / 2 ? i : LL - i;
     const int jn = kni <= Nh ? kni : b.N - kni;
     const size_t ci = 2 * (((size_t)il * (Mh + 1) + kmi) * (Nh + 1) + jn);
@@ -704,6 +710,7 @@ __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs&
                     ph = Cx<R>{R(0), R(0)};
                     co = Cx<R>{R(0), R(0)};
                 }
+                if (want_e) g += mode_energy(b, LL, i, kmi, kni, co);
                 a = Cx<R>{ph.x * cc[q].x, ph.y * cc[q].x};
                 bb = Cx<R>{ph.x * cc[q].y, ph.y * cc[q].y};
             }
@@ -759,12 +766,17 @@ __device__ void energy_reduce_block(const DevBufs& b, const PhysArgs& p, int ins
     if (threadIdx.x == 0) energy_pair(b, (double)p.alpha, inst, se, sg, b.etrace);
 }
 
-// LEAN: the plain step (mode STEP, no noise, no energy) with the other paths
-// compiled out, which frees registers for occupancy.
-template <typename R, int LL, bool LEAN, int TS>
+// KIND 0: every mode and flag (init, energy only, noise, energy).
+// KIND 1 (COL_LEAN): the plain step (mode STEP, no noise, no energy) with the
+// other paths compiled out, which frees registers for occupancy.
+// KIND 2 (COL_LEAN_E): the plain step with its energy partials (energy-traced
+// steps without noise).  Kinds 1 and 2 read FP32 coefficient tables.
+enum { COL_GENERAL = 0, COL_LEAN = 1, COL_LEAN_E = 2 };
+template <typename R, int LL, int KIND, int TS>
 __global__ void __launch_bounds__((ColCfg<R, LL, TS>::NT))
-__attribute__((amdgpu_waves_per_eu(LEAN ? ColCfg<R, LL, TS>::WPE_LEAN : ColCfg<R, LL, TS>::WPE)))
+__attribute__((amdgpu_waves_per_eu(KIND != COL_GENERAL ? ColCfg<R, LL, TS>::WPE_LEAN : ColCfg<R, LL, TS>::WPE)))
 k_col(DevBufs b, PhysArgs p, int mode, int flags) {
+    constexpr bool LEAN = KIND != COL_GENERAL;
     using C = ColCfg<R, LL, TS>;
     constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT, EA = C::EA;
     __shared__ __attribute__((aligned(16))) char smem[C::LDS];
@@ -772,7 +784,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     const int tid = threadIdx.x;
     const int col = C::WAVE ? tid / G : tid % T, t = C::WAVE ? tid % G : tid / T;
     const int inst = blockIdx.y;
-    if constexpr (!LEAN) {
+    if constexpr (KIND != COL_LEAN) {
         if (b.etrace && blockIdx.x == gridDim.x - 1) energy_reduce_block<NT>(b, p, inst, gridDim.x, red);
     }
     if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
@@ -857,7 +869,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
 #pragma unroll
         for (int q = 0; q < E; q++) nz[q] = n1[q].c[0];
     }
-    const bool want_e = !LEAN && (flags & COL_ENERGY) != 0;
+    const bool want_e = KIND == COL_LEAN_E || (KIND == COL_GENERAL && (flags & COL_ENERGY) != 0);
     double g = 0.0;
     col_tile_modes<R, LL, E, C::EB, C::GB, G, COEF>(b, p, mode, want_e, use_nz, x, nz, t, j, kmi, kni, packed_tile,
                                                     packed, smem, g, cf, cc);
@@ -1175,10 +1187,21 @@ static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hip
     return hipGetLastError();
 }
 
+// The energy-step table kernel exists for FP32 only.
+template <typename R, int LL>
+static void launch_col_e(dim3 grid, hipStream_t s, hipEvent_t t0, hipEvent_t t1, const DevBufs& b,
+                         const PhysArgs& p, int mode, int flags) {
+    if constexpr (sizeof(R) == 4)
+        PFC_LAUNCH((k_col<R, LL, COL_LEAN_E, 0>), grid, dim3(ColCfg<R, LL, 0>::NT), s, t0, t1, b, p, mode, flags);
+}
+
 template <typename R>
 static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s, hipEvent_t t0,
                         hipEvent_t t1) {
     const bool lean = mode == COL_STEP && !b.NZ && !(flags & COL_ENERGY) && (sizeof(R) == 8 || b.cA != nullptr);
+    // Energy-traced FP32 steps without noise: the table-reading kernel with
+    // the energy partials (default tiles: the partial count is col_blocks).
+    const bool lean_e = sizeof(R) == 4 && mode == COL_STEP && !b.NZ && (flags & COL_ENERGY) && b.cA != nullptr;
     // The plain step runs the tile width the autotuner chose; the other
     // modes (and a width whose tiles do not divide K) the default.
     int ts = lean ? b.col_ts : 0;
@@ -1189,10 +1212,11 @@ static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags
     switch (b.L) {
 #define CS(LL)                                                                                                  \
     case LL:                                                                                                    \
-        if (!lean) PFC_LAUNCH((k_col<R, LL, false, 0>), grid, dim3(ColCfg<R, LL, 0>::NT), s, t0, t1, b, p, mode, flags); \
-        else if (ts == 1) PFC_LAUNCH((k_col<R, LL, true, 1>), grid, dim3(ColCfg<R, LL, 1>::NT), s, t0, t1, b, p, mode, flags); \
-        else if (ts == 2) PFC_LAUNCH((k_col<R, LL, true, 2>), grid, dim3(ColCfg<R, LL, 2>::NT), s, t0, t1, b, p, mode, flags); \
-        else PFC_LAUNCH((k_col<R, LL, true, 0>), grid, dim3(ColCfg<R, LL, 0>::NT), s, t0, t1, b, p, mode, flags); \
+        if (lean_e) launch_col_e<R, LL>(grid, s, t0, t1, b, p, mode, flags);                                      \
+        else if (!lean) PFC_LAUNCH((k_col<R, LL, COL_GENERAL, 0>), grid, dim3(ColCfg<R, LL, 0>::NT), s, t0, t1, b, p, mode, flags); \
+        else if (ts == 1) PFC_LAUNCH((k_col<R, LL, COL_LEAN, 1>), grid, dim3(ColCfg<R, LL, 1>::NT), s, t0, t1, b, p, mode, flags); \
+        else if (ts == 2) PFC_LAUNCH((k_col<R, LL, COL_LEAN, 2>), grid, dim3(ColCfg<R, LL, 2>::NT), s, t0, t1, b, p, mode, flags); \
+        else PFC_LAUNCH((k_col<R, LL, COL_LEAN, 0>), grid, dim3(ColCfg<R, LL, 0>::NT), s, t0, t1, b, p, mode, flags); \
         break;
         CS(32) CS(64) CS(128) CS(256) CS(384) CS(512) CS(768) CS(1024)
 #undef CS
