@@ -57,6 +57,7 @@ struct Job {
     std::vector<double> phi, con;  // final fields (keep_fields)
     bool cancel = false;
     std::string error;
+    std::string write_error;  // set by the writer thread; the job fails at its final bundle
 };
 
 struct Batch {
@@ -201,6 +202,28 @@ void write_bundle(const Bundle& b) {
     }
 }
 
+// CVD GR every checkpoint, Q6_GLO when kd % 20 == 0 (cvd.cpp:1253-1254), on
+// the captured atoms; runs on the writer thread (its own stream), beside
+// the scheduler's stepping.
+bool cvd_order(Bundle& b, std::string& err) {
+    const pfc_params& p = b.job->spec.p;
+    int nbin = 0;
+    bool ok = pfc_cvd_gr(p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.py.data(), b.pz.data(), nullptr, &nbin) == 0;
+    if (ok) {
+        b.hist.resize((size_t)nbin);
+        ok = pfc_cvd_gr(p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.py.data(), b.pz.data(), b.hist.data(), &nbin) == 0;
+    }
+    if (ok && b.kd >= 0 && b.kd % 20 == 0 && b.na > 0) {
+        b.q6nb.resize((size_t)b.na);
+        b.q6l.resize((size_t)b.na);
+        ok = pfc_cvd_q6(p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.py.data(), b.pz.data(), b.q6nb.data(),
+                        b.q6l.data(), &b.q6g, b.q6h) == 0;
+        b.has_q6 = ok;
+    }
+    if (!ok) err = pfc_last_error();
+    return ok;
+}
+
 void writer_loop(pfc_server* s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
@@ -212,9 +235,19 @@ void writer_loop(pfc_server* s) {
         Bundle b = std::move(s->wq.front());
         s->wq.pop_front();
         lk.unlock();
-        write_bundle(b);
+        std::string err;
+        const bool ok = b.job->spec.p.model != PFC_MODEL_CVD || cvd_order(b, err);
+        if (ok) write_bundle(b);
         lk.lock();
-        if (b.final) finish(s, b.job, PFC_JOB_DONE, s->now());
+        if (!ok && b.job->write_error.empty()) b.job->write_error = "checkpoint " + std::to_string(b.kd) + ": " + err;
+        if (b.final) {
+            if (b.job->write_error.empty()) {
+                finish(s, b.job, PFC_JOB_DONE, s->now());
+            } else {
+                b.job->error = b.job->write_error;
+                finish(s, b.job, PFC_JOB_FAILED, s->now());
+            }
+        }
     }
 }
 
@@ -251,24 +284,8 @@ bool capture(Batch& bt, int slot, Bundle& b, const double* e_all, std::string& e
             err = pfc_last_error();
             return false;
         }
-        int nbin = 0;
-        bool ok = pfc_cvd_gr(p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.py.data(), b.pz.data(), nullptr, &nbin) == 0;
-        if (ok) {
-            b.hist.resize((size_t)nbin);
-            ok = pfc_cvd_gr(p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.py.data(), b.pz.data(), b.hist.data(),
-                            &nbin) == 0;
-        }
-        if (ok && b.kd >= 0 && b.kd % 20 == 0 && b.na > 0) {
-            b.q6nb.resize((size_t)b.na);
-            b.q6l.resize((size_t)b.na);
-            ok = pfc_cvd_q6(p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.py.data(), b.pz.data(), b.q6nb.data(),
-                            b.q6l.data(), &b.q6g, b.q6h) == 0;
-            b.has_q6 = ok;
-        }
-        if (!ok) {
-            err = pfc_last_error();
-            return false;
-        }
+        // GR and Q6 need only these atoms: the writer thread computes them.
+        b.px.resize((size_t)b.na); b.py.resize((size_t)b.na); b.pz.resize((size_t)b.na); b.pp.resize((size_t)b.na);
     }
     return true;
 }
