@@ -335,10 +335,16 @@ static int run_elastic(cli_t* c) {
     char indir[4096] = "";
     if (c->indir[0]) {
         if (c->indir[0] == '/') snprintf(indir, sizeof indir, "%s", c->indir);
-        else { char cwd[2048]; if (!getcwd(cwd, sizeof cwd)) return 2; snprintf(indir, sizeof indir, "%s/%s", cwd, c->indir); }
+        else {
+            char cwd[2048];
+            if (!getcwd(cwd, sizeof cwd)) return 2;
+            const int w = snprintf(indir, sizeof indir, "%s/%s", cwd, c->indir);
+            if (w < 0 || w >= (int)sizeof indir) return 2;
+        }
     }
     if (setup_outdir(c->outdir) != 0) return 2;
     FILE* f = fopen("run_config.txt", "w");
+    if (!f) return 2;
     fprintf(f, "model elastic\nu0 %g\ncon0 %g\nsig %g\ndt %g\ndx %g\n", p->u0, p->con0, p->sig, p->dt, p->dx);
     fprintf(f, "steps %d\nmod %d\nseed %u\nL %d\nM %d\nN %d\n", p->steps, p->mod, p->seed, p->L, p->M, p->N);
     fprintf(f, "grainx %d\ngrainy %d\ngrainz %d\nbenchel %g\n", p->grainx, p->grainy, p->grainz, c->benchel);
