@@ -138,6 +138,36 @@ def test_batch_equals_single(hip):
         assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
 
 
+@pytest.mark.parametrize("L", [512, 384])
+def test_sweep_class_full_size(hip, L):
+    """A sweep grid class at its benchmark size (16 instances of L^2, FP32,
+    graph-captured steps): instances 0 and 15 equal the same fields stepped
+    alone, bit for bit, and every instance's read-back fields keep their
+    initial means (mass conservation) to FP32 transform round-off."""
+    insts = []
+    for k in range(16):
+        q = hip.default_params(model=0, L=L, M=L, precision=hip.FP32, u0=0.02 + 0.02 * (k % 4),
+                               con0=0.1 + 0.1 * (k // 4), seed=20200604 + k)
+        insts.append(hip.init_fields(q))
+    p = hip.default_params(model=0, L=L, M=L, precision=hip.FP32)
+    b = hip.Sim(p, batch=16)
+    for k, (phi, con) in enumerate(insts):
+        b.set_fields(phi, con, inst=k)
+    b.set_graph(25)
+    b.step(100)
+    for k, (phi, con) in enumerate(insts):
+        x = b.get_fields(k)
+        assert abs(x[0].mean() - phi.mean()) < 1e-6 and abs(x[1].mean() - con.mean()) < 1e-6, k
+    for k in (0, 15):
+        s1 = hip.Sim(p)
+        s1.set_fields(*insts[k])
+        s1.step(100)
+        x, y = s1.get_fields(), b.get_fields(k)
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), k
+        s1.close()
+    b.close()
+
+
 @pytest.mark.parametrize("model,L,M", [(0, 128, 64), (1, 384, 128), (0, 384, 384), (1, 1024, 64)])
 def test_energy_traced_steps_match_plain(hip, model, L, M):
     """FP32: energy-traced steps (the column kernel that also sums the
