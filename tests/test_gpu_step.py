@@ -45,6 +45,29 @@ def test_fp64_exact_math(hip, oracle, kind, L, M, kw):
         assert err < 1e-11
 
 
+@pytest.mark.parametrize("kind,L,n,prec,narrow,tol", [
+    (0, 512, 5, 0, 0, 1e-11),    # Misfit 512^2 (the accuracy-study grid): FP64 exact math
+    (0, 512, 5, 0, 1, 1e-6),     # ... FP64 with the reference narrowing (the drop-in default)
+    (0, 512, 5, 1, 1, 1e-5),     # ... FP32 (measured 8.7e-7)
+    (1, 1024, 3, 0, 1, 1e-6),    # CVD 1024^2 (the checkpoint-run grid): FP64 drop-in
+    (1, 1024, 3, 1, 1, 1e-5),    # ... FP32 (measured 1.3e-6)
+])
+def test_full_size_vs_oracle(hip, oracle, kind, L, n, prec, narrow, tol):
+    """The benchmark's single-instance grids against the oracle on the same
+    seeded initial fields (a few steps: the oracle takes about a second per
+    step at 1024^2)."""
+    kw = dict(grainx=16, grainy=16) if kind == 0 else {}
+    sim, osim = _pair(hip, oracle, kind, L, L, narrow, prec, **kw)
+    sim.step(n)
+    osim.run(n)
+    phi, con = sim.get_fields()
+    ophi, ocon = osim.fields()
+    err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
+    print(f"kind={kind} {L}^2 prec={prec} narrow={narrow} t={n} err={err:.3e}", flush=True)
+    assert err < tol
+    sim.close()
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 def test_fp64_narrow(hip, oracle, kind):
     kw = dict(grainx=4, grainy=4) if kind == 0 else {}
