@@ -89,7 +89,7 @@ size_t ctx_bytes(const pfc_params& p, int batch) {
         b += nreal * rs + nvec * 2 * rs + noise_scratch_bytes((long)P, batch);
     const size_t tables = ((size_t)(p.L / 2 + 1) * (p.M / 2 + 1) * (p.N / 2 + 1) * 2 + p.L + p.M + p.N) * 4 +
                           (size_t)(p.L + p.M + p.N + p.M / 2) * 2 * rs +
-                          (rs == 4 ? (size_t)p.L * p.N * (p.M / 2) * 24 : 0);
+                          (col_coef(rs == 8, p.L) ? (size_t)p.L * p.N * (p.M / 2) * 24 : 0);
     return b + tables + (size_t)batch * 256 + (1u << 20);
 }
 }  // namespace pfc
@@ -186,7 +186,7 @@ static int tune_cols(pfc_ctx* c) {
     const char* env = std::getenv("PFC_AUTOTUNE");
     if (env && env[0] == '0') return 0;
     if (c->noise) return 0;  // steps with the noise term run the general kernel
-    if (c->prec == PREC_FP32 && !c->bufs.cA) return 0;  // no plain-step kernel in use
+    if (col_coef(c->prec == PREC_FP64, c->L) && !c->bufs.cA) return 0;  // no plain-step kernel in use
     const std::array<int, 5> key{c->prec, c->L, c->M, c->N, c->B};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -367,18 +367,18 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     b.active = c->active_d;
     b.L = c->L; b.M = c->M; b.N = c->N; b.K = c->K; b.B = batch;
     b.cA = b.cC = nullptr;
-    // FP32 plain steps read the per-mode coefficients from tables
-    // (coalesced, 24 B per mode, shared by the batch) instead of gathering
-    // C2 and forming k^2 per mode: faster at every batch size measured,
-    // one instance included (512^2: 24.8 -> 19.4 us per step).
-    if (prec == PREC_FP32) {
+    // Plain steps read the per-mode coefficients from tables (coalesced,
+    // 24 B per mode, shared by the batch) instead of gathering C2 and
+    // forming k^2 per mode: faster at every batch size measured, one
+    // instance included (FP32 512^2: 24.8 -> 19.4 us per step).
+    if (col_coef(prec == PREC_FP64, c->L)) {
         const size_t nm = (size_t)c->L * c->N * c->K;
         e = hipMalloc(&c->coefA, nm * 16);
         if (e == hipSuccess) e = hipMalloc(&c->coefC, nm * 8);
         if (e == hipSuccess) {
             b.cA = c->coefA;
             b.cC = c->coefC;
-            e = launch_coef(b, ph, c->stream);
+            e = launch_coef(prec, b, ph, c->stream);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) {

@@ -53,9 +53,8 @@ struct DevBufs {
     const void* NZ;       // [B][L][K] complex: row-spectral noise (nullptr: no noise)
     long long* tcount;    // [B] step counter per instance (advanced by the column pass)
     const int* active;    // [B] 0 = instance idle (step kernels skip it); nullptr = all
-    // FP32 per-mode coefficients in the column layout [L][N][K] (nullptr: the
-    // column pass evaluates them from kl/km/kn/c2): cA = {mn dt k^2, mc dt k^2,
-    // 1/(1 + mn dt k^2), 1/(1 + mc dt alpha k^4)}, cC = {C2A, C2B}.
+    // Per-mode coefficients in the column layout [L][N][K] (launch_coef;
+    // nullptr: the column pass evaluates them from kl/km/kn/c2), cC = {C2A, C2B}.
     const void* cA;
     const void* cC;
     int col_ts;  // plain-step column tile width chosen by the autotuner (0 default, 1 wider, 2 narrower)
@@ -73,8 +72,16 @@ enum { ROW_WRITE_IN = 1, ROW_ENERGY = 2 };
 enum { COL_INIT = 0, COL_STEP = 1, COL_ENERGY_ONLY = 2 };
 enum { COL_ENERGY = 1 };
 
-// Fill cA/cC (FP32 contexts) from kl/km/kn/c2 and the step constants.
-hipError_t launch_coef(const DevBufs& b, const PhysArgs& p, hipStream_t s);
+// Whether the plain-step column kernels of this precision and length read
+// per-mode coefficient tables: FP32, every length.  (FP64 tables for the
+// powers of two below 1024 were built and timed: no gain on single
+// instances, whose columns are not held up by the C2 gather.)
+constexpr bool col_coef(bool f64, int L) { return !f64 && L > 0; }
+
+// Fill cA/cC from kl/km/kn/c2 and the step constants: FP32 {mk, mkc,
+// 1/(1 + mk), 1/(1 + den)}, FP64 {mk, mkc, den, 0} (floats, as the reference
+// forms them), and {C2A, C2B}.
+hipError_t launch_coef(int prec, const DevBufs& b, const PhysArgs& p, hipStream_t s);
 
 bool size_supported(int n);
 bool depth_supported(int n);                    // N > 1 lengths of the depth pass

@@ -147,16 +147,25 @@ __device__ __forceinline__ Cx<float> noise_term(const PhysArgs& p, float ks, boo
     return Cx<float>{p.db * nz.x * sks * ca * damp, p.db * nz.y * sks * cb * damp};
 }
 
-// Semi-implicit update of one mode (misfit.cpp:1002-1010, cvd.cpp:872-880);
-// nt is the noise term (zero when b0 = 0).
-__device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<double, 2>& s,
-                                            const Vec<double, 2>& o, Cx<double> nt) {
+// FP64 coefficients of one mode as the reference forms them in float:
+// {mk, mkc, den, 0} = {mn dt k^2, mc dt k^2, mc dt alpha k^4, 0}.
+__device__ __forceinline__ float4 mode_coef64(const PhysArgs& p, float ks) {
 #pragma clang fp contract(off)
     const float mk = p.mn_dt * ks;
-    double r1 = s.c[0].x - (double)mk * o.c[0].x + nt.x;
-    double r2 = s.c[0].y - (double)mk * o.c[0].y + nt.y;
     const float mkc = p.mc_dt * ks;
     const float den = (p.mc_dt_a * ks) * ks;
+    return make_float4(mk, mkc, den, 0.0f);
+}
+
+// Semi-implicit update of one mode (misfit.cpp:1002-1010, cvd.cpp:872-880)
+// from its coefficients c = mode_coef64(ks); nt is the noise term (zero
+// when b0 = 0).
+__device__ __forceinline__ void update_mode_coef64(const PhysArgs& p, float4 c, Vec<double, 2>& s,
+                                                   const Vec<double, 2>& o, Cx<double> nt) {
+#pragma clang fp contract(off)
+    const float mk = c.x, mkc = c.y, den = c.z;
+    double r1 = s.c[0].x - (double)mk * o.c[0].x + nt.x;
+    double r2 = s.c[0].y - (double)mk * o.c[0].y + nt.y;
     double r3 = s.c[1].x - (double)mkc * o.c[1].x;
     double r4 = s.c[1].y - (double)mkc * o.c[1].y;
     if (p.narrow) {
@@ -167,6 +176,11 @@ __device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<dou
     const double d2 = 1.0 + (double)den;
     s.c[0].x = r1 / d1; s.c[0].y = r2 / d1;
     s.c[1].x = r3 / d2; s.c[1].y = r4 / d2;
+}
+
+__device__ __forceinline__ void update_mode(const PhysArgs& p, float ks, Vec<double, 2>& s,
+                                            const Vec<double, 2>& o, Cx<double> nt) {
+    update_mode_coef64(p, mode_coef64(p, ks), s, o, nt);
 }
 
 // FP32 coefficients of one mode: {mk, mkc, 1/(1 + mk), 1/(1 + den)}.  The
@@ -261,7 +275,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #define PFC_XCD_REMAP 1  // workgroup x takes tile (x % 8)*(nx/8) + x/8: one XCD's tiles are adjacent
 #endif
 #ifndef PFC_COEF
-#define PFC_COEF 1  // FP32 plain steps read per-mode coefficient tables (k_coef)
+#define PFC_COEF 1  // plain steps read per-mode coefficient tables (k_coef) where col_coef holds
 #endif
 // Both passes
 #ifndef PFC_GEN
@@ -701,18 +715,19 @@ __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs&
         Cx<R> ph = x[q].c[0], co = x[q].c[1], a, bb;
         if (COEF && !packed) {
             if constexpr (COEF) {
-                Vec<float, 2> sv, ov;
+                Vec<R, 2> sv, ov;
                 sv.c[0] = ph; sv.c[1] = co;
                 ov.c[0] = x[q].c[2]; ov.c[1] = x[q].c[3];
-                update_mode_coef(cf[q], sv, ov, Cx<float>{0.0f, 0.0f});
+                if constexpr (sizeof(R) == 4) update_mode_coef(cf[q], sv, ov, Cx<float>{0.0f, 0.0f});
+                else update_mode_coef64(p, cf[q], sv, ov, Cx<double>{0.0, 0.0});
                 ph = sv.c[0]; co = sv.c[1];
                 if (i == 0 && kmi == 0 && kni == 0) {
                     ph = Cx<R>{R(0), R(0)};
                     co = Cx<R>{R(0), R(0)};
                 }
                 if (want_e) g += mode_energy(b, LL, i, kmi, kni, co);
-                a = Cx<R>{ph.x * cc[q].x, ph.y * cc[q].x};
-                bb = Cx<R>{ph.x * cc[q].y, ph.y * cc[q].y};
+                a = Cx<R>{ph.x * (R)cc[q].x, ph.y * (R)cc[q].x};
+                bb = Cx<R>{ph.x * (R)cc[q].y, ph.y * (R)cc[q].y};
             }
         } else if (!packed) {
             col_mode(b, p, mode, want_e, use_nz, LL, i, kmi, kni, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
@@ -770,7 +785,8 @@ __device__ void energy_reduce_block(const DevBufs& b, const PhysArgs& p, int ins
 // KIND 1 (COL_LEAN): the plain step (mode STEP, no noise, no energy) with the
 // other paths compiled out, which frees registers for occupancy.
 // KIND 2 (COL_LEAN_E): the plain step with its energy partials (energy-traced
-// steps without noise).  Kinds 1 and 2 read FP32 coefficient tables.
+// steps without noise).  Kinds 1 and 2 read coefficient tables where
+// col_coef holds.
 enum { COL_GENERAL = 0, COL_LEAN = 1, COL_LEAN_E = 2 };
 template <typename R, int LL, int KIND, int TS>
 __global__ void __launch_bounds__((ColCfg<R, LL, TS>::NT))
@@ -823,7 +839,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     // as one instance of S, loaded with the state so their latency overlaps),
     // in the mapping the per-mode work sees (after the forward transform):
     // rows t + GB*q, q < EB, in threads t < GB.
-    constexpr bool COEF = PFC_COEF && LEAN && sizeof(R) == 4;
+    constexpr bool COEF = PFC_COEF && LEAN && col_coef(sizeof(R) == 8, LL);
     constexpr int EBc = C::EB, GBc = C::GB;
     float4 cf[COEF ? EBc : 1];
     float2 cc[COEF ? EBc : 1];
@@ -902,7 +918,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
 // (kc = 0 of slots other than 0 and N/2 carries k_m = 0 or M/2 with k_n
 // remapped); the packed columns' entries are unused.  Same float expressions
 // as update_mode, so the coefficient path is bit-identical.
-__global__ void __launch_bounds__(256) k_coef(DevBufs b, PhysArgs p) {
+__global__ void __launch_bounds__(256) k_coef(DevBufs b, PhysArgs p, int f64) {
     const long long n = (long long)b.L * b.N * b.K;
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
@@ -919,14 +935,14 @@ __global__ void __launch_bounds__(256) k_coef(DevBufs b, PhysArgs p) {
     const int il = i <= b.L / 2 ? i : b.L - i;
     const int jn = kni <= Nh ? kni : b.N - kni;
     const size_t ci = 2 * (((size_t)il * (Mh + 1) + kmi) * (Nh + 1) + jn);
-    ((float4*)const_cast<void*>(b.cA))[e] = mode_coef(p, ks);
+    ((float4*)const_cast<void*>(b.cA))[e] = f64 ? mode_coef64(p, ks) : mode_coef(p, ks);
     ((float2*)const_cast<void*>(b.cC))[e] = make_float2(b.c2[ci], b.c2[ci + 1]);
 }
 
-hipError_t launch_coef(const DevBufs& b, const PhysArgs& p, hipStream_t s) {
+hipError_t launch_coef(int prec, const DevBufs& b, const PhysArgs& p, hipStream_t s) {
     if (!b.cA || !b.cC) return hipErrorInvalidValue;
     const long long n = (long long)b.L * b.N * b.K;
-    hipLaunchKernelGGL(k_coef, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, p);
+    hipLaunchKernelGGL(k_coef, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, p, prec == PREC_FP64 ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -1187,21 +1203,22 @@ static hipError_t row_step_t(const DevBufs& b, const PhysArgs& p, int flags, hip
     return hipGetLastError();
 }
 
-// The energy-step table kernel exists for FP32 only.
 template <typename R, int LL>
 static void launch_col_e(dim3 grid, hipStream_t s, hipEvent_t t0, hipEvent_t t1, const DevBufs& b,
                          const PhysArgs& p, int mode, int flags) {
-    if constexpr (sizeof(R) == 4)
-        PFC_LAUNCH((k_col<R, LL, COL_LEAN_E, 0>), grid, dim3(ColCfg<R, LL, 0>::NT), s, t0, t1, b, p, mode, flags);
+    PFC_LAUNCH((k_col<R, LL, COL_LEAN_E, 0>), grid, dim3(ColCfg<R, LL, 0>::NT), s, t0, t1, b, p, mode, flags);
 }
 
 template <typename R>
 static hipError_t col_t(const DevBufs& b, const PhysArgs& p, int mode, int flags, hipStream_t s, hipEvent_t t0,
                         hipEvent_t t1) {
-    const bool lean = mode == COL_STEP && !b.NZ && !(flags & COL_ENERGY) && (sizeof(R) == 8 || b.cA != nullptr);
-    // Energy-traced FP32 steps without noise: the table-reading kernel with
-    // the energy partials (default tiles: the partial count is col_blocks).
-    const bool lean_e = sizeof(R) == 4 && mode == COL_STEP && !b.NZ && (flags & COL_ENERGY) && b.cA != nullptr;
+    // The specialised kernels of a length whose plain step reads coefficient
+    // tables need them (every context allocates them where col_coef holds).
+    const bool tables_ok = !col_coef(sizeof(R) == 8, b.L) || b.cA != nullptr;
+    const bool lean = mode == COL_STEP && !b.NZ && !(flags & COL_ENERGY) && tables_ok;
+    // Energy-traced steps without noise: the plain-step kernel with the
+    // energy partials (default tiles: the partial count is col_blocks).
+    const bool lean_e = mode == COL_STEP && !b.NZ && (flags & COL_ENERGY) && tables_ok;
     // The plain step runs the tile width the autotuner chose; the other
     // modes (and a width whose tiles do not divide K) the default.
     int ts = lean ? b.col_ts : 0;
