@@ -147,7 +147,7 @@ int pfc_col_tiles(pfc_ctx* ctx, int* columns);
 int pfc_set_col_tiles(pfc_ctx* ctx, int candidate);
 
 /* Contexts stepped as one graph (no counterpart in the reference, which runs
- * one job per process; for a sweep's grid classes, SURVEY cfg 3): the graph
+ * one job per process; for the grid classes of an uneven sweep): the graph
  * holds `chunk` steps of every context as parallel branches, so one launch
  * advances all of them.  pfc_group_step runs nchunks launches, ordered after
  * work already queued on the member contexts and before work queued on them
