@@ -71,14 +71,22 @@ static void to_payload(int L, int M, int N, const float* field, float* pay) {
 
 static void from_payload(int L, int M, int N, const float* pay, size_t count, float* field) {
     constexpr int B = 32;
+    const size_t total = static_cast<size_t>(L) * M * N;
     for (int n = 0; n < N; n++)
-        for (int m0 = 0; m0 < M; m0 += B)
-            for (int l0 = 0; l0 < L; l0 += B) {
+        for (int l0 = 0; l0 < L; l0 += B)
+            for (int m0 = 0; m0 < M; m0 += B) {
                 const int m1 = std::min(M, m0 + B), l1 = std::min(L, l0 + B);
-                for (int m = m0; m < m1; m++) {
-                    const size_t row = (static_cast<size_t>(n) * M + m) * L;
-                    for (int l = l0; l < l1 && row + l < count; l++)
-                        field[n + static_cast<size_t>(N) * (m + static_cast<size_t>(M) * l)] = pay[row + l];
+                for (int l = l0; l < l1; l++) {
+                    float* dst = field + n + static_cast<size_t>(N) * static_cast<size_t>(M) * l;
+                    const float* src = pay + static_cast<size_t>(n) * M * L + l;
+                    if (count == total) {
+                        for (int m = m0; m < m1; m++) dst[static_cast<size_t>(N) * m] = src[static_cast<size_t>(m) * L];
+                    } else {
+                        for (int m = m0; m < m1; m++) {
+                            const size_t at = (static_cast<size_t>(n) * M + m) * L + l;
+                            if (at < count) dst[static_cast<size_t>(N) * m] = src[static_cast<size_t>(m) * L];
+                        }
+                    }
                 }
             }
 }

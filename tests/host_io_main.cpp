@@ -49,6 +49,19 @@ int main(int argc, char** argv) {
     std::fwrite(back.data(), sizeof(float), P, f);
     std::fclose(f);
 
+    // A short file (as the reference's reader allows): the values present fill
+    // the field in payload order, the rest stay as they were.
+    if (FILE* sf = std::fopen((d + "/short.vti").c_str(), "rb")) {
+        std::fclose(sf);
+        std::vector<float> part(P, -7.0f);
+        const long sgot = pfcio::read_vti(d + "/short.vti", L, M, N, part.data());
+        f = std::fopen((d + "/short_back.bin").c_str(), "wb");
+        if (!f) return 2;
+        std::fwrite(part.data(), sizeof(float), P, f);
+        std::fclose(f);
+        std::printf("short %ld\n", sgot);
+    }
+
     // Tasks of the background writer run in the order they were posted.
     {
         pfcio::Writer out(2);

@@ -108,6 +108,26 @@ def test_vti_round_trip_and_order(driver, tmp_path, L, M, N):
     assert raw[i + 4 + 4 * P:] == b"\n</AppendedData>\n</VTKFile>\n"
 
 
+@pytest.mark.parametrize("L,M,N", [(64, 48, 1), (16, 8, 5)])
+def test_vti_short_file(driver, tmp_path, L, M, N):
+    """READPHIVTIN on a truncated file: the values present fill the field in
+    payload order (n, m, l with l fastest), the rest are left untouched."""
+    d = str(tmp_path)
+    P = L * M * N
+    k = P // 3 + 5
+    vals = np.arange(k, dtype=np.float32)
+    with open(os.path.join(d, "short.vti"), "wb") as f:
+        f.write(b"x\n" * 12 + b"_" + np.int32(4 * P + 4).tobytes() + vals.tobytes())
+    a, r = run_case(driver, d, 1, L, M, N, seed=5)
+    assert f"short {k}" in r.stdout.split("\n")
+    got = np.fromfile(os.path.join(d, "short_back.bin"), np.float32).reshape(L, M, N)
+    exp = np.full(P, -7.0, np.float32)
+    pay = exp.copy()
+    pay[:k] = vals                                   # payload order: index (n*M + m)*L + l
+    exp = pay.reshape(N, M, L).transpose(2, 1, 0)    # -> field index n + N*(m + M*l)
+    assert np.array_equal(got, exp)
+
+
 def test_writer_keeps_post_order(driver, tmp_path):
     d = str(tmp_path)
     run_case(driver, d, 1, 4, 4, 1, seed=3)
