@@ -540,6 +540,8 @@ extern "C" int pfc_set_graph(pfc_ctx* c, int chunk) {
     }
     c->graph = g;
     HIP_TRY(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    HIP_TRY(hipGraphUpload(c->gexec, c->stream));  // not on the first launch's time
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return PFC_OK;
 }
 
@@ -600,6 +602,9 @@ extern "C" int pfc_group_create(pfc_ctx** ctxs, int n, int chunk, pfc_group** ou
         e = ec != hipSuccess ? ec : e2;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0);
+    // Upload now, so that the first pfc_group_step does not carry it.
+    if (e == hipSuccess) e = hipGraphUpload(g->gexec, ctxs[0]->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctxs[0]->stream);
     if (e != hipSuccess) {
         free_group(g);
         return fail(PFC_E_HIP, "group graph capture: %s", hipGetErrorString(e));
