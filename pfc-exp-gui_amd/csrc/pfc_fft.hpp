@@ -430,18 +430,27 @@ __device__ __forceinline__ void fft_reg(Vec<R, NC> (&x)[E], const L& lds, int t,
 // needs 12 and falls to one wave per SIMD).  Thread t enters the first pass
 // with elements t + GP0*q (q < N/GP0) and leaves the last with t + GPL*q.
 // ---------------------------------------------------------------------------
-constexpr int gen_npass(int n) {
-    int r = n / 6, c = 1;
-    while (r % 8 == 0) { r /= 8; c++; }
+// Plans: 0 = radix 6, then 8s (the one above); 1 = radix 3, then 4s over
+// N/3 threads (at most 4 elements per thread: half the registers of plan 0,
+// one more exchange).  The row and column configurations choose per length
+// and precision.
+constexpr int gen_r0(int plan) { return plan ? 3 : 6; }  // first radix
+constexpr int gen_rb(int plan) { return plan ? 4 : 8; }  // body radix
+constexpr int gen_g(int n, int plan) { return n / gen_r0(plan); }  // threads of one transform
+constexpr int gen_npass(int n, int plan) {
+    int r = n / gen_r0(plan), c = 1;
+    while (r % gen_rb(plan) == 0) { r /= gen_rb(plan); c++; }
     return r > 1 ? c + 1 : c;
 }
-constexpr int gen_radix_fwd(int n, int p) {
-    if (p == 0) return 6;
-    int r = n / 6;
-    for (int i = 1; i < p; i++) r /= 8;
-    return r % 8 == 0 ? 8 : r;
+constexpr int gen_radix_fwd(int n, int p, int plan) {
+    if (p == 0) return gen_r0(plan);
+    int r = n / gen_r0(plan);
+    for (int i = 1; i < p; i++) r /= gen_rb(plan);
+    return r % gen_rb(plan) == 0 ? gen_rb(plan) : r;
 }
-constexpr int gen_radix(int n, int p, bool rev) { return rev ? gen_radix_fwd(n, gen_npass(n) - 1 - p) : gen_radix_fwd(n, p); }
+constexpr int gen_radix(int n, int p, bool rev, int plan) {
+    return rev ? gen_radix_fwd(n, gen_npass(n, plan) - 1 - p, plan) : gen_radix_fwd(n, p, plan);
+}
 constexpr int gen_threads(int n, int rad, int g) {
     const int m = n / rad;
     int t = g < m ? g : m;
@@ -449,23 +458,25 @@ constexpr int gen_threads(int n, int rad, int g) {
     return t;
 }
 // Elements per thread entering (first) or leaving (last) a plan.
-constexpr int gen_e_in(int n, bool rev) { return n / gen_threads(n, gen_radix(n, 0, rev), n / 6); }
-constexpr int gen_g_in(int n, bool rev) { return gen_threads(n, gen_radix(n, 0, rev), n / 6); }
-constexpr int gen_emax(int n) {
+constexpr int gen_e_in(int n, bool rev, int plan) {
+    return n / gen_threads(n, gen_radix(n, 0, rev, plan), gen_g(n, plan));
+}
+constexpr int gen_g_in(int n, bool rev, int plan) { return gen_threads(n, gen_radix(n, 0, rev, plan), gen_g(n, plan)); }
+constexpr int gen_emax(int n, int plan) {
     int e = 0;
-    for (int p = 0; p < gen_npass(n); p++) {
-        const int x = n / gen_threads(n, gen_radix(n, p, false), n / 6);
+    for (int p = 0; p < gen_npass(n, plan); p++) {
+        const int x = n / gen_threads(n, gen_radix(n, p, false, plan), gen_g(n, plan));
         e = x > e ? x : e;
     }
     return e;
 }
 
-template <typename R, int NC, int N, int DIR, bool REV, int PI, int NS, int EM, typename L>
+template <typename R, int NC, int N, int DIR, bool REV, int PI, int NS, int EM, int PLAN, typename L>
 __device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds, int t,
                                                const Cx<R>* __restrict__ tw, const Cx<R>* wp) {
     if constexpr (NS < N) {
-        constexpr int G = N / 6;
-        constexpr int RAD = gen_radix(N, PI, REV);
+        constexpr int G = gen_g(N, PLAN);
+        constexpr int RAD = gen_radix(N, PI, REV, PLAN);
         constexpr int GP = gen_threads(N, RAD, G);
         constexpr int NB = N / (RAD * GP), STEP = N / RAD;
         constexpr bool LAST = (NS * RAD == N);
@@ -506,7 +517,7 @@ __device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds
             }
         } else {
             constexpr int NS2 = NS * RAD;
-            constexpr int RAD2 = gen_radix(N, PI + 1, REV);
+            constexpr int RAD2 = gen_radix(N, PI + 1, REV, PLAN);
             constexpr int GP2 = gen_threads(N, RAD2, G);
             constexpr int NB2 = N / (RAD2 * GP2);
             Cx<R> wn[NB2 * (RAD2 - 1)];
@@ -522,7 +533,7 @@ __device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds
                 }
             }
             xsync<L::WSYNC>();
-            fft_gen_passes<R, NC, N, DIR, REV, PI + 1, NS2, EM, L>(x, lds, t, tw, wn);
+            fft_gen_passes<R, NC, N, DIR, REV, PI + 1, NS2, EM, PLAN, L>(x, lds, t, tw, wn);
         }
     }
 }
@@ -531,10 +542,10 @@ __device__ __forceinline__ void fft_gen_passes(Vec<R, NC> (&x)[EM], const L& lds
 // loaded/stored side) and side B (the other).  Powers of two: fft_reg, the
 // same mapping on both sides (E elements, G = N/E threads).  N = 3*2^k: the
 // general plan, A -> B in forward order, B -> A reversed.
-template <typename R, int NC, int N, int EM, int DIR, bool A2B, bool GEN, typename L>
+template <typename R, int NC, int N, int EM, int DIR, bool A2B, bool GEN, int PLAN, typename L>
 __device__ __forceinline__ void fft_ab(Vec<R, NC> (&x)[EM], const L& lds, int t, const Cx<R>* __restrict__ tw) {
     if constexpr (GEN) {
-        fft_gen_passes<R, NC, N, DIR, !A2B, 0, 1, EM, L>(x, lds, t, tw, nullptr);
+        fft_gen_passes<R, NC, N, DIR, !A2B, 0, 1, EM, PLAN, L>(x, lds, t, tw, nullptr);
     } else {
         fft_reg<R, NC, N, EM, DIR>(x, lds, t, tw);
     }
@@ -542,13 +553,13 @@ __device__ __forceinline__ void fft_ab(Vec<R, NC> (&x)[EM], const L& lds, int t,
 
 // Mappings of a length N on its two sides: thread counts and elements.
 // GEN: the general plan (N = 3*2^k); otherwise fft_reg with EPOW2 elements.
-template <int N, int EPOW2, bool USEGEN>
+template <int N, int EPOW2, bool USEGEN, int PLAN>
 struct Sides {
     static constexpr bool GEN = USEGEN && N % 3 == 0;
-    static constexpr int EM = GEN ? gen_emax(N) : EPOW2;       // register elements
-    static constexpr int G = GEN ? N / 6 : N / EPOW2;          // threads per transform
-    static constexpr int GA = GEN ? gen_g_in(N, false) : G, EA = GEN ? gen_e_in(N, false) : EPOW2;
-    static constexpr int GB = GEN ? gen_g_in(N, true) : G, EB = GEN ? gen_e_in(N, true) : EPOW2;
+    static constexpr int EM = GEN ? gen_emax(N, PLAN) : EPOW2;  // register elements
+    static constexpr int G = GEN ? gen_g(N, PLAN) : N / EPOW2;  // threads per transform
+    static constexpr int GA = GEN ? gen_g_in(N, false, PLAN) : G, EA = GEN ? gen_e_in(N, false, PLAN) : EPOW2;
+    static constexpr int GB = GEN ? gen_g_in(N, true, PLAN) : G, EB = GEN ? gen_e_in(N, true, PLAN) : EPOW2;
 };
 
 // AoS exchange buffer (one Vec per slot).

@@ -307,12 +307,20 @@ constexpr int col_t(bool f64, int LL) {
                : (LL <= 128 ? 8 : (LL <= 768 ? 4 : 2));
 }
 
+// 3*2^k plan (pfc_fft.hpp): 1 (radix 3, then 4s; half the registers) where
+// it measured faster: rows of 384 points (H = 192), -10% FP32 and -34% FP64
+// per row pass of 384^2; FP64 columns of 384, -14%.  FP32 columns of 384 keep
+// plan 0 (+3% with plan 1).
+constexpr int row_plan(bool f64, int H) { return (void)f64, H == 192 ? 1 : 0; }
+constexpr int col_plan(bool f64, int LL) { return f64 && LL == 384 ? 1 : 0; }
+
 // Row pass: RPW rows of length M = 2H per workgroup, G = H/E threads per row.
 // LDS: one padded exchange buffer of 4-field elements per row.
 template <typename R, int H>
 struct RowCfg {
     static constexpr bool GEN = PFC_GEN && H % 3 == 0;
-    using SD = Sides<H, elems_for(H, PFC_ROW_E), GEN>;
+    static constexpr int PLAN = row_plan(sizeof(R) == 8, H);
+    using SD = Sides<H, elems_for(H, PFC_ROW_E), GEN, PLAN>;
     static constexpr int E = SD::EM, G = SD::G;  // register elements, threads per row
     // Side A: the row spectrum (loads of S/AB, stores of IN); side B: real space.
     static constexpr int EA = SD::EA, GA = SD::GA, EB = SD::EB, GB = SD::GB;
@@ -342,7 +350,8 @@ template <typename R, int LL, int TS = 0>
 struct ColCfg {
     static constexpr bool F64 = sizeof(R) == 8;
     static constexpr bool GEN = PFC_GEN && LL % 3 == 0;
-    using SD = Sides<LL, col_e(F64, LL), GEN>;
+    static constexpr int PLAN = col_plan(F64, LL);
+    using SD = Sides<LL, col_e(F64, LL), GEN, PLAN>;
     static constexpr int E = SD::EM, G = SD::G;  // register elements, threads per column
     // Side A: the loaded/stored row spectra along l; side B: the full spectrum.
     static constexpr int EA = SD::EA, GA = SD::GA, EB = SD::EB, GB = SD::GB;
@@ -416,7 +425,7 @@ __device__ __forceinline__ void row_r2c_store(Vec<R, NC> (&w)[C::E], Vec<R, NC>*
                                               const Cx<R>* __restrict__ twH, const Cx<R>* __restrict__ twM,
                                               Buf dst, uint32_t doff) {
     constexpr int H = C::K, EA = C::EA, GA = C::GA;
-    fft_ab<R, NC, H, C::E, -1, false, C::GEN>(w, LdsBuf<R, NC, 1, 0, C::WAVE>{(char*)lds}, t, twH);
+    fft_ab<R, NC, H, C::E, -1, false, C::GEN, C::PLAN>(w, LdsBuf<R, NC, 1, 0, C::WAVE>{(char*)lds}, t, twH);
     xsync<C::WAVE>();
 #pragma unroll
     for (int q = 0; q < EA; q++) lds[pidx(t + GA * q)] = w[q];
@@ -485,7 +494,7 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
         z[q].c[2] = c2r_split_packed(a0.c[0], a1.c[0], k, w);
         z[q].c[3] = c2r_split_packed(a0.c[1], a1.c[1], k, w);
     }
-    fft_ab<R, 4, H, E, +1, true, C::GEN>(z, LdsBuf<R, 4, C::CH, C::PS, C::WAVE>{smem + rl * C::RB}, t, twH);
+    fft_ab<R, 4, H, E, +1, true, C::GEN, C::PLAN>(z, LdsBuf<R, 4, C::CH, C::PS, C::WAVE>{smem + rl * C::RB}, t, twH);
 
     const double* mu = b.means + 4 * inst;
     const R mph = (R)mu[0], mco = (R)mu[1], ma = (R)mu[2], mb = (R)mu[3];
@@ -598,7 +607,7 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __
         z[q].c[0] = c2r_split_packed(s0.c[0], s1.c[0], k, w);
         z[q].c[1] = c2r_split_packed(s0.c[1], s1.c[1], k, w);
     }
-    fft_ab<R, 2, H, E, +1, true, C::GEN>(z, LdsBuf<R, 2, 1, 0>{smem + rl * C::PB * sizeof(Vec<R, 2>)}, t,
+    fft_ab<R, 2, H, E, +1, true, C::GEN, C::PLAN>(z, LdsBuf<R, 2, 1, 0>{smem + rl * C::PB * sizeof(Vec<R, 2>)}, t,
                                           (const Cx<R>*)b.twH);
     const double* mu = b.means + 4 * inst;
     const R mph = (R)mu[0], mco = (R)mu[1];
@@ -871,7 +880,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
             x[q].c[2] = x[q].c[3] = Cx<R>{R(0), R(0)};
         }
     }
-    fft_ab<R, 4, LL, E, -1, true, C::GEN>(x, lds, t, twL);
+    fft_ab<R, 4, LL, E, -1, true, C::GEN, C::PLAN>(x, lds, t, twL);
     Cx<R> nz[E];
 #pragma unroll
     for (int q = 0; q < E; q++) nz[q] = Cx<R>{R(0), R(0)};
@@ -881,7 +890,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
         for (int q = 0; q < EA; q++)
             n1[q] = bload<Vec<R, 1>>(mkbuf((const Vec<R, 1>*)b.NZ + tbase, tbytes / 2), toff / 2,
                                      (uint32_t)(G * q * CS) * (es / 2));
-        fft_ab<R, 1, LL, E, -1, true, C::GEN>(n1, LdsBuf<R, 1, 1, 0>{smem + col * C::CB}, t, twL);
+        fft_ab<R, 1, LL, E, -1, true, C::GEN, C::PLAN>(n1, LdsBuf<R, 1, 1, 0>{smem + col * C::CB}, t, twL);
 #pragma unroll
         for (int q = 0; q < E; q++) nz[q] = n1[q].c[0];
     }
@@ -897,7 +906,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     if (!LEAN && mode == COL_ENERGY_ONLY) return;
 
     // Inverse transform of the new state and of C2A/C2B times the new psi.
-    fft_ab<R, 4, LL, E, +1, false, C::GEN>(x, lds, t, twL);
+    fft_ab<R, 4, LL, E, +1, false, C::GEN, C::PLAN>(x, lds, t, twL);
     const R invL = R(1) / R(LL);
     const Buf AB = mkbuf((const Vec<R, 2>*)b.AB + tbase, tbytes);
 #pragma unroll
