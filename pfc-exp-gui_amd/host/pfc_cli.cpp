@@ -335,6 +335,9 @@ int run_sim(Kind k, int argc, char** argv) {
     std::vector<float> acon(P), px(P), py(P), pz(P), pp(P);
     if (k == MISFIT) {
         const int every = std::max(1, o.steps / 200);
+        // Steps between progress lines run as one captured graph (replays are
+        // bit-identical to eager launches); the noise term keeps eager steps.
+        if (every >= 8 && !(o.p.axx > 0.0f) && pfc_set_graph(ctx, every)) return gpu_fail("pfc_set_graph");
         int t = 0;
         while (t < o.steps) {
             // Advance to the next step that prints progress or checkpoints.

@@ -123,6 +123,32 @@ def test_elastic_cli_bit_identical(tmp_path):
         assert f"q4q6_{kd}.txt: atoms={a[0]}" in r.stderr
 
 
+def test_cli_graph_steps_match_eager(tmp_path):
+    """With 1600 steps the CLI prints progress every 8 steps and runs those 8
+    as one captured graph; its checkpoint fields equal the same steps taken
+    eagerly through the library, bit for bit."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("pfc_hip_cli", os.path.join(ROOT, "pfc-exp-gui_amd", "pfc_hip.py"))
+    hip = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(hip)
+    g = tmp_path / "g"
+    r = run(CLI, ["--model", "misfit", "--L", "128", "--M", "128", "--grainx", "4", "--grainy", "4", "--steps", "1600",
+                  "--mod", "800", "--outdir", str(g)], tmp_path)
+    steps = [int(PROGRESS.match(l).group(1)) for l in r.stdout.splitlines() if PROGRESS.match(l)]
+    assert steps[:3] == [1, 8, 16] and steps[-1] == 1600
+    p = hip.default_params(model=hip.MISFIT, L=128, M=128, precision=hip.FP64, grainx=4, grainy=4)
+    phi, con = hip.init_fields(p)
+    s = hip.Sim(p)
+    s.set_fields(phi, con)
+    s.step(1600)
+    ephi, econ = s.get_fields()
+    s.close()
+    for name, ref in (("phi", ephi), ("con", econ)):
+        _, pay, _, _ = vti_split(g / f"{name}_1600_0.vti")
+        # payload order n, m, l (l fastest) of the field index n + N*(m + M*l)
+        assert np.array_equal(pay, ref.astype(np.float32).reshape(128, 128).T.ravel()), name
+
+
 def test_fp32_cli_runs(tmp_path):
     g = tmp_path / "g"
     run(CLI, ["--model", "misfit", "--precision", "fp32", "--L", "128", "--M", "128", "--steps", "30",
