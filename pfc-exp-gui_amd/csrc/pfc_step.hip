@@ -309,10 +309,11 @@ constexpr int col_t(bool f64, int LL) {
 
 // 3*2^k plan (pfc_fft.hpp): 1 (radix 3, then 4s; half the registers) where
 // it measured faster: rows of 384 points (H = 192), -10% FP32 and -34% FP64
-// per row pass of 384^2; FP64 columns of 384, -14%.  FP32 columns of 384 keep
-// plan 0 (+3% with plan 1).
-constexpr int row_plan(bool f64, int H) { return (void)f64, H == 192 ? 1 : 0; }
-constexpr int col_plan(bool f64, int LL) { return f64 && LL == 384 ? 1 : 0; }
+// per row pass of 384^2; FP64 rows of 768 (-10%); columns of 768 (-2% FP32,
+// -8% FP64 at 768x256); FP64 columns of 384 (-14%).  FP32 columns of 384
+// (+3%) and FP32 rows of 768 (+6..12%) keep plan 0.
+constexpr int row_plan(bool f64, int H) { return (H == 192 || (f64 && H == 384)) ? 1 : 0; }
+constexpr int col_plan(bool f64, int LL) { return ((f64 && LL == 384) || LL == 768) ? 1 : 0; }
 
 // Row pass: RPW rows of length M = 2H per workgroup, G = H/E threads per row.
 // LDS: one padded exchange buffer of 4-field elements per row.

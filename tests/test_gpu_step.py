@@ -68,6 +68,25 @@ def test_full_size_vs_oracle(hip, oracle, kind, L, n, prec, narrow, tol):
     sim.close()
 
 
+@pytest.mark.parametrize("kind,L,M,prec,tol", [
+    (0, 768, 64, 0, 1e-11), (1, 64, 768, 0, 1e-11),   # FP64 exact math: 768-point columns / rows
+    (0, 768, 64, 1, 2e-4), (1, 64, 768, 1, 2e-4),     # FP32
+])
+def test_768_lengths(hip, oracle, kind, L, M, prec, tol):
+    """768-point transforms (plan 1 for columns in both precisions and for
+    FP64 rows, plan 0 for FP32 rows) against the oracle."""
+    kw = dict(grainx=4, grainy=4) if kind == 0 else {}
+    sim, osim = _pair(hip, oracle, kind, L, M, 0 if prec == 0 else 1, prec, **kw)
+    sim.step(20)
+    osim.run(20)
+    phi, con = sim.get_fields()
+    ophi, ocon = osim.fields()
+    err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
+    print(f"kind={kind} {L}x{M} prec={prec} err={err:.3e}", flush=True)
+    assert err < tol
+    sim.close()
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 def test_fp64_narrow(hip, oracle, kind):
     kw = dict(grainx=4, grainy=4) if kind == 0 else {}
