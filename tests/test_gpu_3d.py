@@ -35,6 +35,7 @@ def _err(sim, osim):
     (0, 32, 32, 16, dict(grainx=2, grainy=2)),
     (1, 64, 32, 32, {}),
     (0, 32, 64, 64, dict(grainx=2, grainy=4)),
+    (1, 32, 32, 128, {}),
 ])
 def test_3d_fp64_exact(hip, oracle, kind, L, M, N, kw):
     _, sim, osim = _pair3(hip, oracle, kind, L, M, N, hip.FP64, **kw)
@@ -55,6 +56,19 @@ def test_3d_fp32(hip, oracle, kind):
     err = _err(sim, osim)
     print(f"kind={kind} 3-D fp32 err={err:.3e}", flush=True)
     assert err < 2e-4
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_3d_serving_grid(hip, oracle, kind):
+    """The request loop's 3-D grid, 128 x 128 x 64, FP32, against the oracle
+    (a few steps: the oracle takes about a second per step here)."""
+    kw = dict(grainx=2, grainy=2) if kind == 0 else {}
+    _, sim, osim = _pair3(hip, oracle, kind, 128, 128, 64, hip.FP32, narrow=1, **kw)
+    sim.step(3)
+    osim.run(3)
+    err = _err(sim, osim)
+    print(f"kind={kind} 128x128x64 fp32 err={err:.3e}", flush=True)
+    assert err < 1e-5
 
 
 @pytest.mark.parametrize("prec", [0, 1])
