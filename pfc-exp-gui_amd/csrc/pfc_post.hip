@@ -776,8 +776,8 @@ hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, cons
 // and its quirks: both loops use l = 6 and the second keeps adding onto the
 // first one's per-atom sum, so q6 comes from the sum taken twice; atoms with
 // no or more than 16 neighbours get q = 0 but still count as neighbours.
-// Three passes: neighbour lists in the reference's j order (LDS-tiled like
-// k_strain), per-atom q_6m, neighbour averages and the sums.
+// Passes: neighbour lists in the reference's j order (the ordered scan
+// above, then k_cryst_fin), per-atom q_6m, neighbour averages and the sums.
 // ---------------------------------------------------------------------------
 constexpr int QL = 6, QM = 2 * QL + 1, CNMAX = 16;
 
