@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <exception>
 #include <filesystem>
 #include <set>
 #include <thread>
@@ -381,7 +382,11 @@ void Writer::loop() {
         busy_ = true;
         cv_.notify_all();
         lk.unlock();
-        task();
+        try {
+            task();
+        } catch (const std::exception& e) {  // a lost file, not a lost process
+            std::fprintf(stderr, "checkpoint writer: %s\n", e.what());
+        }
         lk.lock();
         busy_ = false;
         cv_.notify_all();
