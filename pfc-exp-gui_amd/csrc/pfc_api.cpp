@@ -76,6 +76,13 @@ struct pfc_ctx {
 
 extern "C" const char* pfc_last_error(void) { return g_err.c_str(); }
 
+// C entry points that allocate host memory return PFC_E_NOMEM instead of
+// letting std::bad_alloc cross the C boundary.
+#define PFC_HOST_CATCH                                                              \
+    catch (const std::exception& e) {                                               \
+        return fail(PFC_E_NOMEM, "host allocation failed: %s", e.what());            \
+    }
+
 namespace pfc {
 int api_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 
@@ -132,14 +139,14 @@ extern "C" int pfc_params_default(pfc_params* p, int model) {
     return PFC_OK;
 }
 
-extern "C" int pfc_init_fields(const pfc_params* p, double* phi, double* con) {
+extern "C" int pfc_init_fields(const pfc_params* p, double* phi, double* con) try {
     if (!p || !phi || !con || p->L <= 0 || p->M <= 0 || p->N <= 0) return fail(PFC_E_ARG, "bad arguments");
     if (p->model == PFC_MODEL_MISFIT && (p->grainx <= 0 || p->grainy <= 0 || p->grainz <= 0))
         return fail(PFC_E_ARG, "grain counts must be > 0");
     Model m = make_model(*p);
     init_fields(m, phi, con);
     return PFC_OK;
-}
+} PFC_HOST_CATCH
 
 template <typename R>
 static void* upload_twiddles(int n, hipError_t* err) {
@@ -257,7 +264,7 @@ static int tune_cols(pfc_ctx* c) {
     return best;
 }
 
-extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
+extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) try {
     if (!p || !out || batch <= 0) return fail(PFC_E_ARG, "bad arguments");
     *out = nullptr;
     if (p->model != PFC_MODEL_MISFIT && p->model != PFC_MODEL_CVD) return fail(PFC_E_ARG, "bad model %d", p->model);
@@ -390,7 +397,7 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) {
     b.col_ts = tune_cols(c);
     *out = c;
     return PFC_OK;
-}
+} PFC_HOST_CATCH
 
 extern "C" int pfc_col_tiles(pfc_ctx* ctx, int* columns) {
     if (!ctx || !columns) return fail(PFC_E_ARG, "bad arguments");
@@ -431,7 +438,7 @@ static DevBufs inst_bufs(const pfc_ctx* c, int inst) {
     return b;
 }
 
-extern "C" int pfc_set_fields(pfc_ctx* c, int inst, const double* phi, const double* con) {
+extern "C" int pfc_set_fields(pfc_ctx* c, int inst, const double* phi, const double* con) try {
     if (!c || !phi || !con || inst < 0 || inst >= c->B) return fail(PFC_E_ARG, "bad arguments");
     const size_t P = (size_t)c->L * c->M * c->N;
     double sp = 0.0, sc = 0.0;
@@ -464,7 +471,7 @@ extern "C" int pfc_set_fields(pfc_ctx* c, int inst, const double* phi, const dou
     if (c->N > 1) HIP_TRY(launch_depth(c->prec, b, +1, b.S, b.S, b.AB, b.AB, false, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return PFC_OK;
-}
+} PFC_HOST_CATCH
 
 static hipEvent_t next_event(pfc_ctx* c) {
     if (c->ev_used == c->ev.size()) {
@@ -522,7 +529,7 @@ static hipError_t enqueue_step(pfc_ctx* c, int row_flags, int col_flags, int eco
     return launch_depth(c->prec, b, +1, b.S, b.S, b.AB, b.AB, false, c->stream, ti.t0, ti.t1);
 }
 
-extern "C" int pfc_set_graph(pfc_ctx* c, int chunk) {
+extern "C" int pfc_set_graph(pfc_ctx* c, int chunk) try {
     if (!c || chunk < 0) return fail(PFC_E_ARG, "bad arguments");
     if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -543,7 +550,7 @@ extern "C" int pfc_set_graph(pfc_ctx* c, int chunk) {
     HIP_TRY(hipGraphUpload(c->gexec, c->stream));  // not on the first launch's time
     HIP_TRY(hipStreamSynchronize(c->stream));
     return PFC_OK;
-}
+} PFC_HOST_CATCH
 
 // Several contexts (the grid classes of an uneven sweep) as one graph: each
 // context's `chunk` steps are a branch forked from and joined to the first
@@ -566,7 +573,7 @@ static void free_group(pfc_group* g) {
     delete g;
 }
 
-extern "C" int pfc_group_create(pfc_ctx** ctxs, int n, int chunk, pfc_group** out) {
+extern "C" int pfc_group_create(pfc_ctx** ctxs, int n, int chunk, pfc_group** out) try {
     if (!ctxs || n <= 0 || chunk <= 0 || !out) return fail(PFC_E_ARG, "bad arguments");
     *out = nullptr;
     for (int i = 0; i < n; i++) {
@@ -611,7 +618,7 @@ extern "C" int pfc_group_create(pfc_ctx** ctxs, int n, int chunk, pfc_group** ou
     }
     *out = g;
     return PFC_OK;
-}
+} PFC_HOST_CATCH
 
 // Advances every context of the group by nchunks * chunk steps.  Work queued
 // on the member contexts' streams before the call runs first; work queued on

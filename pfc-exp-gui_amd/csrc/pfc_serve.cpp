@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -236,8 +237,14 @@ void writer_loop(pfc_server* s) {
         s->wq.pop_front();
         lk.unlock();
         std::string err;
-        const bool ok = b.job->spec.p.model != PFC_MODEL_CVD || cvd_order(b, err);
-        if (ok) write_bundle(b);
+        bool ok = false;
+        try {  // a failed checkpoint fails its job, not the writer thread
+            ok = b.job->spec.p.model != PFC_MODEL_CVD || cvd_order(b, err);
+            if (ok) write_bundle(b);
+        } catch (const std::exception& e) {
+            ok = false;
+            err = e.what();
+        }
         lk.lock();
         if (!ok && b.job->write_error.empty()) b.job->write_error = "checkpoint " + std::to_string(b.kd) + ": " + err;
         if (b.final) {
@@ -507,7 +514,14 @@ void sched_loop(pfc_server* s) {
 
 }  // namespace
 
-extern "C" int pfc_server_create(const pfc_server_opts* o, pfc_server** out) {
+// Host allocation failures (a job's initial fields, the server's state)
+// return PFC_E_NOMEM instead of letting std::bad_alloc cross the C boundary.
+#define PFC_SERVE_CATCH                                                                              \
+    catch (const std::exception& e) {                                                                \
+        return pfc::api_fail(PFC_E_NOMEM, (std::string("host allocation failed: ") + e.what()).c_str()); \
+    }
+
+extern "C" int pfc_server_create(const pfc_server_opts* o, pfc_server** out) try {
     if (!out) return pfc::api_fail(PFC_E_ARG, "bad arguments");
     *out = nullptr;
     pfc_server_opts d{};
@@ -535,9 +549,9 @@ extern "C" int pfc_server_create(const pfc_server_opts* o, pfc_server** out) {
     s->writer = std::thread(writer_loop, s);
     *out = s;
     return PFC_OK;
-}
+} PFC_SERVE_CATCH
 
-extern "C" int pfc_server_submit(pfc_server* s, const pfc_job* job, long long* id) {
+extern "C" int pfc_server_submit(pfc_server* s, const pfc_job* job, long long* id) try {
     if (!s || !job || !id || job->steps <= 0 || job->mod < 0) return pfc::api_fail(PFC_E_ARG, "bad arguments");
     const pfc_params& p = job->p;
     if (p.model != PFC_MODEL_MISFIT && p.model != PFC_MODEL_CVD) return pfc::api_fail(PFC_E_ARG, "bad model");
@@ -563,7 +577,7 @@ extern "C" int pfc_server_submit(pfc_server* s, const pfc_job* job, long long* i
     s->jobs[j->id] = std::move(j);
     s->cv_sched.notify_one();
     return PFC_OK;
-}
+} PFC_SERVE_CATCH
 
 extern "C" int pfc_server_poll(pfc_server* s, long long id, pfc_job_status* st) {
     if (!s || !st) return pfc::api_fail(PFC_E_ARG, "bad arguments");
