@@ -481,7 +481,13 @@ void sched_loop(pfc_server* s) {
                     bd.final = final;
                     bd.first = j->steps_done <= (j->spec.mod > 0 ? j->spec.mod : j->spec.steps);
                     std::string cerr;
-                    if (!capture(b, i, bd, e.data(), cerr)) {
+                    bool got = false;
+                    try {  // host buffers of the bundle: a failed allocation fails this job only
+                        got = capture(b, i, bd, e.data(), cerr);
+                    } catch (const std::exception& ex) {
+                        cerr = std::string("checkpoint capture: ") + ex.what();
+                    }
+                    if (!got) {
                         eerr = cerr;
                         release.push_back(i);
                         failed.push_back(i);
