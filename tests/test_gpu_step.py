@@ -257,6 +257,54 @@ def test_misfit_noise_term(hip, oracle, prec, tol):
     assert err < tol and quiet > 100 * tol
 
 
+@pytest.mark.parametrize("L,M,prec,tol", [(512, 512, 0, 1e-7), (384, 256, 0, 1e-7), (512, 512, 1, 5e-4)])
+def test_misfit_noise_full_size(hip, oracle, L, M, prec, tol):
+    """The noise term at the benchmark grid and a 3*2^k rectangle: 10^5 noise
+    values per field, so the ordered compaction of accepted Marsaglia pairs
+    crosses many workgroups' blocks; the GPU stream must stay the oracle's
+    sequential one (misfit.cpp:951-981) value for value."""
+    kw = dict(grainx=16, grainy=8, axx=0.001)
+    sim, osim = _pair(hip, oracle, 0, L, M, 0, prec, **kw)
+    sim.step(3)
+    osim.run(3)
+    phi, con = sim.get_fields()
+    ophi, ocon = osim.fields()
+    err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
+    print(f"noise {L}x{M} prec={prec} err={err:.3e}", flush=True)
+    assert err < tol
+    sim.close()
+
+
+def test_misfit_noise_batched_seeds(hip, oracle):
+    """Two instances of one batched FP64 context with their own seeds
+    (pfc_set_seed): each follows the oracle run of its own seed."""
+    seeds = (20240611, 7)
+    kw = dict(grainx=4, grainy=4, axx=0.001)
+    p = hip.default_params(model=0, L=256, M=128, precision=0, **kw)
+    p.narrow = 0
+    sim = hip.Sim(p, batch=2)
+    runs = []
+    for i, sd in enumerate(seeds):
+        q = hip.default_params(model=0, L=256, M=128, precision=0, seed=sd, **kw)
+        phi, con = hip.init_fields(q)
+        sim.set_fields(phi, con, inst=i)
+        sim.set_seed(sd, inst=i)
+        m = oracle.model(0, 256, 128, 1, narrow=0, seed=sd, **kw)
+        runs.append(oracle.Sim(m, phi, con))
+    sim.step(4)
+    outs = []
+    for i, osim in enumerate(runs):
+        osim.run(4)
+        phi, con = sim.get_fields(inst=i)
+        ophi, ocon = osim.fields()
+        outs.append(phi)
+        err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
+        print(f"inst {i} seed {seeds[i]} err={err:.3e}", flush=True)
+        assert err < 1e-7
+    assert np.abs(outs[0] - outs[1]).max() > 1e-5  # the two streams differ
+    sim.close()
+
+
 @pytest.mark.parametrize("prec,L,B", [(1, 128, 4), (1, 384, 4), (1, 512, 16), (0, 256, 1), (0, 512, 1)])
 def test_column_tile_widths_bit_identical(hip, prec, L, B):
     """Every tile width the autotuner may pick gives the same bits (each
