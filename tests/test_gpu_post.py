@@ -17,9 +17,11 @@ def _state(hip, kind, prec, L=64, M=64, steps=40, **kw):
     return p, sim
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_phimax_misfit(hip, oracle, prec):
-    p, sim = _state(hip, 0, prec, grainx=4, grainy=4)
+@pytest.mark.parametrize("prec,L,g0", [(0, 64, 4), (1, 64, 4), (0, 512, 16), (1, 512, 16)])
+def test_phimax_misfit(hip, oracle, prec, L, g0):
+    """Atom extraction bit-identical to the oracle on the same fields, also at
+    the benchmark grid (512^2, 16x16 grains)."""
+    p, sim = _state(hip, 0, prec, L=L, M=L, grainx=g0, grainy=g0)
     g = sim.phimax_misfit()
     phi, con = sim.get_fields()
     o = oracle.phimax_misfit(p.L, p.M, 1, p.dx, p.con0, phi, con)
@@ -29,9 +31,11 @@ def test_phimax_misfit(hip, oracle, prec):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_phimax_cvd(hip, oracle, prec):
-    p, sim = _state(hip, 1, prec)
+@pytest.mark.parametrize("prec,L", [(0, 64), (1, 64), (0, 1024), (1, 1024)])
+def test_phimax_cvd(hip, oracle, prec, L):
+    """CVD atoms and the phic/pmax pair bit-identical to the oracle on the
+    same field, also at the checkpoint run's grid (1024^2)."""
+    p, sim = _state(hip, 1, prec, L=L, M=L)
     px, py, pz, pp, phic, pmax = sim.phimax_cvd()
     phi, _ = sim.get_fields()
     o = oracle.phimax_cvd(p.L, p.M, 1, p.dx, p.u0, phi)
