@@ -142,6 +142,30 @@ def test_step_energy_trace(hip, oracle):
         assert abs(tr[t, 0, 1] - orho) < 1e-9
 
 
+@pytest.mark.parametrize("kind,L,prec,rtol", [(1, 1024, 1, 2e-5), (1, 1024, 0, 5e-8), (0, 512, 1, 2e-5)])
+def test_step_energy_trace_full_size(hip, oracle, kind, L, prec, rtol):
+    """The energy trace at the benchmark grids (CVD 1024^2 is the checkpoint
+    run, which traces every step): each traced energy against the oracle's
+    ENERGYC2 of its own state after that step (misfit.cpp:1043-1092).  FP32
+    states differ from the FP64 oracle by O(1e-6) per point, so the energy
+    sum by O(1e-6) relative (measured 2.4e-6 at the first step); FP64 sums
+    of 10^6 terms whose total is small beside them (CVD energy ~ -7e-4 at
+    the first step) measured 8.6e-9 relative, 6e-12 absolute.  The density
+    is a plain mean."""
+    kw = dict(grainx=16, grainy=16) if kind == 0 else {}
+    sim, osim = _pair(hip, oracle, kind, L, L, 1, prec, **kw)
+    tr = sim.step_energy(3)
+    for t in range(3):
+        osim.run(1)
+        oe, orho = osim.energy()
+        rel = abs(tr[t, 0, 0] - oe) / abs(oe)
+        print(f"kind={kind} {L}^2 prec={prec} t={t} energy {tr[t, 0, 0]:.12g} oracle {oe:.12g} rel {rel:.2e}",
+              flush=True)
+        assert rel < rtol
+        assert abs(tr[t, 0, 1] - orho) < (1e-9 if prec == 0 else 1e-6)
+    sim.close()
+
+
 def test_graph_replay_matches_eager(hip):
     p = hip.default_params(model=0, L=64, M=64, precision=hip.FP64, grainx=4, grainy=4)
     phi, con = hip.init_fields(p)
