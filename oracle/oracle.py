@@ -46,6 +46,8 @@ def load():
         lib = ctypes.CDLL(LIB_PATH)
         lib.pfo_splitmix64.restype = ctypes.c_uint64
         lib.pfo_rng_u01.restype = ctypes.c_double
+        lib.pfo_gauss.restype = ctypes.c_uint64
+        lib.pfo_gauss.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)]
         lib.pfo_grain_theta.restype = ctypes.c_float
         lib.pfo_create.restype = ctypes.c_void_p
         lib.pfo_create.argtypes = [ctypes.POINTER(Model)]
@@ -264,3 +266,10 @@ def fft3d(z, L, M, N, sign):
     a = np.ascontiguousarray(z, np.complex128).copy()
     lib.pfo_fft3d(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), L, M, N, sign)
     return a
+
+
+def gauss(seed, timestep, n):
+    """The noise stream of one step (pfo_gauss): (values, uniform pairs drawn)."""
+    out = np.zeros(n, np.float64)
+    pairs = load().pfo_gauss(seed, timestep, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out, int(pairs)

@@ -528,22 +528,35 @@ static void updateinc2gra(pfo_sim* s) {
     }
 }
 
-/* Gauss (misfit.cpp:951-981): Marsaglia polar, sequential splitmix64 stream. */
-static void gauss_noise(pfo_sim* s, int timestep) {
-    uint64_t st = (uint64_t)s->p.seed;
+/* Gauss (misfit.cpp:951-981): Marsaglia polar, sequential splitmix64 stream
+ * seeded from (seed, timestep); n values at out[0], out[stride], ...  Returns
+ * the number of uniform pairs drawn (accepted and rejected). */
+static uint64_t gauss_fill(unsigned int seed, int timestep, size_t n, double* out, size_t stride) {
+    uint64_t st = (uint64_t)seed;
     st ^= (uint64_t)(unsigned int)(0 + 1) * 2654435761ULL;
     st ^= (uint64_t)(unsigned int)timestep * 374761393ULL;
-    for (size_t i = 0; i < s->P; i++) {
+    uint64_t pairs = 0;
+    for (size_t i = 0; i < n; i++) {
         float r = 2.0f, v1 = 0, v2 = 0;
         while (r >= 1.0f || r <= 0.0f) {
             v1 = (float)pfo_rng_u11(&st);
             v2 = (float)pfo_rng_u11(&st);
             r = v1 * v1 + v2 * v2;
+            pairs++;
         }
         float f = (float)sqrt(-2. * (double)logf(r) / r);
-        s->nois[2 * i] = f * v1;
-        s->nois[2 * i + 1] = 0.0;
+        out[i * stride] = f * v1;
     }
+    return pairs;
+}
+
+uint64_t pfo_gauss(unsigned int seed, int timestep, size_t n, double* out) {
+    return gauss_fill(seed, timestep, n, out, 1);
+}
+
+static void gauss_noise(pfo_sim* s, int timestep) {
+    gauss_fill(s->p.seed, timestep, s->P, s->nois, 2);
+    for (size_t i = 0; i < s->P; i++) s->nois[2 * i + 1] = 0.0;
 }
 
 /* One step: run_misfit loop body misfit.cpp:479-498 / run_cvd cvd.cpp:400-419. */

@@ -55,6 +55,12 @@ void pfo_derive(pfo_model* p);
 /* splitmix64 / u01 (pfc_common.h:165-181). */
 uint64_t pfo_splitmix64(uint64_t* state);
 double pfo_rng_u01(uint64_t* state);
+/* The --axx noise stream of one step (misfit.cpp:951-981): n Marsaglia polar
+ * values from the sequential splitmix64 stream of (seed, timestep); returns
+ * the uniform pairs drawn.  Pinned by the moments of N(0,1) and the pi/4
+ * acceptance rate; the (seed, timestep) mixing is parity unpinned (the
+ * reference holds no noise vectors). */
+uint64_t pfo_gauss(unsigned int seed, int timestep, size_t n, double* out);
 /* Grain angle theta (degrees*pi/180 as float) for grain (i,j,k) (misfit.cpp:636-644). */
 float pfo_grain_theta(const pfo_model* p, int iii, int jjj, int kkk);
 

@@ -38,6 +38,28 @@ def test_u01_is_53_bit(oracle):
     assert 0.0 <= u < 1.0
 
 
+def test_gauss_noise_stream(oracle):
+    """The --axx noise stream (Marsaglia polar over splitmix64 uniforms,
+    misfit.cpp:951-981), pinned by what the method fixes: N(0,1) moments
+    (standard errors 1e-3 for the mean, 1.4e-3 for the variance, 1e-2 for the
+    4th moment at 2^20 values), the Kolmogorov distance to the normal CDF, and
+    an acceptance rate of pi/4 (the unit disc in the square).  A dropped -2 or
+    /r in the radius, or a Box-Muller radius, changes the variance."""
+    from scipy.stats import kstest
+    n = 1 << 20
+    x, pairs = oracle.gauss(20240611, 3, n)
+    assert np.all(np.isfinite(x))
+    assert abs(x.mean()) < 5e-3
+    assert abs(x.var() - 1.0) < 7e-3
+    assert abs((x ** 4).mean() - 3.0) < 5e-2
+    assert kstest(x, "norm").statistic < 3e-3
+    assert abs(n / pairs - np.pi / 4) < 3e-3
+    y, _ = oracle.gauss(20240611, 3, 1000)
+    assert np.array_equal(x[:1000], y)  # a step's stream starts afresh from (seed, timestep)
+    for other in (oracle.gauss(20240611, 4, 1000)[0], oracle.gauss(7, 3, 1000)[0]):
+        assert np.abs(other - y).max() > 1.0
+
+
 def test_grain_angles_kat(oracle):
     lib = oracle.load()
     kat = json.load(open(os.path.join(GOLD, "kat.json")))["grain_theta_deg"]
