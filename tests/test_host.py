@@ -120,3 +120,13 @@ def test_server_admission_limits(hip):
     info = srv.stats()
     assert info.jobs_done == 0 and info.jobs_unfinished == 0 and info.budget == 1e6
     srv.close()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message before
+    loading the library or touching a device."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
