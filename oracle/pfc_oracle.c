@@ -289,15 +289,41 @@ static int fft_factor(int n, int* f) {
     return nf;
 }
 
+/* Twiddles exp(sign 2 pi i t/n), t < n, kept per thread for the last few
+ * (n, sign) pairs: the same values on every call, computed once. */
+#define PFO_TW_SLOTS 8
+typedef struct { int n, sign; cpx* tw; cpx* y; } pfo_tw_slot;
+static _Thread_local pfo_tw_slot pfo_tw_cache[PFO_TW_SLOTS];
+static _Thread_local int pfo_tw_next;
+
+static pfo_tw_slot* pfo_twiddles(int n, int sign) {
+    for (int i = 0; i < PFO_TW_SLOTS; i++)
+        if (pfo_tw_cache[i].tw && pfo_tw_cache[i].n == n && pfo_tw_cache[i].sign == sign) return &pfo_tw_cache[i];
+    pfo_tw_slot* s = &pfo_tw_cache[pfo_tw_next];
+    pfo_tw_next = (pfo_tw_next + 1) % PFO_TW_SLOTS;
+    free(s->tw);
+    free(s->y);
+    s->tw = (cpx*)malloc(sizeof(cpx) * n);
+    s->y = (cpx*)malloc(sizeof(cpx) * n);
+    if (!s->tw || !s->y) {
+        fprintf(stderr, "pfo_fft1d: out of memory\n");
+        abort();
+    }
+    s->n = n;
+    s->sign = sign;
+    for (int t = 0; t < n; t++) {
+        double a = PFO_TWO_PI_EXACT * (double)t / (double)n;
+        s->tw[t].re = cos(a); s->tw[t].im = sign * sin(a);
+    }
+    return s;
+}
+
 void pfo_fft1d(double* zd, int n, int sign) {
     if (n <= 1) return;
     cpx* x = (cpx*)zd;
-    cpx* y = (cpx*)malloc(sizeof(cpx) * n);
-    cpx* tw = (cpx*)malloc(sizeof(cpx) * n);
-    for (int t = 0; t < n; t++) {
-        double a = PFO_TWO_PI_EXACT * (double)t / (double)n;
-        tw[t].re = cos(a); tw[t].im = sign * sin(a);
-    }
+    pfo_tw_slot* slot = pfo_twiddles(n, sign);
+    cpx* y = slot->y;  /* work buffer of this (n, sign) */
+    const cpx* tw = slot->tw;
     int f[64];
     int nf = fft_factor(n, f);
     cpx* in = x; cpx* out = y;
@@ -311,13 +337,13 @@ void pfo_fft1d(double* zd, int n, int sign) {
             wr[q].re = cos(a); wr[q].im = sign * sin(a);
         }
         const int NB = n / R;
+        const int tstep = n / (Ns * R);  /* k*r*tstep < n for k < Ns, r < R */
         for (int j = 0; j < NB; j++) {
             const int k = j % Ns;
             cpx v[64];
             for (int r = 0; r < R; r++) {
                 cpx a = in[j + r * NB];
-                const int ti = (int)(((long long)k * r * (n / (Ns * R))) % n);
-                const cpx w = tw[ti];
+                const cpx w = tw[k * r * tstep];
                 v[r].re = a.re * w.re - a.im * w.im;
                 v[r].im = a.re * w.im + a.im * w.re;
             }
@@ -329,7 +355,6 @@ void pfo_fft1d(double* zd, int n, int sign) {
         Ns *= R;
     }
     if (in != x) memcpy(x, in, sizeof(cpx) * n);
-    free(y); free(tw);
 }
 
 void pfo_fft3d(double* zd, int L, int M, int N, int sign) {
@@ -348,14 +373,21 @@ void pfo_fft3d(double* zd, int L, int M, int N, int sign) {
                 pfo_fft1d((double*)line, M, sign);
                 for (int m = 0; m < M; m++) base[(size_t)m * N] = line[m];
             }
-    /* along l (stride M*N) */
-    if (L > 1)
-        for (size_t b = 0; b < (size_t)M * N; b++) {
-            cpx* base = z + b;
-            for (int l = 0; l < L; l++) line[l] = base[(size_t)l * M * N];
-            pfo_fft1d((double*)line, L, sign);
-            for (int l = 0; l < L; l++) base[(size_t)l * M * N] = line[l];
+    /* along l (stride M*N), gathered 8 columns at a time for cache reuse */
+    if (L > 1) {
+        enum { CB = 8 };
+        cpx* blk = (cpx*)malloc(sizeof(cpx) * (size_t)L * CB);
+        const size_t cols = (size_t)M * N;
+        for (size_t b0 = 0; b0 < cols; b0 += CB) {
+            const int nb = (int)(cols - b0 < CB ? cols - b0 : CB);
+            for (int l = 0; l < L; l++)
+                for (int c = 0; c < nb; c++) blk[(size_t)c * L + l] = z[(size_t)l * cols + b0 + c];
+            for (int c = 0; c < nb; c++) pfo_fft1d((double*)(blk + (size_t)c * L), L, sign);
+            for (int l = 0; l < L; l++)
+                for (int c = 0; c < nb; c++) z[(size_t)l * cols + b0 + c] = blk[(size_t)c * L + l];
         }
+        free(blk);
+    }
     free(line);
 }
 
