@@ -404,7 +404,7 @@ int main(int argc, char** argv) {
     av[ac++] = argv[0];
     for (int i = 1; i < argc; i++) {
         if (!strcmp(argv[i], "--model") || !strcmp(argv[i], "-m")) {
-            if (i + 1 >= argc) { fprintf(stderr, "Missing value after --model\n"); return 2; }
+            if (i + 1 >= argc) { fprintf(stderr, "Missing value after --model\n"); free(av); return 2; }
             model = argv[++i];
             continue;
         }
@@ -424,13 +424,14 @@ int main(int argc, char** argv) {
     if (!strcmp(model, "misfit")) kind = PFO_MISFIT;
     else if (!strcmp(model, "cvd")) kind = PFO_CVD;
     else if (!strcmp(model, "elastic")) kind = 2;
-    else { fprintf(stderr, "Unknown model: %s\n", model); return 2; }
+    else { fprintf(stderr, "Unknown model: %s\n", model); free(av); return 2; }
     pfo_defaults(&c.p, kind == 2 ? PFO_MISFIT : kind, L, M, N);
     if (kind == 2) {
         /* elastic.cpp:83-105 defaults */
         c.p.dt = 0.1f; c.p.steps = 2002; c.p.mod = 200; c.p.grainx = 2; c.p.grainy = 2; c.p.grainz = 1;
     }
     int rc = parse_args(&c, ac, av, kind == PFO_MISFIT, kind);
+    free(av);
     if (rc == 1) return 0;
     if (rc != 0) return 2;
     if (c.p.L <= 0 || c.p.M <= 0 || c.p.N <= 0) return 2;
