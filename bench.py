@@ -83,6 +83,16 @@ def build_sweep(hip, graph_chunk):
     return sims
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(budget_s=15.0):
     """Oracle (FP64 CPU restatement, one thread, all 14 reference FFTs per
     step) on Misfit 256^2 with the GUI default flags."""
@@ -99,7 +109,10 @@ def cpu_baseline(budget_s=15.0):
         if el > budget_s or steps >= 200:
             break
     return {"value": 256 * 256 * steps / el / 1e6, "unit": "Mpoint-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle Misfit 256x256, {steps} steps (faithful: 14 c2c FFTs/step), {el:.1f} s, 1 thread"}
+            "sample": f"oracle Misfit 256x256, {steps} steps (faithful: 14 c2c FFTs/step), {el:.1f} s, 1 thread",
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
+            "fft_backend": "the oracle's own FP64 mixed-radix Stockham FFT (radix 4/2/3, cached twiddles); "
+                           "FFTW3, the reference's FFT library, is not installed on this host"}
 
 
 def cpu_baseline_concurrent(nproc, steps=150):
@@ -472,7 +485,10 @@ def main():
                 s.step(n)
             done += n
 
-    sweep(args.warmup, True)
+    # Warm-up: at least one launch of the group graph (its first launch is
+    # slower), then the requested steps.
+    group.step(max(1, args.warmup // chunk))
+    sweep(args.warmup % chunk, True)
     for s in sims.values():
         s.sync()
     t0 = time.perf_counter()
@@ -560,13 +576,14 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
+            "bytes_per_launch": col_bytes,
             "launches_timed": col_n,
             "mean_launch_ms": round(col_ms, 5),
             "timing": "hipExtLaunchKernel start/stop events on every launch of the kernel in a second timed run "
-                      "of the same sweep (per-class streams; graph-captured launches carry no per-launch events)",
+                      "of the same sweep (one graph per class on four streams, the 512^2 class eager): "
+                      "graph-captured launches carry no per-launch events",
             "mean_launch_ms_alone": round(iso_ms, 5),
             "launches_alone": iso_n,
-            "bytes_per_launch": col_bytes,
         },
     }
     traffic_file = os.path.join(ROOT, "profiles", "traffic_k_col_float_512_sweep16.json")
@@ -574,6 +591,18 @@ def main():
         tr = json.load(open(traffic_file))
         out["roofline"]["traffic"] = tr.get("bytes_per_dispatch")
         out["roofline"]["traffic_source"] = "profiles/traffic_k_col_float_512_sweep16.json (rocprofv3 --pmc)"
+    # The kernel's duration inside the timed region (the group graph) from
+    # the committed rocprofv3 kernel trace of this command.
+    trace_file = os.path.join(ROOT, "profiles", "k_col_float_512_sweep16_group.json")
+    if os.path.exists(trace_file):
+        tg = json.load(open(trace_file))
+        ms_g = tg.get("mean_ms")
+        if ms_g:
+            ach_g = col_bytes / (ms_g * 1e-3) / 1e9
+            out["roofline"]["timed_region_rocprof"] = {
+                "mean_launch_ms": ms_g, "achieved": round(ach_g, 1), "frac": round(ach_g / HBM_PEAK_GBS, 4),
+                "source": "profiles/k_col_float_512_sweep16_group.json (rocprofv3 --kernel-trace --stats of "
+                          "bench.py: the kernel's launches inside the group graph)"}
     for s in sims.values():
         s.close()
     if args.accuracy_steps > 0:

@@ -168,16 +168,6 @@ int pfc_set_graph(pfc_ctx* ctx, int chunk);
 int pfc_time_kernel(pfc_ctx* ctx, int kernel, int enable);
 int pfc_kernel_time(pfc_ctx* ctx, double* mean_ms, int* count);
 
-/* Self-timing of the row (kernel 0) and column (kernel 1) passes, no
- * counterpart in the reference: while enabled, the workgroups of every launch
- * record the first start and the last end on the device's constant-rate
- * clock, inside captured graphs too (where launches carry no events).
- * Enabling resets the sums and recaptures the context's graph; a group must be
- * created after it.  pfc_self_time reports the mean launch duration and the
- * number of launches since enabling. */
-int pfc_set_self_timing(pfc_ctx* ctx, int enable);
-int pfc_self_time(pfc_ctx* ctx, int kernel, double* mean_ms, long long* launches);
-
 /* Algorithmic HBM bytes one launch of kernel `kernel` moves (all instances). */
 int pfc_kernel_bytes(pfc_ctx* ctx, int kernel, double* bytes);
 

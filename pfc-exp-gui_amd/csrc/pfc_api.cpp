@@ -72,8 +72,6 @@ struct pfc_ctx {
     int time_kid = -1;
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
-    unsigned long long* kstamp_d = nullptr;  // self-timing slots (DevBufs::kstamp), made when first enabled
-    int wclk_khz = 0;                        // the device's constant-rate clock
 };
 
 extern "C" const char* pfc_last_error(void) { return g_err; }
@@ -171,8 +169,7 @@ static void free_ctx(pfc_ctx* c) {
     for (auto e : c->ev) hipEventDestroy(e);
     void* ptrs[] = {c->S, c->AB, c->IN, c->means_d, c->twH, c->twM, c->twL, c->twN, c->kl_d, c->km_d, c->kn_d,
                     c->c2_d, c->active_d, c->erow, c->ecol[0], c->ecol[1], c->eout, c->stage_a, c->stage_b,
-                    c->tcount, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr, c->coefA, c->coefC,
-                    c->kstamp_d};
+                    c->tcount, c->seeds_d, c->nplanes, c->nz, c->nscratch, c->nerr, c->coefA, c->coefC};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->host_stage) hipHostFree(c->host_stage);
@@ -821,42 +818,6 @@ extern "C" int pfc_kernel_time(pfc_ctx* c, double* mean_ms, int* count) {
     }
     *mean_ms = n ? tot / n : 0.0;
     *count = n;
-    return PFC_OK;
-}
-
-extern "C" int pfc_set_self_timing(pfc_ctx* c, int enable) {
-    if (!c) return fail(PFC_E_ARG, "bad arguments");
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (enable) {
-        if (!c->kstamp_d) {
-            int dev = 0;
-            HIP_TRY(hipGetDevice(&dev));
-            HIP_TRY(hipDeviceGetAttribute(&c->wclk_khz, hipDeviceAttributeWallClockRate, dev));
-            if (c->wclk_khz <= 0) return fail(PFC_E_UNSUPPORTED, "no constant-rate device clock");
-            HIP_TRY(hipMalloc(&c->kstamp_d, sizeof(unsigned long long) * 2 * KSTAMP_SLOTS));
-        }
-        unsigned long long init[2 * KSTAMP_SLOTS] = {};
-        init[KSTAMP_ROW] = ~0ull;
-        init[KSTAMP_COL] = ~0ull;
-        HIP_TRY(hipMemcpy(c->kstamp_d, init, sizeof init, hipMemcpyHostToDevice));
-        c->bufs.kstamp = c->kstamp_d;
-    } else {
-        c->bufs.kstamp = nullptr;
-    }
-    return pfc_set_graph(c, c->graph_chunk);  // recapture with the new kernel arguments
-}
-
-extern "C" int pfc_self_time(pfc_ctx* c, int kernel, double* mean_ms, long long* launches) {
-    if (!c || !mean_ms || !launches || (kernel != KID_ROW && kernel != KID_COL)) return fail(PFC_E_ARG, "bad arguments");
-    *mean_ms = 0.0;
-    *launches = 0;
-    if (!c->kstamp_d) return PFC_OK;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    unsigned long long h[2 * KSTAMP_SLOTS];
-    HIP_TRY(hipMemcpy(h, c->kstamp_d, sizeof h, hipMemcpyDeviceToHost));
-    const unsigned long long* k = h + (kernel == KID_ROW ? KSTAMP_ROW : KSTAMP_COL);
-    *launches = (long long)k[4];
-    if (k[4]) *mean_ms = (double)k[3] / (double)k[4] / (double)c->wclk_khz;  // ticks / kHz = ms
     return PFC_OK;
 }
 

@@ -66,16 +66,7 @@ struct DevBufs {
     const double* eprev = nullptr;
     double* etrace = nullptr;
     int nrow = 0;  // row-pass workgroups per instance (erow entries)
-    // Self-timing (nullptr: off): the row pass uses kstamp[0..7], the column
-    // pass kstamp[8..15] = {first workgroup start, last workgroup end,
-    // workgroups finished, sum of launch durations, launches} in ticks of
-    // the device's constant-rate clock; valid inside captured graphs.
-    unsigned long long* kstamp = nullptr;
 };
-
-// kstamp slots per timed kernel class and their initial values.
-constexpr int KSTAMP_SLOTS = 8;
-constexpr int KSTAMP_ROW = 0, KSTAMP_COL = 8;
 
 enum { ROW_WRITE_IN = 1, ROW_ENERGY = 2 };
 enum { COL_INIT = 0, COL_STEP = 1, COL_ENERGY_ONLY = 2 };

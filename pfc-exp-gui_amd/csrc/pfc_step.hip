@@ -233,34 +233,6 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return tot;
 }
 
-// Self-timing of one launch (DevBufs::kstamp): every workgroup takes the
-// constant-rate clock when it starts (atomic min) and, after its stores have
-// completed, when it ends (atomic max); the last workgroup to finish adds
-// end - start to the running sum, counts the launch and resets the slots for
-// the next launch of the kernel (launches of one kernel class on one context
-// are ordered, in a stream or in a captured graph).  Off (nullptr) it costs
-// one uniform branch at each end.
-__device__ __forceinline__ void stamp_begin(unsigned long long* ks) {
-    if (ks && threadIdx.x == 0) atomicMin(&ks[0], (unsigned long long)wall_clock64());
-}
-__device__ __forceinline__ void stamp_end(unsigned long long* ks) {
-    if (!ks) return;  // uniform
-    __threadfence();  // this wave's stores have completed
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    atomicMax(&ks[1], (unsigned long long)wall_clock64());
-    __threadfence();
-    const unsigned long long nwg = (unsigned long long)gridDim.x * gridDim.y * gridDim.z;
-    if (atomicAdd(&ks[2], 1ull) != nwg - 1) return;
-    __threadfence();
-    const unsigned long long t0 = atomicAdd(&ks[0], 0ull), t1 = atomicAdd(&ks[1], 0ull);
-    atomicAdd(&ks[3], t1 - t0);
-    atomicAdd(&ks[4], 1ull);
-    atomicExch(&ks[0], ~0ull);
-    atomicExch(&ks[1], 0ull);
-    atomicExch(&ks[2], 0ull);
-}
-
 // ---------------------------------------------------------------------------
 // Launch shapes
 // ---------------------------------------------------------------------------
@@ -488,10 +460,12 @@ __device__ __forceinline__ Cx<R> c2r_split_packed(Cx<R> xk, Cx<R> xc, int k, Cx<
 // Row pass: S, AB -> IN (+ per-row energy partials)
 // ---------------------------------------------------------------------------
 template <typename R, int H>
-__device__ __forceinline__ void row_step_body(const DevBufs& b, const PhysArgs& p, int flags, char* smem,
-                                              double* red) {
+__global__ void __launch_bounds__((RowCfg<R, H>::NT)) __attribute__((amdgpu_waves_per_eu(RowCfg<R, H>::WPE)))
+k_row_step(DevBufs b, PhysArgs p, int flags) {
     using C = RowCfg<R, H>;
     constexpr int E = C::E, G = C::G, K = C::K;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    __shared__ double red[(C::NT + 63) / 64 + 1];
     const int tid = threadIdx.x, rl = tid / G, t = tid % G;
     const int inst = blockIdx.y;
     if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
@@ -555,18 +529,6 @@ __device__ __forceinline__ void row_step_body(const DevBufs& b, const PhysArgs& 
             b.erow[slot] = se;
         }
     }
-}
-
-template <typename R, int H>
-__global__ void __launch_bounds__((RowCfg<R, H>::NT)) __attribute__((amdgpu_waves_per_eu(RowCfg<R, H>::WPE)))
-k_row_step(DevBufs b, PhysArgs p, int flags) {
-    using C = RowCfg<R, H>;
-    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
-    __shared__ double red[(C::NT + 63) / 64 + 1];
-    unsigned long long* ks = b.kstamp ? b.kstamp + KSTAMP_ROW : nullptr;
-    stamp_begin(ks);
-    row_step_body<R, H>(b, p, flags, smem, red);
-    stamp_end(ks);
 }
 
 // Real planes (dpsi, dcon) -> S.
@@ -837,11 +799,14 @@ __device__ void energy_reduce_block(const DevBufs& b, const PhysArgs& p, int ins
 // col_coef holds.
 enum { COL_GENERAL = 0, COL_LEAN = 1, COL_LEAN_E = 2 };
 template <typename R, int LL, int KIND, int TS>
-__device__ __forceinline__ void col_body(const DevBufs& b, const PhysArgs& p, int mode, int flags, char* smem,
-                                         double* red) {
+__global__ void __launch_bounds__((ColCfg<R, LL, TS>::NT))
+__attribute__((amdgpu_waves_per_eu(KIND != COL_GENERAL ? ColCfg<R, LL, TS>::WPE_LEAN : ColCfg<R, LL, TS>::WPE)))
+k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     constexpr bool LEAN = KIND != COL_GENERAL;
     using C = ColCfg<R, LL, TS>;
     constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT, EA = C::EA;
+    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    __shared__ double red[(NT + 63) / 64 + 1];
     const int tid = threadIdx.x;
     const int col = C::WAVE ? tid / G : tid % T, t = C::WAVE ? tid % G : tid / T;
     const int inst = blockIdx.y;
@@ -956,19 +921,6 @@ __device__ __forceinline__ void col_body(const DevBufs& b, const PhysArgs& p, in
         bstore(sv, S, toff, qo);
         bstore(av, AB, toff, qo);
     }
-}
-
-template <typename R, int LL, int KIND, int TS>
-__global__ void __launch_bounds__((ColCfg<R, LL, TS>::NT))
-__attribute__((amdgpu_waves_per_eu(KIND != COL_GENERAL ? ColCfg<R, LL, TS>::WPE_LEAN : ColCfg<R, LL, TS>::WPE)))
-k_col(DevBufs b, PhysArgs p, int mode, int flags) {
-    using C = ColCfg<R, LL, TS>;
-    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
-    __shared__ double red[(C::NT + 63) / 64 + 1];
-    unsigned long long* ks = b.kstamp ? b.kstamp + KSTAMP_COL : nullptr;
-    stamp_begin(ks);
-    col_body<R, LL, KIND, TS>(b, p, mode, flags, smem, red);
-    stamp_end(ks);
 }
 
 // Per-mode coefficients of the plain FP32 step in the column layout: entry

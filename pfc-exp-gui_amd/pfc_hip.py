@@ -25,8 +25,7 @@ SYMBOLS = [
     "pfc_step", "pfc_sync", "pfc_get_fields", "pfc_get_fields_f32", "pfc_energy",
     "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic",
     "pfc_strain", "pfc_cryst_order", "pfc_cvd_gr", "pfc_cvd_q6", "pfc_col_tiles", "pfc_set_col_tiles", "pfc_group_create", "pfc_group_step",
-    "pfc_group_destroy", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_set_self_timing",
-    "pfc_self_time", "pfc_kernel_bytes",
+    "pfc_group_destroy", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
     "pfc_device_memory", "pfc_copy_probe", "pfc_last_error",
     "pfc_server_create", "pfc_server_submit", "pfc_server_poll", "pfc_server_result", "pfc_server_error",
     "pfc_server_cancel", "pfc_server_drain", "pfc_server_stats", "pfc_server_destroy",
@@ -121,8 +120,6 @@ def load():
         "pfc_set_graph": [vp, ctypes.c_int],
         "pfc_time_kernel": [vp, ctypes.c_int, ctypes.c_int],
         "pfc_kernel_time": [vp, dp, ip],
-        "pfc_set_self_timing": [vp, ctypes.c_int],
-        "pfc_self_time": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
         "pfc_kernel_bytes": [vp, ctypes.c_int, dp],
         "pfc_device_memory": [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
         "pfc_copy_probe": [ctypes.c_size_t, ctypes.c_int, dp],
@@ -264,17 +261,6 @@ class Sim:
         ms = ctypes.c_double()
         n = ctypes.c_int()
         _check(self.lib.pfc_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)))
-        return ms.value, n.value
-
-    def set_self_timing(self, enable=True):
-        """Row/column passes time themselves on the device clock (valid in graphs)."""
-        _check(self.lib.pfc_set_self_timing(self.h, 1 if enable else 0))
-
-    def self_time(self, kid):
-        """(mean launch ms, launches) of the row (KID_ROW) or column (KID_COL) pass."""
-        ms = ctypes.c_double()
-        n = ctypes.c_longlong()
-        _check(self.lib.pfc_self_time(self.h, kid, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
     def kernel_bytes(self, kid):

@@ -66,6 +66,44 @@ def test_misfit_cli_matches_oracle(tmp_path):
     assert ck == ["25", "50"]
 
 
+def test_gui_default_job_matches_oracle(tmp_path):
+    """The GUI's default job (MainWindow.cpp:2401-2428 argv with its default
+    values): Misfit 256^2, 200 steps, a checkpoint every 25, seed 20200604,
+    16 x 16 grains, the FP64 drop-in, against the oracle CLI: the same file
+    set, run_config / PVTI / initial VTI and every VTI header byte-identical,
+    payloads (float32) within FP64-narrowing round-off, Phimax counts equal
+    and rows within the printed 4 decimals, energy.txt within its 6 digits."""
+    args = ["--u0", "0.05", "--con0", "0.2", "--sig", "0.05", "--dt", "0.05", "--dx", "0.125", "--steps", "200",
+            "--mod", "25", "--seed", "20200604"]
+    g, o = tmp_path / "gpu", tmp_path / "cpu"
+    r = run(CLI, ["--model", "misfit"] + args + ["--outdir", str(g)], tmp_path)
+    run(OCLI, ["--model", "misfit"] + args + ["--outdir", str(o)], tmp_path)
+    steps = [int(PROGRESS.match(l).group(1)) for l in r.stdout.splitlines() if PROGRESS.match(l)]
+    assert steps[0] == 1 and steps[-1] == 200 and len(steps) == 200
+    assert sorted(os.listdir(g)) == sorted(os.listdir(o))
+    for f in ("run_config.txt", "phi_0_0.vti", "con_0_0.vti", "phi_0.pvti", "phi_100.pvti", "con_200.pvti"):
+        assert (g / f).read_bytes() == (o / f).read_bytes(), f
+    worst = 0.0
+    for kd in range(25, 201, 25):
+        for fld in ("phi", "con"):
+            hg, pg, tg, ng = vti_split(g / f"{fld}_{kd}_0.vti")
+            ho, po, to, no = vti_split(o / f"{fld}_{kd}_0.vti")
+            assert hg == ho and tg == to and ng == no
+            worst = max(worst, float(np.abs(pg - po).max()))
+        cg, _, ag = read_table(g / f"Phimax_{kd}.txt")
+        co, _, ao = read_table(o / f"Phimax_{kd}.txt")
+        assert cg == co, kd
+        assert np.abs(ag - ao).max() <= 2e-4, kd
+    print(f"GUI default job: max |VTI payload GPU - oracle| = {worst:.3e}", flush=True)
+    assert worst < 1e-5
+    eg = np.loadtxt(g / "energy.txt")
+    eo = np.loadtxt(o / "energy.txt")
+    assert np.array_equal(eg[:, 0], eo[:, 0]) and list(eg[:, 0]) == list(range(25, 201, 25))
+    assert np.allclose(eg[:, 1:], eo[:, 1:], rtol=2e-5, atol=1e-9)
+    ck = [l.split()[1] for l in (g / "checkpoint_timestamps.txt").read_text().splitlines()[1:]]
+    assert ck == [str(k) for k in range(25, 201, 25)]
+
+
 def test_cvd_cli_matches_oracle(tmp_path):
     args = ["--steps", "21", "--mod", "10", "--L", "64", "--M", "64", "--u0", "0.05", "--con0", "0.25"]
     g, o = tmp_path / "gpu", tmp_path / "cpu"

@@ -68,6 +68,58 @@ def test_full_size_vs_oracle(hip, oracle, kind, L, n, prec, narrow, tol):
     sim.close()
 
 
+@pytest.mark.parametrize("kind,prec,narrow,tol", [
+    (0, 0, 0, 1e-11),   # Misfit 256^2 FP64, exact math
+    (0, 0, 1, 1e-6),    # ... FP64 with the reference narrowing (the GUI default job's path)
+    (1, 0, 0, 1e-11),   # CVD 256^2 FP64, exact math
+    (1, 1, 1, 2e-5),    # CVD 256^2 FP32
+])
+def test_256_lengths_vs_oracle(hip, oracle, kind, prec, narrow, tol):
+    """256-point columns (k_col<*,256>) and 128-element row spectra
+    (k_row_step<*,128>): the GUI's default grid (misfit.cpp defaults, 16 x 16
+    grains) against the oracle, 20 steps in two calls."""
+    kw = dict(grainx=16, grainy=16) if kind == 0 else {}
+    sim, osim = _pair(hip, oracle, kind, 256, 256, narrow, prec, **kw)
+    for n in (7, 13):
+        sim.step(n)
+        osim.run(n)
+        phi, con = sim.get_fields()
+        ophi, ocon = osim.fields()
+        err = max(np.abs(phi - ophi).max(), np.abs(con - ocon).max())
+        print(f"kind={kind} 256^2 prec={prec} narrow={narrow} t={osim.t} err={err:.3e}", flush=True)
+        assert err < tol
+    sim.close()
+
+
+def test_fp32_sweep_class_256_vs_oracle(hip, oracle):
+    """The sweep's 256^2 class (16 FP32 instances: u0 x con0, seed 20200604 +
+    1, as the benchmark builds it) against 16 oracle runs of the same fields,
+    3 steps; each instance is checked, so a batch-index mistake fails."""
+    insts = []
+    for u0 in (0.02, 0.04, 0.06, 0.08):
+        for con0 in (0.1, 0.2, 0.3, 0.4):
+            q = hip.default_params(model=0, L=256, M=256, precision=hip.FP32, u0=u0, con0=con0, seed=20200605)
+            insts.append((q, hip.init_fields(q)))
+    p = hip.default_params(model=0, L=256, M=256, precision=hip.FP32)
+    b = hip.Sim(p, batch=16)
+    for k, (_, (phi, con)) in enumerate(insts):
+        b.set_fields(phi, con, inst=k)
+    b.set_graph(3)
+    b.step(3)
+    worst = 0.0
+    for k, (q, (phi, con)) in enumerate(insts):
+        m = oracle.model(0, 256, 256, 1, u0=q.u0, con0=q.con0, seed=q.seed)
+        osim = oracle.Sim(m, phi, con)
+        osim.run(3)
+        ophi, ocon = osim.fields()
+        x = b.get_fields(k)
+        err = max(np.abs(x[0] - ophi).max(), np.abs(x[1] - ocon).max())
+        worst = max(worst, err)
+        assert err < 1e-5, (k, err)
+    print(f"16 x 256^2 FP32 vs oracle after 3 steps: worst {worst:.3e}", flush=True)
+    b.close()
+
+
 @pytest.mark.parametrize("kind,L,M,prec,tol", [
     (0, 768, 64, 0, 1e-11), (1, 64, 768, 0, 1e-11),   # FP64 exact math: 768-point columns / rows
     (0, 768, 64, 1, 2e-4), (1, 64, 768, 1, 2e-4),     # FP32
@@ -394,4 +446,32 @@ def test_group_graph_equals_separate(hip):
             assert np.array_equal(x[0], w[i][0]) and np.array_equal(x[1], w[i][1])
     g.close()
     for s in ref + grp_sims:
+        s.close()
+
+
+def test_benchmark_sweep_group_equals_per_class(hip):
+    """The benchmark's workload itself: the 64-instance sweep (16 each of
+    128^2, 256^2, 384^2, 512^2, FP32) stepped as one group graph of its four
+    classes (what bench.py times) gives, for every instance, the same bits as
+    each class stepped eagerly on its own."""
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("pfc_bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    grouped = bench.build_sweep(hip, 25)
+    g = hip.Group(list(grouped.values()), 25)
+    g.step(2)
+    g.sync()
+    alone = bench.build_sweep(hip, 0)
+    for s in alone.values():
+        s.step(50)
+        s.sync()
+    for L in (128, 256, 384, 512):
+        for i in range(16):
+            x, y = grouped[L].get_fields(i), alone[L].get_fields(i)
+            assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), (L, i)
+    g.close()
+    for s in list(grouped.values()) + list(alone.values()):
         s.close()
