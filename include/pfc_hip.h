@@ -112,6 +112,20 @@ int pfc_phimax_cvd(pfc_ctx* ctx, int inst, float* pc_x, float* pc_y, float* pc_z
  * phimax_atmposi of elastic.cpp:2802-2921: x,y,z,phi doubles. */
 int pfc_phimax_elastic(int L, int M, int N, float dx, const float* phi, double* x, double* y, double* z,
                        double* phimax, int cap, int* n);
+/* No counterpart in the reference: pfc_phimax_elastic for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  phi (L*M*N floats), x, y, z, phimax (cap doubles each) and n (one
+ * int) are DEVICE pointers; the work is enqueued on `stream` (a hipStream_t;
+ * NULL = the default stream) and the call returns without waiting.  *n
+ * receives the number of atoms found, which may exceed cap (then only the
+ * first cap are written).  The library's scratch for it is a fixed device
+ * array, one per GPU for the life of the process, sized for fields of at most
+ * PFC_NOALLOC_MAX_POINTS points (larger fields: PFC_E_UNSUPPORTED); calls that
+ * overlap on one GPU share it and race.  Same results as pfc_phimax_elastic. */
+#define PFC_NOALLOC_MAX_POINTS (1L << 24)
+int pfc_phimax_elastic_noalloc(int L, int M, int N, float dx, const float* phi, double* x, double* y, double* z,
+                               double* phimax, int cap, int* n, void* stream);
+
 /* elastic3D (elastic.cpp:3586-3729): |strain| and neighbour count per atom. */
 int pfc_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
                const double* z, double* strain, unsigned int* nb);

@@ -879,6 +879,19 @@ extern "C" int pfc_phimax_elastic(int L, int M, int N, float dx, const float* ph
     return PFC_OK;
 }
 
+extern "C" int pfc_phimax_elastic_noalloc(int L, int M, int N, float dx, const float* phi, double* x, double* y,
+                                          double* z, double* phimax, int cap, int* n, void* stream) {
+    if (L <= 0 || M <= 0 || N <= 0 || !phi || !n || cap < 0 || (cap > 0 && (!x || !y || !z || !phimax)))
+        return fail(PFC_E_ARG, "bad arguments");
+    const long P = (long)L * M * N;
+    if (P > NOALLOC_MAX_POINTS)
+        return fail(PFC_E_UNSUPPORTED, "field of %ld points exceeds the allocation-free scratch (at most %ld points)", P,
+                    NOALLOC_MAX_POINTS);
+    hipError_t e = post_phimax_elastic_noalloc(L, M, N, dx, phi, x, y, z, phimax, cap, n, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(PFC_E_HIP, "phimax_elastic_noalloc: %s", hipGetErrorString(e));
+    return PFC_OK;
+}
+
 extern "C" int pfc_cryst_order(int L, int M, int N, float dx, int n, const double* x, const double* y,
                                const double* z, double* q4, double* q6, unsigned int* nb) {
     if (L <= 0 || M <= 0 || N <= 0 || n < 0 || (n > 0 && (!x || !y || !z || !q4 || !q6)))

@@ -20,6 +20,12 @@ hipError_t post_phimax_cvd(int prec, int L, int M, int N, float dx, float u0, co
 // Elastic phimax_atmposi on a host float field.
 hipError_t post_phimax_elastic(int L, int M, int N, float dx, const float* phi, double* x, double* y, double* z,
                                double* ph, int cap, PostResult* r);
+// Elastic phimax_atmposi on a device float field into device outputs, on
+// stream s, without allocating: scratch is a file-scope device array sized
+// for at most NOALLOC_MAX_POINTS points; *n_out (device) receives the count.
+constexpr long NOALLOC_MAX_POINTS = 1L << 24;
+hipError_t post_phimax_elastic_noalloc(int L, int M, int N, float dx, const float* phi, double* x, double* y,
+                                       double* z, double* ph, int cap, int* n_out, hipStream_t s);
 // elastic3D on host atom positions.
 hipError_t post_strain(int L, int M, int N, float dx, float benchel, int n, const double* x, const double* y,
                        const double* z, double* strain, unsigned* nb);

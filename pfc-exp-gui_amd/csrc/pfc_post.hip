@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -1320,6 +1321,121 @@ __global__ void __launch_bounds__(256) k_copy16_cols(const uint4* __restrict__ s
     const size_t col = (size_t)bx * 4 + threadIdx.x % 4;
     for (size_t r = (size_t)blockIdx.y * 64 + threadIdx.x / 4; r < rows; r += (size_t)gridDim.y * 64)
         dst[r * 256 + col] = src[r * 256 + col];
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free elastic Phimax (pfc_phimax_elastic_noalloc) on a field the
+// caller holds in device memory, with device outputs, on the caller's stream.
+// Its scratch (the compaction's block counts and offsets, the threshold's
+// partial maxima) is the file-scope array below; the library neither
+// allocates nor frees device memory for it, so callers that cannot (graph
+// capture, serving loops) can use it.  Every GPU holds its own copy for the
+// life of the process and every call on that GPU uses it: calls that overlap
+// on one GPU race.  Each kernel writes every scratch word it later reads, so
+// nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_BLOCKS = (int)(NOALLOC_MAX_POINTS / PB);  // compaction blocks of the largest field
+constexpr int NOALLOC_PARTS = 256;                                // partial maxima
+constexpr size_t NOALLOC_BYTES = ((size_t)(2 * NOALLOC_BLOCKS + 1) * sizeof(int) + (NOALLOC_PARTS + 1) * sizeof(float) +
+                                  255) / 256 * 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// Device address of this GPU's `scratch`, looked up once per device with
+// that device current (the array's name is a host-side stand-in).
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[64];
+static hipError_t scratch_address(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
+// elastic.cpp:2809-2816: float running max from 0.  NaN never compares
+// greater and max is exact, so partial maxima in any order give the same bits.
+__global__ void __launch_bounds__(256) k_fmax_part(const float* __restrict__ phi, long P, float* part) {
+    __shared__ float red[256];
+    float m = 0.0f;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < P; i += (long)gridDim.x * 256) {
+        const float v = phi[i];
+        if (v > m) m = v;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s && red[threadIdx.x + s] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// The threshold phic = pmax - pmax * 0.40 (elastic.cpp:2817) from the partials.
+__global__ void __launch_bounds__(256) k_elastic_phic(const float* __restrict__ part, int n, float* phic) {
+#pragma clang fp contract(off)
+    __shared__ float red[256];
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < n; i += 256)
+        if (part[i] > m) m = part[i];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s && red[threadIdx.x + s] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float pmax = red[0];
+        *phic = pmax - pmax * 0.40f;
+    }
+}
+
+// ElasticOp with its threshold read from device memory.
+struct ElasticDevOp {
+    ElasticOp op;
+    const float* phic;
+    __device__ bool test(long p) const {
+        ElasticOp o = op;
+        o.phic = *phic;
+        return o.test(p);
+    }
+    __device__ void emit(long p, size_t o) const { op.emit(p, o); }
+};
+
+__global__ void k_store_total(const int* __restrict__ offs, int nb, int* n_out) {
+    if (threadIdx.x == 0) *n_out = offs[nb];
+}
+
+hipError_t post_phimax_elastic_noalloc(int L, int M, int N, float dx, const float* phi, double* x, double* y,
+                                       double* z, double* ph, int cap, int* n_out, hipStream_t s) {
+    const long P = (long)L * M * N;
+    if (P <= 0 || P > NOALLOC_MAX_POINTS) return hipErrorInvalidValue;
+    void* base = nullptr;
+    hipError_t e = scratch_address(&base);
+    if (e != hipSuccess) return e;
+    int* counts = (int*)base;
+    int* offs = counts + NOALLOC_BLOCKS;
+    float* part = (float*)(offs + NOALLOC_BLOCKS + 1);
+    float* phic = part + NOALLOC_PARTS;
+    const int nb = (int)((P + PB - 1) / PB);
+    const int np = nb < NOALLOC_PARTS ? nb : NOALLOC_PARTS;
+    hipLaunchKernelGGL(k_fmax_part, dim3(np), dim3(256), 0, s, phi, P, part);
+    hipLaunchKernelGGL(k_elastic_phic, dim3(1), dim3(256), 0, s, (const float*)part, np, phic);
+    ElasticDevOp op;
+    op.op.phi = phi; op.op.L = L; op.op.M = M; op.op.N = N; op.op.dx = dx; op.op.phic = 0.0f;
+    op.op.x = x; op.op.y = y; op.op.z = z; op.op.ph = ph;
+    op.phic = phic;
+    hipLaunchKernelGGL(k_count<ElasticDevOp>, dim3(nb), dim3(PB), 0, s, op, P, counts);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const int*)counts, nb, offs);
+    hipLaunchKernelGGL(k_emit<ElasticDevOp>, dim3(nb), dim3(PB), 0, s, op, P, (const int*)offs, cap);
+    hipLaunchKernelGGL(k_store_total, dim3(1), dim3(64), 0, s, (const int*)offs, nb, n_out);
+    return hipGetLastError();
 }
 
 hipError_t launch_copy16(const void* src, void* dst, size_t n, int pattern, hipStream_t s) {

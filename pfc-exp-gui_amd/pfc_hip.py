@@ -23,7 +23,7 @@ SYMBOLS = [
     "pfc_params_default", "pfc_init_fields", "pfc_create", "pfc_destroy", "pfc_set_fields", "pfc_set_seed",
     "pfc_set_active",
     "pfc_step", "pfc_sync", "pfc_get_fields", "pfc_get_fields_f32", "pfc_energy",
-    "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic",
+    "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic", "pfc_phimax_elastic_noalloc",
     "pfc_strain", "pfc_cryst_order", "pfc_cvd_gr", "pfc_cvd_q6", "pfc_col_tiles", "pfc_set_col_tiles", "pfc_group_create", "pfc_group_step",
     "pfc_group_destroy", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
     "pfc_device_memory", "pfc_copy_probe", "pfc_last_error",
@@ -103,6 +103,7 @@ def load():
         "pfc_phimax_misfit": [vp, ctypes.c_int, dp, dp, dp, dp, fp, ctypes.c_int, ip],
         "pfc_phimax_cvd": [vp, ctypes.c_int, fp, fp, fp, fp, ctypes.c_int, ip, fp, fp],
         "pfc_phimax_elastic": [ctypes.c_int] * 3 + [ctypes.c_float, fp, dp, dp, dp, dp, ctypes.c_int, ip],
+        "pfc_phimax_elastic_noalloc": [ctypes.c_int] * 3 + [ctypes.c_float, vp, vp, vp, vp, vp, ctypes.c_int, vp, vp],
         "pfc_strain": [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_float, ctypes.c_int, dp, dp, dp, dp,
                                             ctypes.POINTER(ctypes.c_uint)],
         "pfc_cryst_order": [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int, dp, dp, dp, dp, dp,
