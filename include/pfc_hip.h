@@ -95,6 +95,17 @@ int pfc_energy(pfc_ctx* ctx, double* e_out);
 /* nsteps steps with ENERGYC2 after every step (run_cvd, cvd.cpp:442), fused
  * into the step kernels.  trace: nsteps*batch*2 doubles, step-major. */
 int pfc_step_energy(pfc_ctx* ctx, int nsteps, double* trace);
+/* The same in two halves (no counterpart in the reference, which steps one
+ * run synchronously): _enqueue queues the steps and the trace's copy to a
+ * host buffer of the context and returns; _result waits for the context and
+ * copies the trace of the last enqueued call (same nsteps) to `trace`. */
+int pfc_step_energy_enqueue(pfc_ctx* ctx, int nsteps);
+int pfc_step_energy_result(pfc_ctx* ctx, int nsteps, double* trace);
+
+/* *done = 1 when all work queued on the context has finished, else 0; does
+ * not wait (no counterpart in the reference: lets a caller step several
+ * contexts and handle whichever finishes first). */
+int pfc_query(pfc_ctx* ctx, int* done);
 
 /* Phimax of the current state of instance `inst`.
  * Misfit phimax_atmposi (misfit.cpp:3261-3464): x,y,z,phi doubles, con float;
@@ -236,8 +247,11 @@ typedef struct pfc_job {
 typedef struct pfc_job_status {
     int state;              /* PFC_JOB_* */
     int steps_done;
-    double t_submit, t_start, t_end; /* seconds on the server clock; t_end: files written */
+    double t_submit, t_start, t_end; /* seconds on the server clock: arrival, admission into a slot,
+                                        final files written */
     double energy, density;          /* ENERGYC2 of the final state (DONE) */
+    double t_stepped;                /* the last step done (before the final files), or -1 */
+    double create_s;                 /* seconds of context creation the job's admission waited for */
 } pfc_job_status;
 
 typedef struct pfc_server_info {

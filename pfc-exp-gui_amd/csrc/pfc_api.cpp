@@ -68,6 +68,11 @@ struct pfc_ctx {
     int graph_chunk = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
+    // energy trace of pfc_step_energy_enqueue: device entries, pinned host copy
+    double* trace_d = nullptr;
+    double* trace_h = nullptr;
+    size_t trace_cap = 0;  // doubles in each
+    int trace_n = 0;       // steps of the last enqueued trace
     // timing
     int time_kid = -1;
     std::vector<hipEvent_t> ev;
@@ -173,6 +178,8 @@ static void free_ctx(pfc_ctx* c) {
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->host_stage) hipHostFree(c->host_stage);
+    if (c->trace_d) hipFree(c->trace_d);
+    if (c->trace_h) hipHostFree(c->trace_h);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -768,11 +775,23 @@ extern "C" int pfc_energy(pfc_ctx* c, double* e_out) {
     return PFC_OK;
 }
 
-extern "C" int pfc_step_energy(pfc_ctx* c, int nsteps, double* trace) {
-    if (!c || !trace || nsteps < 0) return fail(PFC_E_ARG, "bad arguments");
+extern "C" int pfc_step_energy_enqueue(pfc_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return fail(PFC_E_ARG, "bad arguments");
+    c->trace_n = nsteps;
     if (nsteps == 0) return PFC_OK;
-    double* dtr = nullptr;
-    HIP_TRY(hipMalloc(&dtr, sizeof(double) * 2 * c->B * (size_t)nsteps));
+    const size_t need = 2 * (size_t)c->B * (size_t)nsteps;
+    if (need > c->trace_cap) {
+        // Grown rarely (the buffers are kept): freeing waits for the device.
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->trace_d) { hipFree(c->trace_d); c->trace_d = nullptr; }
+        if (c->trace_h) { hipHostFree(c->trace_h); c->trace_h = nullptr; }
+        c->trace_cap = 0;
+        size_t cap = need < 4096 ? 4096 : need;
+        HIP_TRY(hipMalloc(&c->trace_d, sizeof(double) * cap));
+        HIP_TRY(hipHostMalloc(&c->trace_h, sizeof(double) * cap, hipHostMallocDefault));
+        c->trace_cap = cap;
+    }
+    double* dtr = c->trace_d;
     hipError_t e = hipSuccess;
     for (int i = 0; i < nsteps && e == hipSuccess; i++) {
         // Row pass of step i also sums the pointwise energy of the state it
@@ -790,11 +809,30 @@ extern "C" int pfc_step_energy(pfc_ctx* c, int nsteps, double* trace) {
         if (e == hipSuccess)
             e = launch_energy_reduce(b, c->nrow, c->ncol, (double)c->m.alpha, dtr + 2 * c->B * (size_t)(nsteps - 1), c->stream);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(trace, dtr, sizeof(double) * 2 * c->B * (size_t)nsteps, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(dtr);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->trace_h, dtr, sizeof(double) * need, hipMemcpyDeviceToHost, c->stream);
     if (e != hipSuccess) return fail(PFC_E_HIP, "step_energy: %s", hipGetErrorString(e));
     return PFC_OK;
+}
+
+extern "C" int pfc_step_energy_result(pfc_ctx* c, int nsteps, double* trace) {
+    if (!c || !trace || nsteps < 0 || nsteps != c->trace_n) return fail(PFC_E_ARG, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (nsteps) std::memcpy(trace, c->trace_h, sizeof(double) * 2 * (size_t)c->B * (size_t)nsteps);
+    return PFC_OK;
+}
+
+extern "C" int pfc_step_energy(pfc_ctx* c, int nsteps, double* trace) {
+    if (!c || !trace || nsteps < 0) return fail(PFC_E_ARG, "bad arguments");
+    int rc = pfc_step_energy_enqueue(c, nsteps);
+    return rc ? rc : pfc_step_energy_result(c, nsteps, trace);
+}
+
+extern "C" int pfc_query(pfc_ctx* c, int* done) {
+    if (!c || !done) return fail(PFC_E_ARG, "bad arguments");
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) { *done = 1; return PFC_OK; }
+    if (e == hipErrorNotReady) { *done = 0; return PFC_OK; }
+    return fail(PFC_E_HIP, "stream query: %s", hipGetErrorString(e));
 }
 
 extern "C" int pfc_time_kernel(pfc_ctx* c, int kernel, int enable) {

@@ -9,9 +9,10 @@
 // slots while the others keep stepping (idle slots are masked with
 // pfc_set_active), steps every busy context concurrently (one HIP stream
 // each) to the next job event, and hands checkpoints to a writer thread that
-// writes the files the CLI would write (run_config.txt, energy.txt, phi/con
-// VTI + PVTI, Phimax_<kd>.txt, checkpoint_timestamps.txt) while stepping
-// continues.
+// writes the files the CLI would write, on the CLI's schedule (host/pfc_cli.cpp
+// run_sim: Misfit kd = 0 fields, then a checkpoint every mod steps and at the
+// end; CVD energy after every step and a checkpoint after step t when
+// t % mod == 0, cvd.cpp:389-454) while stepping continues.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -59,6 +60,9 @@ struct Job {
     bool cancel = false;
     std::string error;
     std::string write_error;  // set by the writer thread; the job fails at its final bundle
+    double t_stepped = -1;    // last step done (before the final files)
+    double create_s = 0;      // context creation this job waited for
+    int energy_rows = 0;      // CVD: per-step energy rows handed to the writer
 };
 
 struct Batch {
@@ -67,6 +71,9 @@ struct Batch {
     int slots = 0;
     std::vector<Job*> slot_job;  // nullptr = free
     size_t bytes = 0;
+    bool busy = false;    // steps in flight
+    int n = 0;            // steps of the launch in flight
+    bool traced = false;  // that launch traces the energy of every step
     int running() const {
         int n = 0;
         for (Job* j : slot_job) n += j != nullptr;
@@ -74,12 +81,16 @@ struct Batch {
     }
 };
 
-// One checkpoint of one job, written by the writer thread.
+// Files of one job written by the writer thread: the start of the run
+// (run_config.txt, the timestamps header, Misfit's kd = 0 fields), CVD
+// energy rows, or a checkpoint.  `final` marks the job's last bundle.
+enum { B_START = 0, B_ENERGY = 1, B_CKPT = 2 };
 struct Bundle {
     Job* job = nullptr;
+    int kind = B_CKPT;
     int kd = 0;
     bool final = false;
-    bool first = false;  // also write run_config.txt and the timestamps header
+    std::vector<double> erows;  // B_ENERGY: (energy, density) per step from step index kd
     std::vector<float> phi, con;
     double energy = 0, density = 0;
     int na = 0;
@@ -94,14 +105,38 @@ struct Bundle {
     bool has_q6 = false;
 };
 
+// Steps done at the job's next checkpoint (or its end).  Misfit: every mod
+// steps; CVD: after step index t with t % mod == 0, i.e. steps done = t + 1.
 int next_event(const Job& j) {
-    const int steps = j.spec.steps;
-    if (j.spec.mod <= 0 || j.outdir.empty()) return steps;
-    const int nxt = (j.steps_done / j.spec.mod + 1) * j.spec.mod;
+    const int steps = j.spec.steps, mod = j.spec.mod, s = j.steps_done;
+    if (mod <= 0 || j.outdir.empty()) return steps;
+    int nxt;
+    if (j.spec.p.model == PFC_MODEL_CVD) nxt = s < 1 ? 1 : ((s - 1) / mod + 1) * mod + 1;
+    else nxt = (s / mod + 1) * mod;
     return nxt < steps ? nxt : steps;
 }
 
+// Whether steps_done = s is a checkpoint of the CLI's schedule.
+bool is_checkpoint(const Job& j, int s) {
+    const int mod = j.spec.mod;
+    if (mod <= 0) return s == j.spec.steps;
+    if (j.spec.p.model == PFC_MODEL_CVD) return (s - 1) % mod == 0;
+    return s % mod == 0 || s == j.spec.steps;
+}
+
 }  // namespace
+
+// P-sized host arrays of the checkpoint extraction, reused across
+// checkpoints (scheduler thread only); a bundle gets copies of the atoms.
+struct CaptureScratch {
+    std::vector<double> x, y, z, ph;
+    std::vector<float> co, px, py, pz, pp;
+    void fit(size_t P) {
+        if (x.size() >= P) return;
+        x.resize(P); y.resize(P); z.resize(P); ph.resize(P);
+        co.resize(P); px.resize(P); py.resize(P); pz.resize(P); pp.resize(P);
+    }
+};
 
 struct pfc_server {
     pfc_server_opts o{};
@@ -120,24 +155,12 @@ struct pfc_server {
     int unfinished = 0;  // queued + running + awaiting writes
     double pt_steps = 0; // grid-point steps of finished jobs
     long long done = 0;
-    std::string fatal;
-    // Busy contexts step the whole graph chunks they share as one group graph
-    // (scheduler thread only; rebuilt when the set of stepping contexts
-    // changes, dropped before any member is destroyed).
-    bool use_group = true;
-    pfc_group* grp = nullptr;
-    std::vector<pfc_ctx*> grp_key;
+    CaptureScratch cap;  // scheduler thread only
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
 
 namespace {
-
-void drop_group(pfc_server* s) {
-    if (s->grp) pfc_group_destroy(s->grp);
-    s->grp = nullptr;
-    s->grp_key.clear();
-}
 
 void finish(pfc_server* s, Job* j, int state, double t) {
     // caller holds s->mu
@@ -151,56 +174,52 @@ void finish(pfc_server* s, Job* j, int state, double t) {
     s->cv_done.notify_all();
 }
 
-void write_bundle(const Bundle& b) {
+bool write_fields(const std::string& d, int kd, const pfc_params& p, const float* a, const float* b) {
+    const std::string k = std::to_string(kd);
+    bool ok = pfcio::write_vti(d + "phi_" + k + "_0.vti", "phi", p.L, p.M, p.N, a);
+    ok = pfcio::write_pvti(d + "phi_" + k + ".pvti", "phi", "phi_", kd, p.L, p.M, p.N) && ok;
+    ok = pfcio::write_vti(d + "con_" + k + "_0.vti", "con", p.L, p.M, p.N, b) && ok;
+    ok = pfcio::write_pvti(d + "con_" + k + ".pvti", "con", "con_", kd, p.L, p.M, p.N) && ok;
+    return ok;
+}
+
+// The files of one bundle, in the CLI's order (host/pfc_cli.cpp run_sim).
+bool write_bundle(const Bundle& b) {
     const Job& j = *b.job;
     const pfc_params& p = j.spec.p;
     const std::string d = j.outdir + "/";
-    const std::string kd = std::to_string(b.kd);
-    if (b.first) {
-        FILE* f = std::fopen((d + "run_config.txt").c_str(), "w");
-        if (f) {
-            using pfcio::g6;
-            std::fprintf(f, "model %s\n", p.model == PFC_MODEL_MISFIT ? "misfit" : "cvd");
-            std::fprintf(f, "u0 %s\ncon0 %s\nsig %s\ndt %s\ndx %s\n", g6(p.u0).c_str(), g6(p.con0).c_str(),
-                         g6(p.sig).c_str(), g6(p.dt).c_str(), g6(p.dx).c_str());
-            std::fprintf(f, "steps %d\nmod %d\nseed %u\nL %d\nM %d\nN %d\n", j.spec.steps, j.spec.mod, p.seed, p.L,
-                         p.M, p.N);
-            if (p.model == PFC_MODEL_MISFIT)
-                std::fprintf(f, "grainx %d\ngrainy %d\ngrainz %d\naxx %s\n", p.grainx, p.grainy, p.grainz,
-                             g6(p.axx).c_str());
-            std::fprintf(f, "precision %s\n", p.precision == PFC_PREC_FP64 ? "fp64" : "fp32");
-            std::fclose(f);
-        }
-        f = std::fopen((d + "checkpoint_timestamps.txt").c_str(), "w");
-        if (f) {
-            std::fprintf(f, "# Checkpoint timestamps - Step number and milliseconds since epoch\n");
-            std::fclose(f);
-        }
+    const bool misfit = p.model == PFC_MODEL_MISFIT;
+    if (b.kind == B_START) {
+        pfcio::RunConfig c;
+        c.model = misfit ? "misfit" : "cvd";
+        c.u0 = p.u0; c.con0 = p.con0; c.sig = p.sig; c.dt = p.dt; c.dx = p.dx;
+        c.steps = j.spec.steps; c.mod = j.spec.mod; c.seed = p.seed;
+        c.L = p.L; c.M = p.M; c.N = p.N;
+        c.grainx = p.grainx; c.grainy = p.grainy; c.grainz = p.grainz;
+        c.axx = p.axx; c.b0 = pfcio::noise_b0(p.axx, p.dx, p.dt);
+        bool ok = pfcio::write_run_config(d + "run_config.txt", c);
+        ok = pfcio::start_timestamps(d + "checkpoint_timestamps.txt") && ok;
+        if (misfit) ok = write_fields(d, 0, p, b.phi.data(), b.con.data()) && ok;  // kd = 0, before stepping
+        return ok;
     }
-    pfcio::write_vti(d + "phi_" + kd + "_0.vti", "phi", p.L, p.M, p.N, b.phi.data());
-    pfcio::write_pvti(d + "phi_" + kd + ".pvti", "phi", "phi_", b.kd, p.L, p.M, p.N);
-    pfcio::write_vti(d + "con_" + kd + "_0.vti", "con", p.L, p.M, p.N, b.con.data());
-    pfcio::write_pvti(d + "con_" + kd + ".pvti", "con", "con_", b.kd, p.L, p.M, p.N);
-    if (p.model == PFC_MODEL_MISFIT) {
-        pfcio::write_phimax_misfit(d + "Phimax_" + kd + ".txt", (int)b.x.size(), b.x.data(), b.y.data(), b.z.data(),
-                                   b.ph.data(), b.co.data());
+    if (b.kind == B_ENERGY)
+        return b.erows.empty() || pfcio::append_energy(d + "energy.txt", b.kd, (int)(b.erows.size() / 2), b.erows.data());
+    bool ok = true;
+    if (misfit) {
+        ok = pfcio::write_phimax_misfit(d + "Phimax_" + std::to_string(b.kd) + ".txt", (int)b.x.size(), b.x.data(),
+                                        b.y.data(), b.z.data(), b.ph.data(), b.co.data());
+        ok = write_fields(d, b.kd, p, b.phi.data(), b.con.data()) && ok;
+        const double e[2] = {b.energy, b.density};
+        ok = pfcio::append_energy(d + "energy.txt", b.kd, 1, e) && ok;
     } else {
-        pfcio::write_cvd_atoms(b.kd, p.L, p.M, p.N, p.dx, b.phic, b.pmax, b.na, b.px.data(), b.py.data(),
-                               b.pz.data(), b.pp.data(), j.outdir);
-        pfcio::write_cvd_gr(b.kd, p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.hist.data(), (int)b.hist.size(),
-                            j.outdir);
-        if (b.has_q6) pfcio::write_cvd_q6(b.kd, b.na, b.q6nb.data(), b.q6l.data(), b.q6g, b.q6h, j.outdir);
+        ok = pfcio::write_cvd_atoms(b.kd, p.L, p.M, p.N, p.dx, b.phic, b.pmax, b.na, b.px.data(), b.py.data(),
+                                    b.pz.data(), b.pp.data(), j.outdir);
+        ok = pfcio::write_cvd_gr(b.kd, p.L, p.M, p.N, p.dx, b.na, b.px.data(), b.hist.data(), (int)b.hist.size(),
+                                 j.outdir) && ok;
+        if (b.has_q6) ok = pfcio::write_cvd_q6(b.kd, b.na, b.q6nb.data(), b.q6l.data(), b.q6g, b.q6h, j.outdir) && ok;
+        ok = write_fields(d, b.kd, p, b.phi.data(), b.con.data()) && ok;
     }
-    FILE* f = std::fopen((d + "energy.txt").c_str(), "a");
-    if (f) {
-        std::fprintf(f, "%d %s %s\n", b.kd, pfcio::g6(b.energy).c_str(), pfcio::g6(b.density).c_str());
-        std::fclose(f);
-    }
-    f = std::fopen((d + "checkpoint_timestamps.txt").c_str(), "a");
-    if (f) {
-        std::fprintf(f, "step %d timestamp_ms %lld\n", b.kd, pfcio::now_ms());
-        std::fclose(f);
-    }
+    return pfcio::append_timestamp(d + "checkpoint_timestamps.txt", b.kd) && ok;
 }
 
 // CVD GR every checkpoint, Q6_GLO when kd % 20 == 0 (cvd.cpp:1253-1254), on
@@ -239,14 +258,15 @@ void writer_loop(pfc_server* s) {
         std::string err;
         bool ok = false;
         try {  // a failed checkpoint fails its job, not the writer thread
-            ok = b.job->spec.p.model != PFC_MODEL_CVD || cvd_order(b, err);
-            if (ok) write_bundle(b);
+            const bool order = b.kind == B_CKPT && b.job->spec.p.model == PFC_MODEL_CVD;
+            ok = !order || cvd_order(b, err);
+            if (ok && !(ok = write_bundle(b))) err = "cannot write the files in " + b.job->outdir;
         } catch (const std::exception& e) {
             ok = false;
             err = e.what();
         }
         lk.lock();
-        if (!ok && b.job->write_error.empty()) b.job->write_error = "checkpoint " + std::to_string(b.kd) + ": " + err;
+        if (!ok && b.job->write_error.empty()) b.job->write_error = "files of step " + std::to_string(b.kd) + ": " + err;
         if (b.final) {
             if (b.job->write_error.empty()) {
                 finish(s, b.job, PFC_JOB_DONE, s->now());
@@ -259,7 +279,7 @@ void writer_loop(pfc_server* s) {
 }
 
 // Fill a checkpoint bundle of slot `slot` (scheduler thread, no lock held).
-bool capture(Batch& bt, int slot, Bundle& b, const double* e_all, std::string& err) {
+bool capture(pfc_server* s, Batch& bt, int slot, Bundle& b, const double* e_all, std::string& err) {
     const pfc_params& p = bt.key;
     const size_t P = (size_t)p.L * p.M * p.N;
     b.phi.resize(P);
@@ -267,38 +287,57 @@ bool capture(Batch& bt, int slot, Bundle& b, const double* e_all, std::string& e
     if (pfc_get_fields_f32(bt.ctx, slot, b.phi.data(), b.con.data())) { err = pfc_last_error(); return false; }
     b.energy = e_all[2 * slot];
     b.density = e_all[2 * slot + 1];
+    CaptureScratch& c = s->cap;
+    c.fit(P);
     if (p.model == PFC_MODEL_MISFIT) {
-        std::vector<double> x(P), y(P), z(P), ph(P);
-        std::vector<float> co(P);
         int n = 0;
-        if (pfc_phimax_misfit(bt.ctx, slot, x.data(), y.data(), z.data(), ph.data(), co.data(), (int)P, &n)) {
+        if (pfc_phimax_misfit(bt.ctx, slot, c.x.data(), c.y.data(), c.z.data(), c.ph.data(), c.co.data(), (int)P, &n)) {
             err = pfc_last_error();
             return false;
         }
         // The 2-D extraction is repeated for each n (misfit.cpp:3345).
+        const size_t k = (size_t)n * p.N;
+        b.x.reserve(k); b.y.reserve(k); b.z.reserve(k); b.ph.reserve(k); b.co.reserve(k);
         for (int nn = 0; nn < p.N; nn++)
             for (int i = 0; i < n; i++) {
-                b.x.push_back(x[i]);
-                b.y.push_back(y[i]);
-                b.z.push_back(nn == 0 ? z[i] : (double)((float)(nn + 1) * p.dx));
-                b.ph.push_back(ph[i]);
-                b.co.push_back(co[i]);
+                b.x.push_back(c.x[i]);
+                b.y.push_back(c.y[i]);
+                b.z.push_back(nn == 0 ? c.z[i] : (double)((float)(nn + 1) * p.dx));
+                b.ph.push_back(c.ph[i]);
+                b.co.push_back(c.co[i]);
             }
     } else {
-        b.px.resize(P); b.py.resize(P); b.pz.resize(P); b.pp.resize(P);
-        if (pfc_phimax_cvd(bt.ctx, slot, b.px.data(), b.py.data(), b.pz.data(), b.pp.data(), (int)P, &b.na, &b.phic,
+        if (pfc_phimax_cvd(bt.ctx, slot, c.px.data(), c.py.data(), c.pz.data(), c.pp.data(), (int)P, &b.na, &b.phic,
                            &b.pmax)) {
             err = pfc_last_error();
             return false;
         }
         // GR and Q6 need only these atoms: the writer thread computes them.
-        b.px.resize((size_t)b.na); b.py.resize((size_t)b.na); b.pz.resize((size_t)b.na); b.pp.resize((size_t)b.na);
+        b.px.assign(c.px.begin(), c.px.begin() + b.na);
+        b.py.assign(c.py.begin(), c.py.begin() + b.na);
+        b.pz.assign(c.pz.begin(), c.pz.begin() + b.na);
+        b.pp.assign(c.pp.begin(), c.pp.begin() + b.na);
     }
     return true;
 }
 
-// Admit queued jobs into free slots; creates contexts within the budget.
-// Called with the lock held; releases it around GPU work.
+// The first bundle of a job with an output directory (lock held): run
+// configuration, timestamps header, Misfit's kd = 0 fields.
+void start_bundle(pfc_server* s, Job* j) {
+    Bundle b;
+    b.job = j;
+    b.kind = B_START;
+    if (j->spec.p.model == PFC_MODEL_MISFIT) {
+        b.phi.assign(j->phi0.begin(), j->phi0.end());  // double -> float, as the CLI casts
+        b.con.assign(j->con0.begin(), j->con0.end());
+    }
+    s->wq.push_back(std::move(b));
+    s->cv_writer.notify_one();
+}
+
+// Admit queued jobs into free slots of idle contexts (a busy one takes its
+// new jobs once its steps in flight are done); creates contexts within the
+// budget.  Called with the lock held; releases it around GPU work.
 void admit(pfc_server* s, std::unique_lock<std::mutex>& lk) {
     for (size_t qi = 0; qi < s->pending.size();) {
         Job* j = s->pending[qi];
@@ -309,12 +348,15 @@ void admit(pfc_server* s, std::unique_lock<std::mutex>& lk) {
         }
         Batch* bt = nullptr;
         int slot = -1;
+        bool class_busy = false;
         for (auto& b : s->batches) {
             if (!same_class(b->key, j->spec.p)) continue;
+            if (b->busy) { class_busy = true; continue; }
             for (int i = 0; i < b->slots && slot < 0; i++)
                 if (!b->slot_job[i]) { bt = b.get(); slot = i; }
             if (bt) break;
         }
+        double create_s = 0;
         if (!bt) {
             // New context for this class, as large as the budget allows.
             const size_t one = pfc::ctx_bytes(j->spec.p, 1);
@@ -325,10 +367,9 @@ void admit(pfc_server* s, std::unique_lock<std::mutex>& lk) {
                 finish(s, j, PFC_JOB_FAILED, s->now());
                 continue;
             }
-            // Free contexts with no running job (oldest first) until it fits.
+            // Free idle contexts with no running job (oldest first) until it fits.
             for (size_t bi = 0; bi < s->batches.size() && (double)(s->used + one) > s->budget;) {
-                if (s->batches[bi]->running() == 0) {
-                    drop_group(s);
+                if (s->batches[bi]->running() == 0 && !s->batches[bi]->busy) {
                     pfc_destroy(s->batches[bi]->ctx);
                     s->used -= s->batches[bi]->bytes;
                     s->batches.erase(s->batches.begin() + bi);
@@ -336,8 +377,10 @@ void admit(pfc_server* s, std::unique_lock<std::mutex>& lk) {
                     bi++;
                 }
             }
-            if ((double)(s->used + one) > s->budget) {
-                qi++;  // waits until running jobs release memory
+            if ((double)(s->used + one) > s->budget || class_busy) {
+                // waits until running jobs release memory, or for the busy
+                // context of its class to take it when its steps are done
+                qi++;
                 continue;
             }
             int slots = s->o.max_slots;
@@ -348,11 +391,13 @@ void admit(pfc_server* s, std::unique_lock<std::mutex>& lk) {
             nb->slot_job.assign(slots, nullptr);
             nb->bytes = pfc::ctx_bytes(j->spec.p, slots);
             lk.unlock();
+            const double t_c = s->now();
             std::string err;
             int rc = pfc_create(&nb->key, slots, &nb->ctx);
             if (rc == 0 && s->o.graph_chunk > 0) rc = pfc_set_graph(nb->ctx, s->o.graph_chunk);
             for (int i = 0; rc == 0 && i < slots; i++) rc = pfc_set_active(nb->ctx, i, 0);
             if (rc) err = pfc_last_error();
+            create_s = s->now() - t_c;
             lk.lock();
             if (rc) {
                 if (nb->ctx) pfc_destroy(nb->ctx);
@@ -371,6 +416,8 @@ void admit(pfc_server* s, std::unique_lock<std::mutex>& lk) {
         bt->slot_job[slot] = j;
         j->state = PFC_JOB_RUNNING;
         j->t_start = s->now();
+        j->create_s = create_s;
+        if (!j->outdir.empty()) start_bundle(s, j);
         lk.unlock();
         int rc = pfc_set_fields(bt->ctx, slot, j->phi0.data(), j->con0.data());
         if (rc == 0) rc = pfc_set_seed(bt->ctx, slot, j->spec.p.seed);
@@ -382,139 +429,206 @@ void admit(pfc_server* s, std::unique_lock<std::mutex>& lk) {
         if (rc) {
             bt->slot_job[slot] = nullptr;
             j->error = "load failed: " + err;
-            finish(s, j, PFC_JOB_FAILED, s->now());
+            if (j->outdir.empty()) {
+                finish(s, j, PFC_JOB_FAILED, s->now());
+            } else {  // after its start bundle: the writer ends it
+                j->write_error = j->error;
+                Bundle fb;
+                fb.job = j;
+                fb.kind = B_ENERGY;
+                fb.final = true;
+                s->wq.push_back(std::move(fb));
+                s->cv_writer.notify_one();
+            }
         }
     }
 }
 
-void sched_loop(pfc_server* s) {
-    std::unique_lock<std::mutex> lk(s->mu);
-    while (true) {
-        admit(s, lk);
-        // Plan: every context with running jobs advances to its next job event.
-        struct Plan { Batch* b; int n; };
-        std::vector<Plan> plan;
-        for (auto& b : s->batches) {
-            int n = -1;
-            for (Job* j : b->slot_job) {
-                if (!j) continue;
-                const int d = j->cancel ? 0 : next_event(*j) - j->steps_done;
-                n = n < 0 ? d : std::min(n, d);
+// Launch the next steps of an idle batch with running jobs: every job
+// advances to its next event (quantum at most).  CVD batches whose jobs
+// write files trace the energy of every step (energy.txt, cvd.cpp:442).
+// Lock held; the launches only enqueue.
+void launch_batch(pfc_server* s, Batch& b, std::string& err) {
+    int n = -1;
+    bool trace = false;
+    for (Job* j : b.slot_job) {
+        if (!j) continue;
+        const int d = j->cancel ? 0 : next_event(*j) - j->steps_done;
+        n = n < 0 ? d : std::min(n, d);
+        trace = trace || (b.key.model == PFC_MODEL_CVD && !j->outdir.empty());
+    }
+    if (n < 0) return;
+    n = std::min(n, s->o.quantum);
+    b.n = n;
+    b.traced = trace && n > 0;
+    b.busy = true;
+    int rc = 0;
+    if (n > 0) rc = b.traced ? pfc_step_energy_enqueue(b.ctx, n) : pfc_step(b.ctx, n);
+    if (rc) err = pfc_last_error();
+}
+
+// Events of a batch whose launched steps have finished (lock held on entry
+// and exit, released around GPU work): energy rows, cancellations,
+// checkpoints and completions.  `err` is a failure of the batch's stepping
+// (fails all its jobs); other failures fail their own job only.
+void batch_events(pfc_server* s, std::unique_lock<std::mutex>& lk, Batch& b, std::string err) {
+    const int n = b.n;
+    b.busy = false;
+    std::vector<double> tr;
+    lk.unlock();
+    if (err.empty() && pfc_sync(b.ctx)) err = pfc_last_error();
+    if (err.empty() && b.traced) {
+        tr.resize(2 * (size_t)b.slots * n);
+        if (pfc_step_energy_result(b.ctx, n, tr.data())) err = pfc_last_error();
+    }
+    lk.lock();
+    const double t_done = s->now();
+    std::vector<int> ev;
+    std::vector<Bundle> out;
+    for (int i = 0; i < b.slots; i++) {
+        Job* j = b.slot_job[i];
+        if (!j) continue;
+        const int target = next_event(*j);
+        const int before = j->steps_done;
+        if (!j->cancel) j->steps_done += n;
+        if (err.empty() && b.traced && !j->cancel && !j->outdir.empty() && n > 0) {
+            Bundle eb;  // energy after each of these steps, labelled by step index
+            eb.job = j;
+            eb.kind = B_ENERGY;
+            eb.kd = before;
+            eb.erows.resize(2 * (size_t)n);
+            for (int t = 0; t < n; t++) {
+                eb.erows[2 * t] = tr[(2 * (size_t)t * b.slots) + 2 * i];
+                eb.erows[2 * t + 1] = tr[(2 * (size_t)t * b.slots) + 2 * i + 1];
             }
-            if (n >= 0) plan.push_back({b.get(), std::min(n, s->o.quantum)});
+            out.push_back(std::move(eb));
         }
-        if (plan.empty()) {
-            if (s->stop) return;
-            s->cv_sched.wait(lk, [&] { return s->stop || !s->pending.empty(); });
-            continue;
-        }
+        if (j->cancel || !err.empty() || j->steps_done == target) ev.push_back(i);
+    }
+    std::vector<int> release;
+    std::map<int, std::string> failed;  // slot -> its job's failure
+    if (!ev.empty()) {
         lk.unlock();
-        std::string err;
-        // Whole graph chunks every stepping context takes this round run as
-        // one group graph; the rest of each context's steps on its own stream.
-        const int chunk = s->o.graph_chunk;
-        int common = -1, nstep = 0;
-        std::vector<pfc_ctx*> key;
-        for (auto& pl : plan)
-            if (pl.n > 0) {
-                common = common < 0 ? pl.n : std::min(common, pl.n);
-                nstep++;
-                key.push_back(pl.b->ctx);
+        std::vector<double> e(2 * (size_t)b.slots, NAN);
+        std::string berr = err;  // failure of the whole batch
+        if (berr.empty() && pfc_energy(b.ctx, e.data())) berr = pfc_last_error();
+        for (int i : ev) {
+            Job* j = b.slot_job[i];
+            if (j->cancel || !berr.empty()) {
+                release.push_back(i);
+                if (!j->cancel) failed[i] = berr;
+                continue;
             }
-        int k = (s->use_group && chunk > 0 && nstep >= 2) ? common / chunk : 0;
-        if (k > 0 && (!s->grp || key != s->grp_key)) {
-            drop_group(s);
-            if (pfc_group_create(key.data(), (int)key.size(), chunk, &s->grp) == 0) s->grp_key = key;
-            else { s->grp = nullptr; k = 0; }
-        }
-        if (k > 0 && pfc_group_step(s->grp, k) && err.empty()) err = pfc_last_error();
-        for (auto& pl : plan)
-            if (pl.n - k * chunk > 0 && pfc_step(pl.b->ctx, pl.n - k * chunk) && err.empty()) err = pfc_last_error();
-        for (auto& pl : plan)
-            if (pl.n > 0 && pfc_sync(pl.b->ctx) && err.empty()) err = pfc_last_error();
-        lk.lock();
-        // Events: cancellations, checkpoints, completions.
-        for (auto& pl : plan) {
-            Batch& b = *pl.b;
-            std::vector<int> ev;
-            for (int i = 0; i < b.slots; i++) {
-                Job* j = b.slot_job[i];
-                if (!j) continue;
-                const int target = next_event(*j);
-                if (!j->cancel) j->steps_done += pl.n;
-                if (j->cancel || !err.empty() || j->steps_done == target) ev.push_back(i);
+            const bool final = j->steps_done >= j->spec.steps;
+            if (final) {
+                j->t_stepped = t_done;
+                j->energy = e[2 * i];
+                j->density = e[2 * i + 1];
             }
-            if (ev.empty()) continue;
-            lk.unlock();
-            std::vector<double> e(2 * (size_t)b.slots, NAN);
-            std::string eerr = err;
-            if (eerr.empty() && pfc_energy(b.ctx, e.data())) eerr = pfc_last_error();
-            std::vector<Bundle> out;
-            std::vector<int> release, failed;
-            for (int i : ev) {
-                Job* j = b.slot_job[i];
-                if (j->cancel || !eerr.empty()) {
-                    release.push_back(i);
-                    if (!j->cancel) failed.push_back(i);
-                    continue;
-                }
-                const bool final = j->steps_done >= j->spec.steps;
+            std::string jerr;
+            try {  // host buffers: a failed allocation fails this job only
                 if (j->spec.keep_fields && final) {
                     const size_t P = (size_t)b.key.L * b.key.M * b.key.N;
                     j->phi.resize(P);
                     j->con.resize(P);
-                    if (pfc_get_fields(b.ctx, i, j->phi.data(), j->con.data())) {
-                        eerr = pfc_last_error();
-                        release.push_back(i);
-                        failed.push_back(i);
-                        continue;
-                    }
+                    if (pfc_get_fields(b.ctx, i, j->phi.data(), j->con.data())) jerr = pfc_last_error();
                 }
-                if (final) {
-                    j->energy = e[2 * i];
-                    j->density = e[2 * i + 1];
-                }
-                if (!j->outdir.empty()) {
+                if (jerr.empty() && !j->outdir.empty() && is_checkpoint(*j, j->steps_done)) {
                     Bundle bd;
                     bd.job = j;
-                    bd.kd = j->steps_done;
+                    bd.kind = B_CKPT;
+                    // CVD labels a checkpoint by the step index (kd = t after step t, cvd.cpp:425-426)
+                    bd.kd = b.key.model == PFC_MODEL_CVD ? j->steps_done - 1 : j->steps_done;
                     bd.final = final;
-                    bd.first = j->steps_done <= (j->spec.mod > 0 ? j->spec.mod : j->spec.steps);
-                    std::string cerr;
-                    bool got = false;
-                    try {  // host buffers of the bundle: a failed allocation fails this job only
-                        got = capture(b, i, bd, e.data(), cerr);
-                    } catch (const std::exception& ex) {
-                        cerr = std::string("checkpoint capture: ") + ex.what();
-                    }
-                    if (!got) {
-                        eerr = cerr;
-                        release.push_back(i);
-                        failed.push_back(i);
-                        continue;
-                    }
-                    out.push_back(std::move(bd));
+                    if (capture(s, b, i, bd, e.data(), jerr)) out.push_back(std::move(bd));
+                } else if (jerr.empty() && !j->outdir.empty() && final) {
+                    Bundle fb;  // marks the end after the job's last files
+                    fb.job = j;
+                    fb.kind = B_ENERGY;
+                    fb.final = true;
+                    out.push_back(std::move(fb));
                 }
-                if (final) release.push_back(i);
+            } catch (const std::exception& ex) {
+                jerr = std::string("checkpoint: ") + ex.what();
             }
-            for (int i : release) pfc_set_active(b.ctx, i, 0);
-            lk.lock();
-            for (int i : release) {
-                Job* j = b.slot_job[i];
-                b.slot_job[i] = nullptr;
-                const bool fail = std::find(failed.begin(), failed.end(), i) != failed.end();
-                if (j->cancel) {
-                    finish(s, j, PFC_JOB_CANCELLED, s->now());
-                } else if (fail) {
-                    j->error = eerr;
-                    finish(s, j, PFC_JOB_FAILED, s->now());
-                } else if (j->outdir.empty()) {
-                    finish(s, j, PFC_JOB_DONE, s->now());
-                }  // else: DONE once the writer has written the final bundle
+            if (!jerr.empty()) {
+                failed[i] = jerr;
+                release.push_back(i);
+                continue;
             }
-            for (auto& bd : out) s->wq.push_back(std::move(bd));
-            if (!out.empty()) s->cv_writer.notify_one();
+            if (final) release.push_back(i);
         }
+        for (int i : release) pfc_set_active(b.ctx, i, 0);
+        lk.lock();
+    }
+    for (int i : release) {
+        Job* j = b.slot_job[i];
+        b.slot_job[i] = nullptr;
+        auto f = failed.find(i);
+        if (j->cancel) {
+            finish(s, j, PFC_JOB_CANCELLED, s->now());
+        } else if (f != failed.end()) {
+            if (j->outdir.empty()) {
+                j->error = f->second;
+                finish(s, j, PFC_JOB_FAILED, s->now());
+            } else {  // the writer ends it after the bundles already queued
+                j->write_error = f->second;
+                Bundle fb;
+                fb.job = j;
+                fb.kind = B_ENERGY;
+                fb.final = true;
+                out.push_back(std::move(fb));
+            }
+        } else if (j->outdir.empty()) {
+            finish(s, j, PFC_JOB_DONE, s->now());
+        }  // else: DONE once the writer has written the final bundle
+    }
+    for (auto& bd : out) s->wq.push_back(std::move(bd));
+    if (!out.empty()) s->cv_writer.notify_one();
+}
+
+// The scheduler: contexts step independently.  Each idle context with
+// running jobs is launched to its jobs' next event; the loop then handles
+// whichever launched context finishes first (pfc_query) and launches it
+// again, so a context with short intervals never waits for a long one.
+void sched_loop(pfc_server* s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        admit(s, lk);
+        bool any_busy = false;
+        for (auto& b : s->batches) {
+            if (!b->busy && b->running() > 0) {
+                std::string err;
+                launch_batch(s, *b, err);
+                if (!err.empty()) batch_events(s, lk, *b, err);
+            }
+            any_busy = any_busy || b->busy;
+        }
+        if (!any_busy) {
+            if (s->stop && s->pending.empty()) return;
+            bool running = false;
+            for (auto& b : s->batches) running = running || b->running() > 0;
+            if (!running) s->cv_sched.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+            if (s->stop && s->pending.empty()) {
+                bool r = false;
+                for (auto& b : s->batches) r = r || b->running() > 0;
+                if (!r) return;
+            }
+            continue;
+        }
+        // Handle every finished context; if none is, wait briefly (new
+        // submissions wake the loop early).
+        bool handled = false;
+        for (auto& b : s->batches) {
+            if (!b->busy) continue;
+            int done = 0;
+            std::string err;
+            if (pfc_query(b->ctx, &done)) { err = pfc_last_error(); done = 1; }
+            if (!done) continue;
+            batch_events(s, lk, *b, err);
+            handled = true;
+        }
+        if (!handled) s->cv_sched.wait_for(lk, std::chrono::microseconds(200));
     }
 }
 
@@ -539,8 +653,6 @@ extern "C" int pfc_server_create(const pfc_server_opts* o, pfc_server** out) try
     if (d.max_slots <= 0 || d.graph_chunk < 0 || d.quantum <= 0) return pfc::api_fail(PFC_E_ARG, "bad server options");
     pfc_server* s = new pfc_server();
     s->o = d;
-    const char* ge = std::getenv("PFC_SERVE_GROUP");  // "0": each context on its own stream only
-    s->use_group = !(ge && ge[0] == '0');
     if (d.hbm_budget > 0) {
         s->budget = d.hbm_budget;
     } else {
@@ -598,6 +710,8 @@ extern "C" int pfc_server_poll(pfc_server* s, long long id, pfc_job_status* st) 
     st->t_end = j.t_end;
     st->energy = j.energy;
     st->density = j.density;
+    st->t_stepped = j.t_stepped;
+    st->create_s = j.create_s;
     return PFC_OK;
 }
 
@@ -674,7 +788,6 @@ extern "C" int pfc_server_destroy(pfc_server* s) {
         s->cv_writer.notify_all();
     }
     s->writer.join();
-    drop_group(s);
     for (auto& b : s->batches) pfc_destroy(b->ctx);
     delete s;
     return PFC_OK;

@@ -225,22 +225,14 @@ int gpu_fail(const char* what) {
 }
 
 void write_run_config(Kind k, const Opts& o, float b0) {
-    FILE* f = std::fopen("run_config.txt", "w");
-    if (!f) return;
-    using pfcio::g6;
-    std::fprintf(f, "model %s\n", k == MISFIT ? "misfit" : (k == CVD ? "cvd" : "elastic"));
-    std::fprintf(f, "u0 %s\ncon0 %s\nsig %s\ndt %s\ndx %s\n", g6(o.p.u0).c_str(), g6(o.p.con0).c_str(),
-                 g6(o.p.sig).c_str(), g6(o.p.dt).c_str(), g6(o.p.dx).c_str());
-    std::fprintf(f, "steps %d\nmod %d\nseed %u\nL %d\nM %d\nN %d\n", o.steps, o.mod, o.p.seed, o.p.L, o.p.M, o.p.N);
-    if (k == MISFIT)
-        std::fprintf(f, "grainx %d\ngrainy %d\ngrainz %d\naxx %s\nb0 %s\n", o.p.grainx, o.p.grainy, o.p.grainz,
-                     g6(o.p.axx).c_str(), g6(b0).c_str());
-    if (k == ELASTIC) {
-        std::fprintf(f, "grainx %d\ngrainy %d\ngrainz %d\nbenchel %s\n", o.p.grainx, o.p.grainy, o.p.grainz,
-                     g6(o.benchel).c_str());
-        if (!o.indir.empty()) std::fprintf(f, "input_dir %s\n", o.indir.c_str());
-    }
-    std::fclose(f);
+    pfcio::RunConfig c;
+    c.model = k == MISFIT ? "misfit" : (k == CVD ? "cvd" : "elastic");
+    c.u0 = o.p.u0; c.con0 = o.p.con0; c.sig = o.p.sig; c.dt = o.p.dt; c.dx = o.p.dx;
+    c.steps = o.steps; c.mod = o.mod; c.seed = o.p.seed;
+    c.L = o.p.L; c.M = o.p.M; c.N = o.p.N;
+    c.grainx = o.p.grainx; c.grainy = o.p.grainy; c.grainz = o.p.grainz;
+    c.axx = o.p.axx; c.b0 = b0; c.benchel = o.benchel; c.indir = o.indir;
+    pfcio::write_run_config("run_config.txt", c);
 }
 
 // VTI + PVTI pair of both fields at checkpoint kd (misfit.cpp:1588-1698).
@@ -290,11 +282,7 @@ struct CvdOrder {
     }
 };
 
-float b0_of(const pfc_params& p) {
-    if (p.dx > 0.0f && p.dt > 0.0f && p.axx >= 0.0f)
-        return static_cast<float>(std::sqrt(static_cast<double>(p.axx / p.dx / p.dx / p.dx / p.dt)));
-    return 0.0f;
-}
+float b0_of(const pfc_params& p) { return pfcio::noise_b0(p.axx, p.dx, p.dt); }
 
 // run_misfit (misfit.cpp:311-531).
 int run_sim(Kind k, int argc, char** argv) {

@@ -393,6 +393,52 @@ void Writer::loop() {
     }
 }
 
+bool write_run_config(const std::string& path, const RunConfig& c) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    const bool misfit = std::string(c.model) == "misfit", elastic = std::string(c.model) == "elastic";
+    std::fprintf(f, "model %s\n", c.model);
+    std::fprintf(f, "u0 %s\ncon0 %s\nsig %s\ndt %s\ndx %s\n", g6(c.u0).c_str(), g6(c.con0).c_str(), g6(c.sig).c_str(),
+                 g6(c.dt).c_str(), g6(c.dx).c_str());
+    std::fprintf(f, "steps %d\nmod %d\nseed %u\nL %d\nM %d\nN %d\n", c.steps, c.mod, c.seed, c.L, c.M, c.N);
+    if (misfit)
+        std::fprintf(f, "grainx %d\ngrainy %d\ngrainz %d\naxx %s\nb0 %s\n", c.grainx, c.grainy, c.grainz,
+                     g6(c.axx).c_str(), g6(c.b0).c_str());
+    if (elastic) {
+        std::fprintf(f, "grainx %d\ngrainy %d\ngrainz %d\nbenchel %s\n", c.grainx, c.grainy, c.grainz,
+                     g6(c.benchel).c_str());
+        if (!c.indir.empty()) std::fprintf(f, "input_dir %s\n", c.indir.c_str());
+    }
+    return std::fclose(f) == 0;
+}
+
+float noise_b0(float axx, float dx, float dt) {
+    if (dx > 0.0f && dt > 0.0f && axx >= 0.0f)
+        return static_cast<float>(std::sqrt(static_cast<double>(axx / dx / dx / dx / dt)));
+    return 0.0f;
+}
+
+bool start_timestamps(const std::string& path) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::fprintf(f, "# Checkpoint timestamps - Step number and milliseconds since epoch\n");
+    return std::fclose(f) == 0;
+}
+
+bool append_timestamp(const std::string& path, int step) {
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) return false;
+    std::fprintf(f, "step %d timestamp_ms %lld\n", step, now_ms());
+    return std::fclose(f) == 0;
+}
+
+bool append_energy(const std::string& path, int kd0, int n, const double* e) {
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) return false;
+    for (int s = 0; s < n; s++) std::fprintf(f, "%d %s %s\n", kd0 + s, g6(e[2 * s]).c_str(), g6(e[2 * s + 1]).c_str());
+    return std::fclose(f) == 0;
+}
+
 long long now_ms() {
     return std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::system_clock::now().time_since_epoch())
         .count();

@@ -49,6 +49,30 @@ bool write_cvd_q6(int kd, int n, const unsigned* nb, const float* ql, float qglo
 // Default iostream formatting of a double (%g, 6 significant digits).
 std::string g6(double v);
 
+// run_config.txt of one run ("key value" lines; misfit.cpp:369-389,
+// cvd.cpp:260-286, elastic.cpp:380-391): model, u0 con0 sig dt dx, steps
+// mod seed, the grid; Misfit adds grainx/y/z, axx and b0; elastic grainx/y/z,
+// benchel and the input directory when set.
+struct RunConfig {
+    const char* model = "misfit";  // "misfit", "cvd" or "elastic"
+    float u0 = 0, con0 = 0, sig = 0, dt = 0, dx = 0;
+    int steps = 0, mod = 0;
+    unsigned seed = 0;
+    int L = 0, M = 0, N = 1;
+    int grainx = 0, grainy = 0, grainz = 0;
+    float axx = 0, b0 = 0, benchel = 0;
+    std::string indir;
+};
+bool write_run_config(const std::string& path, const RunConfig& c);
+// Noise amplitude b0 = sqrt(axx / dx^3 / dt) (misfit.cpp:174-178).
+float noise_b0(float axx, float dx, float dt);
+// checkpoint_timestamps.txt: the header, and one "step N timestamp_ms T" line.
+bool start_timestamps(const std::string& path);
+bool append_timestamp(const std::string& path, int step);
+// energy.txt rows "kd E rho" appended (misfit.cpp:1084-1085), n rows of
+// (energy, density) pairs labelled kd0, kd0 + 1, ...
+bool append_energy(const std::string& path, int kd0, int n, const double* e);
+
 // CollectAvailableCheckpointSteps (elastic.cpp:37-74).
 std::vector<int> checkpoint_steps(const std::string& dir);
 

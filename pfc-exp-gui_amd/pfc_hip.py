@@ -23,7 +23,7 @@ SYMBOLS = [
     "pfc_params_default", "pfc_init_fields", "pfc_create", "pfc_destroy", "pfc_set_fields", "pfc_set_seed",
     "pfc_set_active",
     "pfc_step", "pfc_sync", "pfc_get_fields", "pfc_get_fields_f32", "pfc_energy",
-    "pfc_step_energy", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic", "pfc_phimax_elastic_noalloc",
+    "pfc_step_energy", "pfc_step_energy_enqueue", "pfc_step_energy_result", "pfc_query", "pfc_phimax_misfit", "pfc_phimax_cvd", "pfc_phimax_elastic", "pfc_phimax_elastic_noalloc",
     "pfc_strain", "pfc_cryst_order", "pfc_cvd_gr", "pfc_cvd_q6", "pfc_col_tiles", "pfc_set_col_tiles", "pfc_group_create", "pfc_group_step",
     "pfc_group_destroy", "pfc_set_graph", "pfc_time_kernel", "pfc_kernel_time", "pfc_kernel_bytes",
     "pfc_device_memory", "pfc_copy_probe", "pfc_last_error",
@@ -58,7 +58,7 @@ class Job(ctypes.Structure):
 class JobStatus(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int), ("steps_done", ctypes.c_int), ("t_submit", ctypes.c_double),
                 ("t_start", ctypes.c_double), ("t_end", ctypes.c_double), ("energy", ctypes.c_double),
-                ("density", ctypes.c_double)]
+                ("density", ctypes.c_double), ("t_stepped", ctypes.c_double), ("create_s", ctypes.c_double)]
 
 
 class ServerInfo(ctypes.Structure):
@@ -100,6 +100,9 @@ def load():
         "pfc_get_fields_f32": [vp, ctypes.c_int, fp, fp],
         "pfc_energy": [vp, dp],
         "pfc_step_energy": [vp, ctypes.c_int, dp],
+        "pfc_step_energy_enqueue": [vp, ctypes.c_int],
+        "pfc_step_energy_result": [vp, ctypes.c_int, dp],
+        "pfc_query": [vp, ctypes.POINTER(ctypes.c_int)],
         "pfc_phimax_misfit": [vp, ctypes.c_int, dp, dp, dp, dp, fp, ctypes.c_int, ip],
         "pfc_phimax_cvd": [vp, ctypes.c_int, fp, fp, fp, fp, ctypes.c_int, ip, fp, fp],
         "pfc_phimax_elastic": [ctypes.c_int] * 3 + [ctypes.c_float, fp, dp, dp, dp, dp, ctypes.c_int, ip],

@@ -1,0 +1,81 @@
+"""The FP32 accuracy bar over long horizons (bench.py ACC_K / ACC_FLOOR,
+stated in the design notes), and the FP64 drop-in against the oracle's own
+trajectory.
+
+At every checkpoint each metric of the FP32 run (field L-inf, energy, Phimax
+atom count) may differ from the FP64 drop-in (reference narrowing) by at most
+ACC_K times the FP64 envelope -- the largest difference of the drop-in
+without the narrowing and the drop-in from fields one FP32 round-off away
+(three seeds) -- or by a floor; once that envelope is saturated (the
+trajectory chaotic) the energy and atom count stay within ACC_SPREAD times
+the FP64 ensemble's own spread; the mean density must be equal (the k = 0
+mode is FP64 on both paths).  Cases: Misfit 128^2 (seeded everywhere; the trajectory settles, and
+FP32 also meets L-inf <= 1e-4 after 10^4 steps there) and CVD 128x384 (the
+grid with the largest FP32 distance after 50 steps, from its sharp two-phase
+interface; checkpoints from step 10)."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def bench():
+    spec = importlib.util.spec_from_file_location("pfc_bench_acc", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _report(name, trace):
+    for r in trace:
+        print(f"{name} t={r['step']} fp32 {r['linf_fp32']:.2e} envelope {r['envelope']['linf']:.2e} "
+              f"E32 {r['energy_rel_fp32']:.1e} (env {r['envelope']['energy_rel']:.1e}) atoms "
+              f"{r['atoms_ref']}/{r['atoms_fp32']} saturated={r['saturated']} bar {r['bar']}", flush=True)
+
+
+def test_fp32_bar_misfit_128(hip, bench):
+    marks = list(range(1000, 10001, 1000))
+    trace, _ = bench.accuracy_case(hip, hip.MISFIT, 128, 128, marks)
+    _report("misfit 128^2", trace)
+    for r in trace:
+        assert r["bar_ok"], r
+    assert trace[-1]["linf_fp32"] < 1e-4  # the stated FP32 tolerance, on a trajectory that is not chaotic
+
+
+def test_fp32_bar_cvd_128x384(hip, bench):
+    marks = [10, 20, 50, 100, 200, 500, 1000, 2000, 5000, 10000]
+    trace, _ = bench.accuracy_case(hip, hip.CVD, 128, 384, marks)
+    _report("cvd 128x384", trace)
+    for r in trace:
+        assert r["bar_ok"], r
+
+
+def test_fp64_dropin_vs_oracle_2000_steps(hip, oracle, bench):
+    """The FP64 drop-in against the oracle over 2000 steps of Misfit 128^2
+    (both with the reference narrowing): they differ only where an FFT
+    round-off difference flips one float rounding of a coefficient, which the
+    dynamics then carries (measured 5.6e-6 at 1000 and 3.2e-6 at 2000)."""
+    p = bench._acc_params(hip, hip.MISFIT, 128, 128, hip.FP64, 1)
+    phi, con = hip.init_fields(p)
+    m = oracle.model(0, 128, 128, 1, narrow=1, grainx=p.grainx, grainy=p.grainy)
+    ophi, ocon = oracle.init_fields(m)
+    assert np.array_equal(phi, ophi) and np.array_equal(con, ocon)
+    s = hip.Sim(p)
+    s.set_fields(phi, con)
+    s.set_graph(100)
+    osim = oracle.Sim(m, phi, con)
+    for t in (1000, 2000):
+        s.step(1000)
+        osim.run(1000)
+        g, o = s.get_fields(), osim.fields()
+        err = max(np.abs(g[0] - o[0]).max(), np.abs(g[1] - o[1]).max())
+        print(f"FP64 drop-in vs oracle, Misfit 128^2, t={t}: {err:.3e}", flush=True)
+        assert err < 5e-5
+        assert abs(g[0].mean() - o[0].mean()) < 1e-14
+    s.close()

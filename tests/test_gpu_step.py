@@ -381,30 +381,38 @@ def test_misfit_noise_batched_seeds(hip, oracle):
     sim.close()
 
 
-@pytest.mark.parametrize("prec,L,B", [(1, 128, 4), (1, 384, 4), (1, 512, 16), (0, 256, 1), (0, 512, 1)])
+@pytest.mark.parametrize("prec,L,B", [(1, 128, 4), (1, 384, 4), (1, 512, 16), (1, 256, 16), (1, 1024, 2),
+                                       (0, 256, 1), (0, 512, 1)])
 def test_column_tile_widths_bit_identical(hip, prec, L, B):
-    """Every tile width the autotuner may pick gives the same bits (each
-    column is computed by the same code whatever the workgroup holds)."""
+    """Every tile width the autotuner may pick gives the same bits for every
+    instance (each column is computed by the same code whatever the
+    workgroup holds); the instances start from different fields, so a tile
+    given to the wrong instance shows."""
+    fields = []
+    for i in range(B):
+        q = hip.default_params(model=0, L=L, M=L, precision=prec, grainx=4, grainy=4, u0=0.02 + 0.01 * (i % 7),
+                               seed=100 + i)
+        fields.append(hip.init_fields(q))
     p = hip.default_params(model=0, L=L, M=L, precision=prec, grainx=4, grainy=4)
-    phi, con = hip.init_fields(p)
     out = {}
     for cand in (0, 1, 2):
         s = hip.Sim(p, batch=B)
         for i in range(B):
-            s.set_fields(phi, con, inst=i)
+            s.set_fields(*fields[i], inst=i)
         try:
             s.set_col_tiles(cand)
         except hip.PfcError:
             s.close()
             continue
         s.step(6)
-        out[(cand, s.col_tiles())] = s.get_fields(B - 1)
+        out[(cand, s.col_tiles())] = [s.get_fields(i) for i in range(B)]
         s.close()
-    print(f"prec={prec} L={L} B={B} widths tried: {sorted(k[1] for k in out)}", flush=True)
+    print(f"prec={prec} L={L} B={B} candidates tried (candidate, columns): {sorted(out)}", flush=True)
     assert len(out) >= 2
     ref = next(iter(out.values()))
     for k, v in out.items():
-        assert np.array_equal(v[0], ref[0]) and np.array_equal(v[1], ref[1]), k
+        for i in range(B):
+            assert np.array_equal(v[i][0], ref[i][0]) and np.array_equal(v[i][1], ref[i][1]), (k, i)
 
 
 def test_group_graph_equals_separate(hip):
