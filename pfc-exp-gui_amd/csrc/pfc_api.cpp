@@ -2,6 +2,7 @@
 // energy, graphs and kernel timing.  See include/pfc_hip.h for the contract.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdarg>
@@ -9,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -286,7 +288,10 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) try {
     const int nrow = row_blocks(prec, p->M, p->L * p->N), ncol = col_blocks(prec, p->L, p->M, p->N);
     if (nrow <= 0 || ncol <= 0) return fail(PFC_E_UNSUPPORTED, "no launch shape for %dx%dx%d", p->L, p->M, p->N);
 
-    pfc_ctx* c = new pfc_ctx();
+    // The context is freed on every failed return and on a host allocation
+    // failure thrown anywhere below; released to the caller at the end.
+    std::unique_ptr<pfc_ctx, void (*)(pfc_ctx*)> owner(new pfc_ctx(), free_ctx);
+    pfc_ctx* c = owner.get();
     c->p = *p;
     c->m = m;
     c->B = batch;
@@ -329,10 +334,7 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) try {
     if (e == hipSuccess) e = hipMemsetAsync(c->S, 0, nvec * c->vsize, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->AB, 0, nvec * c->vsize, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->IN, 0, nvec * c->vsize, c->stream);
-    if (e != hipSuccess) {
-        free_ctx(c);
-        return fail(PFC_E_NOMEM, "device allocation failed: %s", hipGetErrorString(e));
-    }
+    if (e != hipSuccess) return fail(PFC_E_NOMEM, "device allocation failed: %s", hipGetErrorString(e));
     if (prec == PREC_FP64) {
         c->twH = upload_twiddles<double>(c->M / 2, &e);
         if (e == hipSuccess) c->twM = upload_twiddles<double>(c->M, &e);
@@ -357,10 +359,7 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) try {
     if (e == hipSuccess) e = hipMemcpy(c->c2_d, c2.data(), sizeof(float) * c2.size(), hipMemcpyHostToDevice);
     c->means_h.assign(4 * (size_t)batch, 0.0);
     if (e == hipSuccess) e = hipMemcpy(c->means_d, c->means_h.data(), sizeof(double) * 4 * batch, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        free_ctx(c);
-        return fail(PFC_E_HIP, "table upload failed: %s", hipGetErrorString(e));
-    }
+    if (e != hipSuccess) return fail(PFC_E_HIP, "table upload failed: %s", hipGetErrorString(e));
     PhysArgs& ph = c->phys;
     ph.niu = m.niu; ph.ksi = m.ksi; ph.w = m.w; ph.c0 = m.c0; ph.alpha = m.alpha;
     ph.mn_dt = m.mn * m.dt;
@@ -395,14 +394,11 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) try {
             e = launch_coef(prec, b, ph, c->stream);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) {
-            free_ctx(c);
-            return fail(PFC_E_NOMEM, "coefficient tables: %s", hipGetErrorString(e));
-        }
+        if (e != hipSuccess) return fail(PFC_E_NOMEM, "coefficient tables: %s", hipGetErrorString(e));
     }
     b.col_ts = 0;
     b.col_ts = tune_cols(c);
-    *out = c;
+    *out = owner.release();
     return PFC_OK;
 } PFC_HOST_CATCH
 
@@ -589,7 +585,8 @@ extern "C" int pfc_group_create(pfc_ctx** ctxs, int n, int chunk, pfc_group** ou
             if (ctxs[j] == ctxs[i]) return fail(PFC_E_ARG, "context %d given twice", i);
         if (ctxs[i]->time_kid >= 0) return fail(PFC_E_ARG, "kernel timing is on for context %d", i);
     }
-    pfc_group* g = new pfc_group();
+    std::unique_ptr<pfc_group, void (*)(pfc_group*)> owner(new pfc_group(), free_group);
+    pfc_group* g = owner.get();
     g->ctxs.assign(ctxs, ctxs + n);
     g->chunk = chunk;
     g->ev.assign(2 * (size_t)n + 1, nullptr);
@@ -597,16 +594,22 @@ extern "C" int pfc_group_create(pfc_ctx** ctxs, int n, int chunk, pfc_group** ou
     for (auto& ev : g->ev)
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
     for (int i = 0; i < n && e == hipSuccess; i++) e = hipStreamSynchronize(ctxs[i]->stream);
-    if (e != hipSuccess) {
-        free_group(g);
-        return fail(PFC_E_HIP, "group set-up: %s", hipGetErrorString(e));
-    }
+    if (e != hipSuccess) return fail(PFC_E_HIP, "group set-up: %s", hipGetErrorString(e));
+    // Branches are captured largest first (points x instances): the device
+    // starts on the longest chain and the shorter ones fill in beside it,
+    // instead of the longest running alone at the end of every launch.
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (double)ctxs[a]->B * ctxs[a]->L * ctxs[a]->M * ctxs[a]->N >
+               (double)ctxs[b]->B * ctxs[b]->L * ctxs[b]->M * ctxs[b]->N;
+    });
     hipStream_t s0 = ctxs[0]->stream;
     e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         hipError_t ec = hipEventRecord(g->ev[0], s0);
         for (int i = 1; i < n && ec == hipSuccess; i++) ec = hipStreamWaitEvent(ctxs[i]->stream, g->ev[0], 0);
-        for (int i = 0; i < n && ec == hipSuccess; i++)
+        for (int i : order)
             for (int k = 0; k < chunk && ec == hipSuccess; k++) ec = enqueue_step(ctxs[i], ROW_WRITE_IN, 0, 0);
         for (int i = 1; i < n && ec == hipSuccess; i++) {
             ec = hipEventRecord(g->ev[i], ctxs[i]->stream);
@@ -619,11 +622,8 @@ extern "C" int pfc_group_create(pfc_ctx** ctxs, int n, int chunk, pfc_group** ou
     // Upload now, so that the first pfc_group_step does not carry it.
     if (e == hipSuccess) e = hipGraphUpload(g->gexec, ctxs[0]->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctxs[0]->stream);
-    if (e != hipSuccess) {
-        free_group(g);
-        return fail(PFC_E_HIP, "group graph capture: %s", hipGetErrorString(e));
-    }
-    *out = g;
+    if (e != hipSuccess) return fail(PFC_E_HIP, "group graph capture: %s", hipGetErrorString(e));
+    *out = owner.release();
     return PFC_OK;
 } PFC_HOST_CATCH
 

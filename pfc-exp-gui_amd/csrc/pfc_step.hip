@@ -274,6 +274,9 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #ifndef PFC_XCD_REMAP
 #define PFC_XCD_REMAP 1  // workgroup x takes tile (x % 8)*(nx/8) + x/8: one XCD's tiles are adjacent
 #endif
+#ifndef PFC_COL_TWO
+#define PFC_COL_TWO 1  // FP32 plain column step: transforms of two fields at a time (loads/stores overlap them)
+#endif
 #ifndef PFC_COEF
 #define PFC_COEF 1  // plain steps read per-mode coefficient tables (k_coef) where col_coef holds
 #endif
@@ -851,6 +854,44 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     // rows t + GB*q, q < EB, in threads t < GB.
     constexpr bool COEF = PFC_COEF && LEAN && col_coef(sizeof(R) == 8, LL);
     constexpr int EBc = C::EB, GBc = C::GB;
+
+    // TWO (plain FP32 step): the transforms take two fields at a time in the
+    // same per-field arithmetic, so {psi, con} is transformed while the
+    // nonlinear terms are still arriving and the new state's stores drain
+    // while C2A/C2B x psi is transformed back.  Global accesses complete in
+    // issue order (one counter for loads and stores), so nothing the
+    // transforms wait for may be issued after the data: the twiddles are
+    // loaded first and read from LDS, then the state, the nonlinear terms and
+    // the coefficients, in that order.  Power-of-two lengths, and only where
+    // the extra LDS keeps two default-tile workgroups per CU (16x512^2:
+    // 42.7 -> 41.3 us per launch; the 3*2^k plans measured slower).  Decided
+    // per length, never per tile width: the two forms round differently
+    // (the compiler fuses a few multiply-adds differently), and every tile
+    // width must give the same bits.
+    constexpr size_t TWB = (size_t)LL * sizeof(Cx<R>);
+    constexpr bool TWO = PFC_COL_TWO && COEF && sizeof(R) == 4 && !C::SPLIT && !C::GEN &&
+                         ColCfg<R, LL, 0>::LDS + TWB <= 80 * 1024;
+    static_assert(!TWO || C::LDS + TWB <= 160 * 1024, "every tile width of a two-field length fits the LDS");
+    __shared__ __attribute__((aligned(16))) Cx<R> tws[TWO ? LL : 1];
+    const LdsBuf<R, 2, 1, 0, C::WAVE> lds2{smem + col * C::CB};
+    constexpr int NTW = (LL + NT - 1) / NT;
+    Cx<R> twv[NTW];
+    Vec<R, 2> xs[E], xi[E];
+    if constexpr (TWO) {
+#pragma unroll
+        for (int k = 0; k < NTW; k++) {
+            const int i = tid + k * NT;
+            if (i < LL) twv[k] = twL[i];
+        }
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < EA; q++) xs[q] = bload<Vec<R, 2>>(S, toff, (uint32_t)(G * q * CS) * es);
+        __asm__ volatile("" ::: "memory");
+        const Buf IN = mkbuf((const Vec<R, 2>*)b.IN + tbase, tbytes);
+#pragma unroll
+        for (int q = 0; q < EA; q++) xi[q] = bload<Vec<R, 2>>(IN, toff, (uint32_t)(G * q * CS) * es);
+        __asm__ volatile("" ::: "memory");
+    }
     float4 cf[COEF ? EBc : 1];
     float2 cc[COEF ? EBc : 1];
     if constexpr (COEF) {
@@ -865,23 +906,41 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
         }
     }
 
-    // Forward transform of {psi, con, inn, inc} along l in one pass.
+    // Forward transform of {psi, con, inn, inc} along l.
     Vec<R, 4> x[E];
+    if constexpr (TWO) {
 #pragma unroll
-    for (int q = 0; q < EA; q++) {
-        const uint32_t qo = (uint32_t)(G * q * CS) * es;  // uniform
-        const Vec<R, 2> sv = bload<Vec<R, 2>>(S, toff, qo);
-        x[q].c[0] = sv.c[0];
-        x[q].c[1] = sv.c[1];
-        if (step) {
-            const Vec<R, 2> iv = bload<Vec<R, 2>>(mkbuf((const Vec<R, 2>*)b.IN + tbase, tbytes), toff, qo);
-            x[q].c[2] = iv.c[0];
-            x[q].c[3] = iv.c[1];
-        } else {
-            x[q].c[2] = x[q].c[3] = Cx<R>{R(0), R(0)};
+        for (int k = 0; k < NTW; k++) {
+            const int i = tid + k * NT;
+            if (i < LL) tws[i] = twv[k];
         }
+        __syncthreads();
+        fft_ab<R, 2, LL, E, -1, true, C::GEN, C::PLAN>(xs, lds2, t, tws);
+        fft_ab<R, 2, LL, E, -1, true, C::GEN, C::PLAN>(xi, lds2, t, tws);
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            x[q].c[0] = xs[q].c[0];
+            x[q].c[1] = xs[q].c[1];
+            x[q].c[2] = xi[q].c[0];
+            x[q].c[3] = xi[q].c[1];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < EA; q++) {
+            const uint32_t qo = (uint32_t)(G * q * CS) * es;  // uniform
+            const Vec<R, 2> sv = bload<Vec<R, 2>>(S, toff, qo);
+            x[q].c[0] = sv.c[0];
+            x[q].c[1] = sv.c[1];
+            if (step) {
+                const Vec<R, 2> iv = bload<Vec<R, 2>>(mkbuf((const Vec<R, 2>*)b.IN + tbase, tbytes), toff, qo);
+                x[q].c[2] = iv.c[0];
+                x[q].c[3] = iv.c[1];
+            } else {
+                x[q].c[2] = x[q].c[3] = Cx<R>{R(0), R(0)};
+            }
+        }
+        fft_ab<R, 4, LL, E, -1, true, C::GEN, C::PLAN>(x, lds, t, twL);
     }
-    fft_ab<R, 4, LL, E, -1, true, C::GEN, C::PLAN>(x, lds, t, twL);
     Cx<R> nz[E];
 #pragma unroll
     for (int q = 0; q < E; q++) nz[q] = Cx<R>{R(0), R(0)};
@@ -907,9 +966,36 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     if (!LEAN && mode == COL_ENERGY_ONLY) return;
 
     // Inverse transform of the new state and of C2A/C2B times the new psi.
-    fft_ab<R, 4, LL, E, +1, false, C::GEN, C::PLAN>(x, lds, t, twL);
     const R invL = R(1) / R(LL);
     const Buf AB = mkbuf((const Vec<R, 2>*)b.AB + tbase, tbytes);
+    if constexpr (TWO) {
+        Vec<R, 2> ys[E], ya[E];
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            ys[q].c[0] = x[q].c[0];
+            ys[q].c[1] = x[q].c[1];
+            ya[q].c[0] = x[q].c[2];
+            ya[q].c[1] = x[q].c[3];
+        }
+        fft_ab<R, 2, LL, E, +1, false, C::GEN, C::PLAN>(ys, lds2, t, tws);
+#pragma unroll
+        for (int q = 0; q < EA; q++) {
+            Vec<R, 2> sv;
+            sv.c[0] = Cx<R>{ys[q].c[0].x * invL, ys[q].c[0].y * invL};
+            sv.c[1] = Cx<R>{ys[q].c[1].x * invL, ys[q].c[1].y * invL};
+            bstore(sv, S, toff, (uint32_t)(G * q * CS) * es);
+        }
+        fft_ab<R, 2, LL, E, +1, false, C::GEN, C::PLAN>(ya, lds2, t, tws);
+#pragma unroll
+        for (int q = 0; q < EA; q++) {
+            Vec<R, 2> av;
+            av.c[0] = Cx<R>{ya[q].c[0].x * invL, ya[q].c[0].y * invL};
+            av.c[1] = Cx<R>{ya[q].c[1].x * invL, ya[q].c[1].y * invL};
+            bstore(av, AB, toff, (uint32_t)(G * q * CS) * es);
+        }
+        return;
+    }
+    fft_ab<R, 4, LL, E, +1, false, C::GEN, C::PLAN>(x, lds, t, twL);
 #pragma unroll
     for (int q = 0; q < EA; q++) {
         const uint32_t qo = (uint32_t)(G * q * CS) * es;
