@@ -79,3 +79,31 @@ def test_fp64_dropin_vs_oracle_2000_steps(hip, oracle, bench):
         assert err < 5e-5
         assert abs(g[0].mean() - o[0].mean()) < 1e-14
     s.close()
+
+
+@pytest.mark.parametrize("model,L,M", [(0, 128, 128), (1, 128, 384)])
+def test_fp32_one_step_round_off(hip, bench, model, L, M):
+    """One FP32 step against one FP64 step without the narrowing, from the same
+    fields: the FP32 arithmetic's own error before any dynamics amplifies it.
+    CVD 128x384 shows FP32 distances of 1e-3 after 50 steps, but its FP64
+    ensemble (fields one FP32 round-off apart) spreads from 1e-7 at step 10
+    to 7e-5 at step 50 as well; a single step stays within a few FP32
+    round-offs of the field scale on both models (measured 6.7 Misfit, 9.9
+    CVD), so the CVD distance is the
+    trajectory's amplification of per-step round-off, not an arithmetic loss
+    of the FP32 path (its logarithms, 1 - c)."""
+    p32 = bench._acc_params(hip, model, L, M, hip.FP32, 1)
+    p64 = bench._acc_params(hip, model, L, M, hip.FP64, 0)
+    phi, con = hip.init_fields(p64)
+    out = {}
+    for tag, p in (("fp32", p32), ("fp64", p64)):
+        s = hip.Sim(p)
+        s.set_fields(phi, con)
+        s.step(1)
+        out[tag] = s.get_fields()
+        s.close()
+    scale = max(np.abs(phi).max(), np.abs(con).max())
+    err = max(np.abs(out["fp32"][0] - out["fp64"][0]).max(), np.abs(out["fp32"][1] - out["fp64"][1]).max())
+    print(f"model {model} {L}x{M}: one FP32 step vs FP64 {err:.2e} = {err / (bench.EPS32 * scale):.1f} "
+          f"FP32 round-offs of the field scale {scale:.3f}", flush=True)
+    assert err <= 16 * bench.EPS32 * scale
