@@ -277,6 +277,12 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #ifndef PFC_COL_TWO
 #define PFC_COL_TWO 1  // FP32 plain column step: transforms of two fields at a time (loads/stores overlap them)
 #endif
+#ifndef PFC_COL_TWO_LDS16
+#define PFC_COL_TWO_LDS16 1  // the two-field step's exchange buffers hold 16 bytes per slot (else 32)
+#endif
+#ifndef PFC_COL_COEF_LATE
+#define PFC_COL_COEF_LATE 1  // the two-field step loads the mode coefficients after the forward transforms
+#endif
 #ifndef PFC_COEF
 #define PFC_COEF 1  // plain steps read per-mode coefficient tables (k_coef) where col_coef holds
 #endif
@@ -389,9 +395,26 @@ struct ColCfg {
     // noise value per slot) reuses the space from the start.
     static constexpr size_t MIRROR = (size_t)PB * (sizeof(Vec<R, 4>) + sizeof(Cx<R>));
     static constexpr size_t LDS = (size_t)T * CB > MIRROR ? (size_t)T * CB : MIRROR;
+    // The two-field column step's buffers: 16 bytes per slot per column
+    // (columns PS1 slots apart, staggered by PFC_LDS_MOD bank groups like the
+    // chunked buffers), and the packed column's mirror exchange from the start.
+    static constexpr size_t CB2 = PFC_COL_TWO_LDS16 ? (size_t)PS1 * 16 : CB;
+    static constexpr size_t LDS2 = (size_t)T * CB2 > MIRROR ? (size_t)T * CB2 : MIRROR;
     static constexpr int WPE = (!GEN && E == 12) ? PFC_WPE12 : PFC_WPE;
     static constexpr int WPE_LEAN = (!GEN && E == 12) ? PFC_WPE12 : PFC_COL_WPE_LEAN;
 };
+
+// The two-field column step (k_col, TWO): FP32 lengths on the power-of-two
+// plans whose default tile, with its twiddle table, keeps two workgroups per
+// CU.  Decided per length, never per tile width: the two forms round
+// differently (the compiler fuses a few multiply-adds differently), and every
+// tile width must give the same bits.
+template <typename R, int LL>
+constexpr bool col_two_len() {
+    using C0 = ColCfg<R, LL, 0>;
+    return PFC_COL_TWO && sizeof(R) == 4 && !C0::SPLIT && !C0::GEN &&
+           C0::LDS2 + (size_t)LL * sizeof(Cx<R>) <= 80 * 1024;
+}
 
 // ---------------------------------------------------------------------------
 // Real-to-complex helpers for one row of length M = 2H.
@@ -808,7 +831,9 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     constexpr bool LEAN = KIND != COL_GENERAL;
     using C = ColCfg<R, LL, TS>;
     constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT, EA = C::EA;
-    __shared__ __attribute__((aligned(16))) char smem[C::LDS];
+    constexpr bool COEF = PFC_COEF && LEAN && col_coef(sizeof(R) == 8, LL);
+    constexpr bool TWO = COEF && col_two_len<R, LL>();
+    __shared__ __attribute__((aligned(16))) char smem[TWO ? C::LDS2 : C::LDS];
     __shared__ double red[(NT + 63) / 64 + 1];
     const int tid = threadIdx.x;
     const int col = C::WAVE ? tid / G : tid % T, t = C::WAVE ? tid % G : tid / T;
@@ -849,31 +874,28 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     const bool use_nz = !LEAN && step && b.NZ != nullptr;
 
     // Plain FP32 steps take the modes' coefficients from cA/cC (same layout
-    // as one instance of S, loaded with the state so their latency overlaps),
-    // in the mapping the per-mode work sees (after the forward transform):
-    // rows t + GB*q, q < EB, in threads t < GB.
-    constexpr bool COEF = PFC_COEF && LEAN && col_coef(sizeof(R) == 8, LL);
+    // as one instance of S; loaded with the state, or after the two-field
+    // forward transforms, so their latency overlaps work), in the mapping the
+    // per-mode work sees (after the forward transform): rows t + GB*q,
+    // q < EB, in threads t < GB.
     constexpr int EBc = C::EB, GBc = C::GB;
 
-    // TWO (plain FP32 step): the transforms take two fields at a time in the
-    // same per-field arithmetic, so {psi, con} is transformed while the
-    // nonlinear terms are still arriving and the new state's stores drain
-    // while C2A/C2B x psi is transformed back.  Global accesses complete in
-    // issue order (one counter for loads and stores), so nothing the
+    // TWO (plain FP32 step, col_two_len): the transforms take two fields at a
+    // time in the same per-field arithmetic, so {psi, con} is transformed
+    // while the nonlinear terms are still arriving and the new state's stores
+    // drain while C2A/C2B x psi is transformed back.  Global accesses complete
+    // in issue order (one counter for loads and stores), so nothing the
     // transforms wait for may be issued after the data: the twiddles are
     // loaded first and read from LDS, then the state, the nonlinear terms and
-    // the coefficients, in that order.  Power-of-two lengths, and only where
-    // the extra LDS keeps two default-tile workgroups per CU (16x512^2:
-    // 42.7 -> 41.3 us per launch; the 3*2^k plans measured slower).  Decided
-    // per length, never per tile width: the two forms round differently
-    // (the compiler fuses a few multiply-adds differently), and every tile
-    // width must give the same bits.
+    // the coefficients (after the forward transforms, which do not need them),
+    // in that order.  With 16-byte exchange slots the default tiles need half
+    // the LDS.  16x512^2: 42.7 -> 40.9 us per launch, the four-class sweep
+    // 0.1064 -> 0.1005 ms per step, four 1024^2 instances 81.6 -> 77.7 us per
+    // step; one 1024^2 instance +2%; the 3*2^k plans measured slower.
     constexpr size_t TWB = (size_t)LL * sizeof(Cx<R>);
-    constexpr bool TWO = PFC_COL_TWO && COEF && sizeof(R) == 4 && !C::SPLIT && !C::GEN &&
-                         ColCfg<R, LL, 0>::LDS + TWB <= 80 * 1024;
-    static_assert(!TWO || C::LDS + TWB <= 160 * 1024, "every tile width of a two-field length fits the LDS");
+    static_assert(!TWO || C::LDS2 + TWB <= 160 * 1024, "every tile width of a two-field length fits the LDS");
     __shared__ __attribute__((aligned(16))) Cx<R> tws[TWO ? LL : 1];
-    const LdsBuf<R, 2, 1, 0, C::WAVE> lds2{smem + col * C::CB};
+    const LdsBuf<R, 2, 1, 0, C::WAVE> lds2{smem + col * C::CB2};
     constexpr int NTW = (LL + NT - 1) / NT;
     Cx<R> twv[NTW];
     Vec<R, 2> xs[E], xi[E];
@@ -894,17 +916,21 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     }
     float4 cf[COEF ? EBc : 1];
     float2 cc[COEF ? EBc : 1];
-    if constexpr (COEF) {
-        if (GBc == G || t < GBc) {
-            const Buf CA = mkbuf((const float4*)b.cA + tile0, 0), CC = mkbuf((const float2*)b.cC + tile0, 0);
-            const uint32_t ca_off = (uint32_t)(col + t * CS) * 16u, cc_off = (uint32_t)(col + t * CS) * 8u;
+    auto load_coef = [&]() {
+        if constexpr (COEF) {
+            if (GBc == G || t < GBc) {
+                const Buf CA = mkbuf((const float4*)b.cA + tile0, 0), CC = mkbuf((const float2*)b.cC + tile0, 0);
+                const uint32_t ca_off = (uint32_t)(col + t * CS) * 16u, cc_off = (uint32_t)(col + t * CS) * 8u;
 #pragma unroll
-            for (int q = 0; q < EBc; q++) {
-                cf[q] = bload<float4>(CA, ca_off, (uint32_t)(GBc * q * CS) * 16u);
-                cc[q] = bload<float2>(CC, cc_off, (uint32_t)(GBc * q * CS) * 8u);
+                for (int q = 0; q < EBc; q++) {
+                    cf[q] = bload<float4>(CA, ca_off, (uint32_t)(GBc * q * CS) * 16u);
+                    cc[q] = bload<float2>(CC, cc_off, (uint32_t)(GBc * q * CS) * 8u);
+                }
             }
         }
-    }
+    };
+    constexpr bool COEF_LATE = TWO && PFC_COL_COEF_LATE;
+    if constexpr (!COEF_LATE) load_coef();
 
     // Forward transform of {psi, con, inn, inc} along l.
     Vec<R, 4> x[E];
@@ -917,6 +943,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
         __syncthreads();
         fft_ab<R, 2, LL, E, -1, true, C::GEN, C::PLAN>(xs, lds2, t, tws);
         fft_ab<R, 2, LL, E, -1, true, C::GEN, C::PLAN>(xi, lds2, t, tws);
+        if constexpr (COEF_LATE) load_coef();
 #pragma unroll
         for (int q = 0; q < E; q++) {
             x[q].c[0] = xs[q].c[0];
