@@ -6,6 +6,7 @@ the job run alone in a one-instance context of the same precision -- bit
 for bit, since every instance runs the same kernels on its own data.
 """
 import os
+import shutil
 import time
 
 import numpy as np
@@ -145,6 +146,34 @@ def test_outdir_files_and_cancel(hip, tmp_path):
     start = pos + 1 + 4
     vals = np.frombuffer(raw[start:start + 4 * phi.size], dtype=np.float32).reshape(p.M, p.L)  # [m][l]
     assert np.array_equal(vals, phi.reshape(p.L, p.M).T.astype(np.float32))
+    srv.close()
+
+
+def test_failing_job_fails_alone(hip, tmp_path):
+    """Two jobs of one class checkpoint in the same rounds of one batch
+    context; the first one's output directory is replaced by a plain file
+    right after submission, so its checkpoint files cannot be written.  It
+    ends FAILED with a file error; the other ends DONE with its files and the
+    same result as run alone (a job's failure is its own, not its batch's)."""
+    srv = hip.Server(max_slots=4, graph_chunk=5, quantum=10)
+    pa = hip.default_params(model=hip.MISFIT, L=64, M=64, precision=hip.FP32, grainx=4, grainy=4, seed=31)
+    pb = hip.default_params(model=hip.MISFIT, L=64, M=64, precision=hip.FP32, grainx=4, grainy=4, seed=32)
+    bad, good = tmp_path / "bad", tmp_path / "good"
+    a = srv.submit(pa, 2000, mod=200, outdir=str(bad))
+    b = srv.submit(pb, 2000, mod=200, keep_fields=True, outdir=str(good))
+    shutil.rmtree(bad, ignore_errors=True)
+    bad.write_text("not a directory\n")
+    srv.drain(120)
+    sa, sb = srv.poll(a), srv.poll(b)
+    print(f"failing job: state {sa.state} error {srv.error(a)!r}; other job: state {sb.state}", flush=True)
+    assert sa.state == hip.JOB_FAILED and "files" in srv.error(a)
+    assert sb.state == hip.JOB_DONE and sb.steps_done == 2000, srv.error(b)
+    phi, con = srv.result(b)
+    (aphi, acon), e = _alone(hip, pb, 2000)
+    assert np.array_equal(phi, aphi) and np.array_equal(con, acon)
+    assert sb.energy == e[0]
+    names = set(os.listdir(good))
+    assert {"phi_0_0.vti", "phi_1000_0.vti", "phi_2000_0.vti", "Phimax_2000.txt", "energy.txt"} <= names
     srv.close()
 
 
