@@ -274,6 +274,9 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #ifndef PFC_XCD_REMAP
 #define PFC_XCD_REMAP 1  // workgroup x takes tile (x % 8)*(nx/8) + x/8: one XCD's tiles are adjacent
 #endif
+#ifndef PFC_ROW_TWO
+#define PFC_ROW_TWO 1  // FP32 row pass: the c2r of psi, con runs while a, b still load (twiddles in LDS)
+#endif
 #ifndef PFC_COL_TWO
 #define PFC_COL_TWO 1  // FP32 plain column step: transforms of two fields at a time (loads/stores overlap them)
 #endif
@@ -508,8 +511,71 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
     constexpr int EA = C::EA, EB = C::EB, GB = C::GB;
     const bool actB = GB == G || t < GB;  // holds side-B elements
     Vec<R, 4> z[E];
+    // TWO (FP32, power-of-two rows): the same c2r two fields at a time, so
+    // psi and con are transformed while a and b still arrive.  Loads complete
+    // in issue order, so the split twiddles and the transform's twiddle table
+    // (staged in LDS) are loaded first, then S, then AB.
+    constexpr bool TWO = PFC_ROW_TWO && sizeof(R) == 4 && !C::GEN;
+    __shared__ __attribute__((aligned(16))) Cx<R> tws[TWO ? H : 1];
+    if constexpr (TWO) {
+        constexpr int NTW = (H + C::NT - 1) / C::NT;
+        constexpr uint32_t es = sizeof(Vec<R, 2>);
+        Cx<R> twv[NTW], wm[EA];
+        Vec<R, 2> s0[EA], s1[EA], a0[EA], a1[EA];
 #pragma unroll
-    for (int q = 0; q < EA; q++) {
+        for (int k2 = 0; k2 < NTW; k2++) {
+            const int i = tid + k2 * C::NT;
+            if (i < H) twv[k2] = twH[i];
+        }
+#pragma unroll
+        for (int q = 0; q < EA; q++) wm[q] = twM[t + G * q];
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < EA; q++) {
+            const int k = t + G * q, kc = (H - k) % H;
+            s0[q] = bload<Vec<R, 2>>(S, (roff + k) * es, 0u);
+            s1[q] = bload<Vec<R, 2>>(S, (roff + kc) * es, 0u);
+        }
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < EA; q++) {
+            const int k = t + G * q, kc = (H - k) % H;
+            a0[q] = bload<Vec<R, 2>>(AB, (roff + k) * es, 0u);
+            a1[q] = bload<Vec<R, 2>>(AB, (roff + kc) * es, 0u);
+        }
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int k2 = 0; k2 < NTW; k2++) {
+            const int i = tid + k2 * C::NT;
+            if (i < H) tws[i] = twv[k2];
+        }
+        __syncthreads();
+        const LdsBuf<R, 2, 1, 0, C::WAVE> lds2{smem + rl * C::RB};
+        Vec<R, 2> zs[E], za[E];
+#pragma unroll
+        for (int q = 0; q < EA; q++) {
+            const int k = t + G * q;
+            zs[q].c[0] = c2r_split_packed(s0[q].c[0], s1[q].c[0], k, wm[q]);
+            zs[q].c[1] = c2r_split_packed(s0[q].c[1], s1[q].c[1], k, wm[q]);
+        }
+        fft_ab<R, 2, H, E, +1, true, C::GEN, C::PLAN>(zs, lds2, t, tws);
+#pragma unroll
+        for (int q = 0; q < EA; q++) {
+            const int k = t + G * q;
+            za[q].c[0] = c2r_split_packed(a0[q].c[0], a1[q].c[0], k, wm[q]);
+            za[q].c[1] = c2r_split_packed(a0[q].c[1], a1[q].c[1], k, wm[q]);
+        }
+        fft_ab<R, 2, H, E, +1, true, C::GEN, C::PLAN>(za, lds2, t, tws);
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            z[q].c[0] = zs[q].c[0];
+            z[q].c[1] = zs[q].c[1];
+            z[q].c[2] = za[q].c[0];
+            z[q].c[3] = za[q].c[1];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < (TWO ? 0 : EA); q++) {
         const int k = t + G * q;
         const int kc = (H - k) % H;
         constexpr uint32_t es = sizeof(Vec<R, 2>);
@@ -521,7 +587,9 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
         z[q].c[2] = c2r_split_packed(a0.c[0], a1.c[0], k, w);
         z[q].c[3] = c2r_split_packed(a0.c[1], a1.c[1], k, w);
     }
-    fft_ab<R, 4, H, E, +1, true, C::GEN, C::PLAN>(z, LdsBuf<R, 4, C::CH, C::PS, C::WAVE>{smem + rl * C::RB}, t, twH);
+    if constexpr (!TWO)
+        fft_ab<R, 4, H, E, +1, true, C::GEN, C::PLAN>(z, LdsBuf<R, 4, C::CH, C::PS, C::WAVE>{smem + rl * C::RB}, t,
+                                                       twH);
 
     const double* mu = b.means + 4 * inst;
     const R mph = (R)mu[0], mco = (R)mu[1], ma = (R)mu[2], mb = (R)mu[3];
@@ -545,7 +613,7 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
         w[q].c[1] = Cx<R>{inc[0], inc[1]};
     }
     if (flags & ROW_WRITE_IN)
-        row_r2c_store<R, C, 2>(w, (Vec<R, 2>*)(smem + rl * C::RB), t, twH, twM,
+        row_r2c_store<R, C, 2>(w, (Vec<R, 2>*)(smem + rl * C::RB), t, TWO ? tws : twH, twM,
                                   mkbuf((Vec<R, 2>*)b.IN + bbase, rows_bytes), roff);
     if (want_e) {
         const double se = block_sum<C::NT>(esum, red);
