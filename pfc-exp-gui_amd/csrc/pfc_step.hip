@@ -275,7 +275,11 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #define PFC_XCD_REMAP 1  // workgroup x takes tile (x % 8)*(nx/8) + x/8: one XCD's tiles are adjacent
 #endif
 #ifndef PFC_ROW_TWO
-#define PFC_ROW_TWO 1  // FP32 row pass: the c2r of psi, con runs while a, b still load (twiddles in LDS)
+#define PFC_ROW_TWO 1  // row pass: the c2r of psi, con runs while a, b still load (twiddles in LDS)
+#endif
+#ifndef PFC_TWO_F64
+#define PFC_TWO_F64 0  // 1: the two-field row and column forms on FP64 too (they spill there: one FP64
+                       // 512^2 step 28.4 -> 30.5 us)
 #endif
 #ifndef PFC_COL_TWO
 #define PFC_COL_TWO 1  // FP32 plain column step: transforms of two fields at a time (loads/stores overlap them)
@@ -398,10 +402,13 @@ struct ColCfg {
     // noise value per slot) reuses the space from the start.
     static constexpr size_t MIRROR = (size_t)PB * (sizeof(Vec<R, 4>) + sizeof(Cx<R>));
     static constexpr size_t LDS = (size_t)T * CB > MIRROR ? (size_t)T * CB : MIRROR;
-    // The two-field column step's buffers: 16 bytes per slot per column
-    // (columns PS1 slots apart, staggered by PFC_LDS_MOD bank groups like the
-    // chunked buffers), and the packed column's mirror exchange from the start.
-    static constexpr size_t CB2 = PFC_COL_TWO_LDS16 ? (size_t)PS1 * 16 : CB;
+    // The two-field column step's buffers: one two-field Vec per slot (16
+    // bytes FP32, two 16-byte chunks FP64), columns staggered by PFC_LDS_MOD
+    // bank groups like the chunked buffers, and the packed column's mirror
+    // exchange from the start.
+    static constexpr int CH2 = (int)(sizeof(Vec<R, 2>) / 16);
+    static constexpr int PS2 = soa_stride(LL, CH2, PFC_LDS_MOD ? PFC_LDS_MOD : 1);
+    static constexpr size_t CB2 = PFC_COL_TWO_LDS16 ? (size_t)CH2 * PS2 * 16 : CB;
     static constexpr size_t LDS2 = (size_t)T * CB2 > MIRROR ? (size_t)T * CB2 : MIRROR;
     static constexpr int WPE = (!GEN && E == 12) ? PFC_WPE12 : PFC_WPE;
     static constexpr int WPE_LEAN = (!GEN && E == 12) ? PFC_WPE12 : PFC_COL_WPE_LEAN;
@@ -415,7 +422,7 @@ struct ColCfg {
 template <typename R, int LL>
 constexpr bool col_two_len() {
     using C0 = ColCfg<R, LL, 0>;
-    return PFC_COL_TWO && sizeof(R) == 4 && !C0::SPLIT && !C0::GEN &&
+    return PFC_COL_TWO && (sizeof(R) == 4 || PFC_TWO_F64) && !C0::SPLIT && !C0::GEN &&
            C0::LDS2 + (size_t)LL * sizeof(Cx<R>) <= 80 * 1024;
 }
 
@@ -515,7 +522,7 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
     // psi and con are transformed while a and b still arrive.  Loads complete
     // in issue order, so the split twiddles and the transform's twiddle table
     // (staged in LDS) are loaded first, then S, then AB.
-    constexpr bool TWO = PFC_ROW_TWO && sizeof(R) == 4 && !C::GEN;
+    constexpr bool TWO = PFC_ROW_TWO && (sizeof(R) == 4 || PFC_TWO_F64) && !C::GEN;
     __shared__ __attribute__((aligned(16))) Cx<R> tws[TWO ? H : 1];
     if constexpr (TWO) {
         constexpr int NTW = (H + C::NT - 1) / C::NT;
@@ -550,7 +557,9 @@ k_row_step(DevBufs b, PhysArgs p, int flags) {
             if (i < H) tws[i] = twv[k2];
         }
         __syncthreads();
-        const LdsBuf<R, 2, 1, 0, C::WAVE> lds2{smem + rl * C::RB};
+        constexpr int CH2 = (int)(sizeof(Vec<R, 2>) / 16);
+        const LdsBuf<R, 2, CH2, PFC_LDS_MOD ? soa_stride(H, CH2, PFC_LDS_MOD) : padded(H), C::WAVE> lds2{
+            smem + rl * C::RB};
         Vec<R, 2> zs[E], za[E];
 #pragma unroll
         for (int q = 0; q < EA; q++) {
@@ -900,7 +909,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     using C = ColCfg<R, LL, TS>;
     constexpr int E = C::E, G = C::G, T = C::T, NT = C::NT, EA = C::EA;
     constexpr bool COEF = PFC_COEF && LEAN && col_coef(sizeof(R) == 8, LL);
-    constexpr bool TWO = COEF && col_two_len<R, LL>();
+    constexpr bool TWO = LEAN && col_two_len<R, LL>();
     __shared__ __attribute__((aligned(16))) char smem[TWO ? C::LDS2 : C::LDS];
     __shared__ double red[(NT + 63) / 64 + 1];
     const int tid = threadIdx.x;
@@ -963,7 +972,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     constexpr size_t TWB = (size_t)LL * sizeof(Cx<R>);
     static_assert(!TWO || C::LDS2 + TWB <= 160 * 1024, "every tile width of a two-field length fits the LDS");
     __shared__ __attribute__((aligned(16))) Cx<R> tws[TWO ? LL : 1];
-    const LdsBuf<R, 2, 1, 0, C::WAVE> lds2{smem + col * C::CB2};
+    const LdsBuf<R, 2, C::CH2, C::PS2, C::WAVE> lds2{smem + col * C::CB2};
     constexpr int NTW = (LL + NT - 1) / NT;
     Cx<R> twv[NTW];
     Vec<R, 2> xs[E], xi[E];
