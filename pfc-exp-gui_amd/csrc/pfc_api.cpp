@@ -202,6 +202,7 @@ static int tune_cols(pfc_ctx* c) {
     const char* env = std::getenv("PFC_AUTOTUNE");
     if (env && env[0] == '0') return 0;
     if (c->noise) return 0;  // steps with the noise term run the general kernel
+    if (fs_col(c->prec, c->L)) return 0;  // the field-split column pass has one tile width
     if (col_coef(c->prec == PREC_FP64, c->L) && !c->bufs.cA) return 0;  // no plain-step kernel in use
     const std::array<int, 5> key{c->prec, c->L, c->M, c->N, c->B};
     {

@@ -72,11 +72,26 @@ enum { ROW_WRITE_IN = 1, ROW_ENERGY = 2 };
 enum { COL_INIT = 0, COL_STEP = 1, COL_ENERGY_ONLY = 2 };
 enum { COL_ENERGY = 1 };
 
+// Field-split passes (pfc_fs.hip): one group of threads per field, at every
+// batch size.  They run the FP64 row pass on power-of-two rows and the FP32
+// row pass on 1024-point rows (one row per workgroup), and the FP64 plain and
+// energy-traced column steps on power-of-two columns (one column per
+// workgroup).  FP32 columns keep the four-field passes: a field-split FP32
+// column moves 8-byte pieces per lane (one cache line per lane and row), and
+// measured slower (one 1024^2 instance 20.4 -> 27.5 us per column pass).
+constexpr bool pow2_len(int n) { return n > 0 && (n & (n - 1)) == 0; }
+constexpr bool fs_row(int prec, int H) {  // H = M/2
+    return prec == PREC_FP64 ? pow2_len(H) && H >= 16 && H <= 512 : H == 512;
+}
+constexpr bool fs_col(int prec, int L) { return prec == PREC_FP64 && pow2_len(L) && L >= 32 && L <= 1024; }
+
 // Whether the plain-step column kernels of this precision and length read
-// per-mode coefficient tables: FP32, every length.  (FP64 tables for the
-// powers of two below 1024 were built and timed: no gain on single
-// instances, whose columns are not held up by the C2 gather.)
-constexpr bool col_coef(bool f64, int L) { return !f64 && L > 0; }
+// per-mode coefficient tables: FP32, every length; FP64 where the
+// field-split column pass runs (it loads them at its start, so their
+// latency hides behind the state's).  (The four-field FP64 column kernels
+// measured no gain from them: their columns are not held up by the C2
+// gather.)
+constexpr bool col_coef(bool f64, int L) { return !f64 ? L > 0 : fs_col(PREC_FP64, L); }
 
 // Fill cA/cC from kl/km/kn/c2 and the step constants: FP32 {mk, mkc,
 // 1/(1 + mk), 1/(1 + den)}, FP64 {mk, mkc, den, 0} (floats, as the reference
@@ -114,6 +129,12 @@ hipError_t launch_energy_reduce(const DevBufs& b, int nrow, int ncol, double alp
 size_t noise_scratch_bytes(long P, int B);
 hipError_t launch_noise(int prec, const DevBufs& b, const unsigned* seeds, void* planes, void* nz, void* scratch,
                         int* err_flag, hipStream_t s);
+
+// The field-split passes (fs_row, fs_col above).
+hipError_t launch_row_fs(int prec, const DevBufs& b, const PhysArgs& p, int flags, hipStream_t s,
+                         hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
+hipError_t launch_col_fs(int prec, const DevBufs& b, const PhysArgs& p, bool want_e, hipStream_t s,
+                         hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 
 // Named kernels (for timing the dominant one).
 enum { KID_ROW = 0, KID_COL = 1, KID_DEPTH = 2 };
