@@ -47,13 +47,25 @@ struct FsRow {
 // TWL: the twiddle table is staged in LDS where that still leaves room for
 // two workgroups per CU (FP64 1024: 74 KB of planes; with the table, one
 // workgroup per CU: 33.7 -> 44.2 us per pass).
+#ifndef PFC_FS_PAIR_MIN
+#define PFC_FS_PAIR_MIN 256  // shortest column whose loads/stores go in lane pairs through the planes
+#endif
+#ifndef PFC_FS_COL512_E
+#define PFC_FS_COL512_E 8  // elements per thread of 512-point field-split columns
+#endif
 template <typename R, int LL>
 struct FsCol {
-    static constexpr int E = fs_e(LL), G = LL / E, NT = 4 * G, PB = padded(LL);
+    static constexpr int E = LL == 512 ? PFC_FS_COL512_E : fs_e(LL), G = LL / E, NT = 4 * G, PB = padded(LL);
     static constexpr int MP = E / 4;  // modes per thread, update phase
     static constexpr bool WAVE = G <= 64 && 64 % G == 0;  // one field's transform inside one wavefront
-    static_assert(WAVE, "one field's transform inside one wavefront");
+    static_assert(WAVE || G % 64 == 0, "whole wavefronts per field");
     static constexpr bool TWL = (4 * PB + LL) * sizeof(Cx<R>) <= 80 * 1024;
+    // PAIR: the column's loads and stores go in pieces of which lane pairs
+    // cover one row's two fields, through the planes.  With one column per
+    // workgroup every row is a separate cache line for each array; split by
+    // field, each line is asked for twice (once per field group).  FP64
+    // 512: 14.4 -> 12.9 us per pass, 1024: 37.7 -> 32.8; 128: +0.3 us.
+    static constexpr bool PAIR = LL >= PFC_FS_PAIR_MIN;
 };
 
 // Row pass, one row of length M = 2H per workgroup (row blockIdx.x of
@@ -226,7 +238,25 @@ __global__ void __launch_bounds__((FsCol<R, LL>::NT)) k_col_fs(DevBufs b, PhysAr
         }
     }
     Vec<R, 1> x[E];
-    {
+    // PAIR: every thread loads pieces of S and IN in which lane pairs cover
+    // one row's two fields (one request per row and array instead of two),
+    // and the pieces reach their field groups through the planes.
+    constexpr int NL = 4 * LL / NT;  // pieces per thread (2 arrays x L rows x 2 fields)
+    if constexpr (C::PAIR) {
+        const Buf s0 = mkbuf((const Vec<R, 2>*)b.S + tbase, 0), s1 = mkbuf((const Vec<R, 2>*)b.IN + tbase, 0);
+        Cx<R> pc[NL];
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int gi = tid + NT * u, h = gi & 1, row = (gi >> 1) % LL;
+            const uint32_t off = (uint32_t)row * rs + (uint32_t)h * (uint32_t)sizeof(Cx<R>);
+            pc[u] = (gi >> 1) / LL ? bload<Cx<R>>(s1, off, 0u) : bload<Cx<R>>(s0, off, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int gi = tid + NT * u;
+            pl[2 * ((gi >> 1) / LL) + (gi & 1)][pidx((gi >> 1) % LL)] = pc[u];
+        }
+    } else {
         const Buf src = mkbuf((const Vec<R, 2>*)(f < 2 ? b.S : b.IN) + tbase, 0);
 #pragma unroll
         for (int q = 0; q < E; q++) x[q].c[0] = bload<Cx<R>>(src, (uint32_t)t * rs + fo, (uint32_t)(G * q) * rs);
@@ -234,7 +264,11 @@ __global__ void __launch_bounds__((FsCol<R, LL>::NT)) k_col_fs(DevBufs b, PhysAr
     if constexpr (C::TWL) {
 #pragma unroll
         for (int k2 = 0; k2 < NTW; k2++) twS[tid + k2 * NT] = tv[k2];
-        __syncthreads();
+    }
+    if (C::TWL || C::PAIR) __syncthreads();
+    if constexpr (C::PAIR) {
+#pragma unroll
+        for (int q = 0; q < E; q++) x[q].c[0] = pl[f][pidx(t + G * q)];
     }
     const Cx<R>* twL = C::TWL ? twS : (const Cx<R>*)b.twL;
 
@@ -302,7 +336,21 @@ __global__ void __launch_bounds__((FsCol<R, LL>::NT)) k_col_fs(DevBufs b, PhysAr
     for (int q = 0; q < E; q++) x[q].c[0] = pl[f][pidx(t + G * q)];
     fft_reg<R, 1, LL, E, +1>(x, lds, t, twL);
     const R invL = R(1) / R(LL);
-    {
+    if constexpr (C::PAIR) {
+        xsync<C::WAVE>();
+#pragma unroll
+        for (int q = 0; q < E; q++) pl[f][pidx(t + G * q)] = Cx<R>{x[q].c[0].x * invL, x[q].c[0].y * invL};
+        __syncthreads();
+        const Buf d0 = mkbuf((Vec<R, 2>*)b.S + tbase, 0), d1 = mkbuf((Vec<R, 2>*)b.AB + tbase, 0);
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int gi = tid + NT * u, h = gi & 1, row = (gi >> 1) % LL, arr = (gi >> 1) / LL;
+            const uint32_t off = (uint32_t)row * rs + (uint32_t)h * (uint32_t)sizeof(Cx<R>);
+            const Cx<R> v = pl[2 * arr + h][pidx(row)];
+            if (arr) bstore(v, d1, off, 0u);
+            else bstore(v, d0, off, 0u);
+        }
+    } else {
         const Buf dst = mkbuf((Vec<R, 2>*)(f < 2 ? b.S : b.AB) + tbase, 0);
 #pragma unroll
         for (int q = 0; q < E; q++)
