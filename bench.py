@@ -570,6 +570,50 @@ def cvd_checkpoint_leg(hip, steps, mod, L=1024):
     }
 
 
+def single_instance_leg(hip, steps=200):
+    """One instance per context, the GUI's drop-in shapes: Misfit 512^2 FP64
+    (the accuracy-study grid, plain steps) and CVD 1024^2 with the energy pair
+    traced every step (the checkpoint run's stepping) in FP32 and FP64.  Per
+    kernel: the mean of hipExtLaunchKernel start/stop events over `steps`
+    launches (the pair also spans 3-5 us of dispatch set-up, so the rate
+    understates the kernel's own; the profiler's durations are in profiles/),
+    the kernel's algorithmic bytes per launch and that rate against 8 TB/s;
+    per step: wall time of `steps` eager steps."""
+    cases = [("Misfit 512^2 FP64 (reference narrowing), plain steps", hip.MISFIT, hip.FP64, 512, False),
+             ("CVD 1024^2 FP32 + FP64 k=0/reductions, energy every step", hip.CVD, hip.FP32, 1024, True),
+             ("CVD 1024^2 FP64 (reference narrowing), energy every step", hip.CVD, hip.FP64, 1024, True)]
+    res = []
+    for label, model, prec, L, en in cases:
+        kw = dict(grainx=L // 32, grainy=L // 32) if model == hip.MISFIT else dict(u0=0.05, con0=0.25)
+        p = hip.default_params(model=model, L=L, M=L, precision=prec, **kw)
+        phi, con = hip.init_fields(p)
+        sim = hip.Sim(p)
+        sim.set_fields(phi, con)
+        stepf = sim.step_energy if en else sim.step
+        stepf(20)
+        sim.sync()
+        row = {"case": label}
+        for kid, name in ((hip.KID_ROW, "row"), (hip.KID_COL, "column")):
+            sim.time_kernel(kid, True)
+            stepf(steps)
+            sim.sync()
+            ms, n = sim.kernel_time()
+            sim.time_kernel(kid, False)
+            by = sim.kernel_bytes(kid)
+            row[name] = {"us_per_launch": round(ms * 1e3, 2), "launches": n, "bytes_per_launch": by,
+                         "achieved_GBps": round(by / (ms * 1e-3) / 1e9, 1),
+                         "frac_of_8TBps": round(by / (ms * 1e-3) / 8e12, 4)}
+        t0 = time.perf_counter()
+        stepf(steps)
+        sim.sync()
+        dt = (time.perf_counter() - t0) / steps
+        row["us_per_step"] = round(dt * 1e6, 2)
+        row["mpoint_steps_per_s"] = round(L * L / dt / 1e6, 1)
+        sim.close()
+        res.append(row)
+    return res
+
+
 COPY_PROBE_BYTES = 256 * 1024 * 1024
 
 
@@ -605,6 +649,8 @@ def main():
     ap.add_argument("--cvd-steps", type=int, default=20000, help="CVD 1024^2 checkpoint run (0: skip)")
     ap.add_argument("--cvd-mod", type=int, default=500)
     ap.add_argument("--pmc-drive", action="store_true")
+    ap.add_argument("--single-drive", action="store_true",
+                    help="only the single-instance leg (for kernel traces of the drop-in shapes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (there is no multi-GPU path); got --gpus {args.gpus}",
@@ -614,6 +660,9 @@ def main():
     hip.load()
     if args.pmc_drive:
         pmc_drive(hip, 3)
+        return
+    if args.single_drive:
+        print(json.dumps({"single_instance": single_instance_leg(hip)}))
         return
 
     # ---- batched FP32 sweep ------------------------------------------------
@@ -735,6 +784,7 @@ def main():
                        "ms_per_sweep_step": round(el64 * 1e3, 4),
                        "ms_per_step_by_grid": {str(g): round(t * 1e3, 4) for g, t in per_grid.items()}},
         "fp64_plain_512": {"value": round(fp64_512, 1), "unit": "Mpoint-steps/s"},
+        "single_instance": single_instance_leg(hip),
         "fp32_batched_over_fp64_plain": round(value / fp64, 2),
         "roofline": {
             "bound": "hbm",

@@ -189,10 +189,15 @@ __global__ void __launch_bounds__((FsCol<R, LL>::NT)) k_col_fs(DevBufs b, PhysAr
     __shared__ double red[NT / 64 + 2];
     const int tid = threadIdx.x, f = tid / G, t = tid % G;
     const int inst = blockIdx.y;
-    if constexpr (WANT_E) {
-        if (b.etrace && blockIdx.x == gridDim.x - 1) energy_reduce_block<NT>(b, p, inst, gridDim.x, red);
+    // WANT_E: the last workgroup of each instance also reduces the previous
+    // state's partial sums, after issuing its column's loads.
+    const bool reduce_here = WANT_E && b.etrace && blockIdx.x == gridDim.x - 1;
+    if (b.active && !b.active[inst]) {  // idle instance (uniform per workgroup)
+        if constexpr (WANT_E) {
+            if (reduce_here) energy_reduce_block<NT>(b, p, inst, gridDim.x, red);
+        }
+        return;
     }
-    if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
     const int K = b.K, Nh = b.N / 2;
     const size_t CS = (size_t)b.N * K;  // elements between consecutive l
     // Adjacent columns on one XCD (workgroup x takes column (x % 8)*(n/8) + x/8),
@@ -228,6 +233,11 @@ __global__ void __launch_bounds__((FsCol<R, LL>::NT)) k_col_fs(DevBufs b, PhysAr
     }
     float4 cf[C::MP];
     float2 cc[C::MP];
+    float klv[WANT_E ? C::MP : 1];  // kl of this thread's modes (energy terms)
+    if constexpr (WANT_E) {
+#pragma unroll
+        for (int r = 0; r < C::MP; r++) klv[r] = b.kl[tid + NT * r];
+    }
     {
         const Buf CA = mkbuf((const float4*)b.cA + bx, 0), CC = mkbuf((const float2*)b.cC + bx, 0);
 #pragma unroll
@@ -265,6 +275,9 @@ __global__ void __launch_bounds__((FsCol<R, LL>::NT)) k_col_fs(DevBufs b, PhysAr
 #pragma unroll
         for (int k2 = 0; k2 < NTW; k2++) twS[tid + k2 * NT] = tv[k2];
     }
+    if constexpr (WANT_E) {
+        if (reduce_here) energy_reduce_block<NT>(b, p, inst, gridDim.x, red);
+    }
     if (C::TWL || C::PAIR) __syncthreads();
     if constexpr (C::PAIR) {
 #pragma unroll
@@ -299,7 +312,7 @@ __global__ void __launch_bounds__((FsCol<R, LL>::NT)) k_col_fs(DevBufs b, PhysAr
             ph = sv.c[0];
             co = sv.c[1];
             if (i == 0 && kmi == 0 && kni == 0) ph = co = z0;
-            if (WANT_E) g += mode_energy(b, LL, i, kmi, kni, co);
+            if constexpr (WANT_E) g += mode_energy_kl(b, LL, i, klv[r], kmi, kni, co);
             a = Cx<R>{ph.x * (R)cc[r].x, ph.y * (R)cc[r].x};
             bb = Cx<R>{ph.x * (R)cc[r].y, ph.y * (R)cc[r].y};
         } else {

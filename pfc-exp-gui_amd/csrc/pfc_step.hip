@@ -513,7 +513,8 @@ template <typename R, int LL, int EM, int E, int G, int GF, bool COEF = false>
 __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs& p, int mode, bool want_e,
                                                bool use_nz, Vec<R, 4> (&x)[EM], const Cx<R> (&nz)[EM], int t, int j,
                                                int kmi, int kni, bool packed_tile, bool packed, char* xch,
-                                               double& g, const float4* cf = nullptr, const float2* cc = nullptr) {
+                                               double& g, const float4* cf = nullptr, const float2* cc = nullptr,
+                                               const float* klp = nullptr) {
     const bool act = G == GF || t < G;
     Vec<R, 4>* mx4 = (Vec<R, 4>*)xch;
     Cx<R>* nzx = (Cx<R>*)(mx4 + padded(LL));
@@ -546,7 +547,7 @@ __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs&
                     ph = Cx<R>{R(0), R(0)};
                     co = Cx<R>{R(0), R(0)};
                 }
-                if (want_e) g += mode_energy(b, LL, i, kmi, kni, co);
+                if (want_e) g += klp ? mode_energy_kl(b, LL, i, klp[q], kmi, kni, co) : mode_energy(b, LL, i, kmi, kni, co);
                 a = Cx<R>{ph.x * (R)cc[q].x, ph.y * (R)cc[q].x};
                 bb = Cx<R>{ph.x * (R)cc[q].y, ph.y * (R)cc[q].y};
             }
@@ -598,10 +599,16 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     const int tid = threadIdx.x;
     const int col = C::WAVE ? tid / G : tid % T, t = C::WAVE ? tid % G : tid / T;
     const int inst = blockIdx.y;
-    if constexpr (KIND != COL_LEAN) {
-        if (b.etrace && blockIdx.x == gridDim.x - 1) energy_reduce_block<NT>(b, p, inst, gridDim.x, red);
+    // Energy-traced steps: the last workgroup of each instance also reduces
+    // the previous state's partial sums, after issuing its own tile's loads
+    // (TWO) so that their latency covers the reduction's.
+    const bool reduce_here = KIND != COL_LEAN && b.etrace && blockIdx.x == gridDim.x - 1;
+    if (b.active && !b.active[inst]) {  // idle instance (uniform per workgroup)
+        if constexpr (KIND != COL_LEAN) {
+            if (reduce_here) energy_reduce_block<NT>(b, p, inst, gridDim.x, red);
+        }
+        return;
     }
-    if (b.active && !b.active[inst]) return;  // idle instance (uniform per workgroup)
     const int K = b.K;
     const size_t CS = (size_t)b.N * K;       // stride between consecutive l
     const int bx = (PFC_XCD_REMAP && gridDim.x % 8 == 0) ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8
@@ -674,6 +681,17 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
         for (int q = 0; q < EA; q++) xi[q] = bload<Vec<R, 2>>(IN, toff, (uint32_t)(G * q * CS) * es);
         __asm__ volatile("" ::: "memory");
     }
+    if constexpr (KIND != COL_LEAN) {
+        if (reduce_here) energy_reduce_block<NT>(b, p, inst, gridDim.x, red);
+    }
+    // Energy-traced plain steps: kl of this thread's modes, loaded here so
+    // that the per-mode energy terms do not wait for them after the update.
+    constexpr bool KLP = COEF && KIND == COL_LEAN_E;
+    float klv[KLP ? C::EB : 1];
+    if constexpr (KLP) {
+#pragma unroll
+        for (int q = 0; q < C::EB; q++) klv[q] = (C::GB == G || t < C::GB) ? b.kl[t + C::GB * q] : 0.0f;
+    }
     float4 cf[COEF ? EBc : 1];
     float2 cc[COEF ? EBc : 1];
     auto load_coef = [&]() {
@@ -744,7 +762,7 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     const bool want_e = KIND == COL_LEAN_E || (KIND == COL_GENERAL && (flags & COL_ENERGY) != 0);
     double g = 0.0;
     col_tile_modes<R, LL, E, C::EB, C::GB, G, COEF>(b, p, mode, want_e, use_nz, x, nz, t, j, kmi, kni, packed_tile,
-                                                    packed, smem, g, cf, cc);
+                                                    packed, smem, g, cf, cc, KLP ? klv : nullptr);
     if (step && b.tcount && blockIdx.x == 0 && tid == 0) b.tcount[inst] += 1;
     if (want_e) {
         const double gs = block_sum<NT>(g, red);

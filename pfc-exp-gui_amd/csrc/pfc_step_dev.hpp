@@ -274,14 +274,20 @@ __device__ __forceinline__ Cx<R> hjoin(Cx<R> x0, Cx<R> xh) {
 // removed (Re of the reference's i k con_hat transform).  A mode whose
 // conjugate is not stored (0 < km < M/2, or km in {0, M/2} with
 // 0 < kn < N/2) counts twice.
+// klv = b.kl[i] (loaded ahead by callers that prefetch it).
 template <typename R>
-__device__ __forceinline__ double mode_energy(const DevBufs& b, int LL, int i, int kmi, int kni, Cx<R> co) {
+__device__ __forceinline__ double mode_energy_kl(const DevBufs& b, int LL, int i, float klv, int kmi, int kni,
+                                                 Cx<R> co) {
     const int Mh = b.M / 2, Nh = b.N / 2;
     const bool mself = kmi == 0 || kmi == Mh;
     const double kmd = (kmi == Mh) ? 0.0 : (double)b.km[kmi];
     const double mult = (mself && (kni == 0 || kni == Nh)) ? 1.0 : 2.0;
-    const double kld = (i == LL / 2) ? 0.0 : (double)b.kl[i];
+    const double kld = (i == LL / 2) ? 0.0 : (double)klv;
     return mult * (kld * kld + 2.0 * kmd * kmd) * ((double)co.x * co.x + (double)co.y * co.y);
+}
+template <typename R>
+__device__ __forceinline__ double mode_energy(const DevBufs& b, int LL, int i, int kmi, int kni, Cx<R> co) {
+    return mode_energy_kl(b, LL, i, b.kl[i], kmi, kni, co);
 }
 
 // Per-mode work of the column pass on one (k_l = i, k_m = kmi, k_n = kni)
