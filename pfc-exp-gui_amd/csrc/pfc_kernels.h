@@ -79,12 +79,12 @@ enum { COL_ENERGY = 1 };
 // workgroup).  FP32 columns keep the four-field passes: a field-split FP32
 // column moves 8-byte pieces per lane (one cache line per lane and row), and
 // measured slower (one 1024^2 instance 20.4 -> 27.5 us per column pass).
-#ifndef PFC_FS_ROW32_MIN
-#define PFC_FS_ROW32_MIN 512  // shortest FP32 row half-length H on the field-split row pass
+#ifndef PFC_FS_ROW32
+#define PFC_FS_ROW32 512  // FP32 row half-lengths H on the field-split row pass (bit mask of powers of two)
 #endif
 constexpr bool pow2_len(int n) { return n > 0 && (n & (n - 1)) == 0; }
 constexpr bool fs_row(int prec, int H) {  // H = M/2
-    return prec == PREC_FP64 ? pow2_len(H) && H >= 16 && H <= 512 : pow2_len(H) && H >= PFC_FS_ROW32_MIN && H <= 512;
+    return prec == PREC_FP64 ? pow2_len(H) && H >= 16 && H <= 512 : pow2_len(H) && H <= 512 && (PFC_FS_ROW32 & H) != 0;
 }
 constexpr bool fs_col(int prec, int L) { return prec == PREC_FP64 && pow2_len(L) && L >= 32 && L <= 1024; }
 
