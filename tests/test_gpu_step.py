@@ -256,6 +256,38 @@ def test_batch_equals_single(hip):
         assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
 
 
+@pytest.mark.parametrize("prec,model,L,M", [(0, 0, 256, 256), (0, 1, 128, 512), (0, 1, 1024, 64),
+                                             (1, 1, 64, 1024)])
+def test_field_split_batch_and_trace(hip, prec, model, L, M):
+    """The field-split passes (FP64 power-of-two rows and columns, FP32
+    1024-point rows) run at every batch size: each member of a batch of three
+    steps bit-identically to the same fields alone, and energy-traced steps
+    (the column pass with the energy partials) advance the state exactly as
+    plain steps do."""
+    kw = dict(grainx=L // 32, grainy=M // 32) if model == 0 else dict(u0=0.05, con0=0.25)
+    p = hip.default_params(model=model, L=L, M=M, precision=prec, **kw)
+    fields = []
+    for s in range(3):
+        p.seed = 700 + s
+        fields.append(hip.init_fields(p))
+    b, t = hip.Sim(p, batch=3), hip.Sim(p, batch=3)
+    for i, (phi, con) in enumerate(fields):
+        b.set_fields(phi, con, inst=i)
+        t.set_fields(phi, con, inst=i)
+    b.step(12)
+    t.step_energy(12)
+    for i, (phi, con) in enumerate(fields):
+        s1 = hip.Sim(p)
+        s1.set_fields(phi, con)
+        s1.step(12)
+        x, y, z = s1.get_fields(), b.get_fields(i), t.get_fields(i)
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), (prec, model, L, M, i)
+        assert np.array_equal(x[0], z[0]) and np.array_equal(x[1], z[1]), (prec, model, L, M, i)
+        s1.close()
+    b.close()
+    t.close()
+
+
 @pytest.mark.parametrize("L", [512, 384])
 def test_sweep_class_full_size(hip, L):
     """A sweep grid class at its benchmark size (16 instances of L^2, FP32,
