@@ -37,9 +37,12 @@ namespace pfc {
 // wavefront per field) from 256 points up, else 4.
 constexpr int fs_e(int n) { return n >= 256 ? n / 64 : 4; }
 
+#ifndef PFC_FS_ROW256_E
+#define PFC_FS_ROW256_E 4  // elements per thread of 256-point field-split rows
+#endif
 template <typename R, int H>
 struct FsRow {
-    static constexpr int E = fs_e(H), G = H / E, NT = 4 * G, PB = padded(H);
+    static constexpr int E = H == 256 ? PFC_FS_ROW256_E : fs_e(H), G = H / E, NT = 4 * G, PB = padded(H);
     static constexpr int EP = E / 4;  // row elements (pairs of real points) per thread, pointwise phase
     static_assert(G <= 64 && 64 % G == 0, "one field's transform inside one wavefront");
 };
