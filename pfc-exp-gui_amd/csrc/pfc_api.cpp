@@ -51,7 +51,7 @@ struct pfc_ctx {
     std::vector<double> means_h;
     void *twH = nullptr, *twM = nullptr, *twL = nullptr, *twN = nullptr;
     float *kl_d = nullptr, *km_d = nullptr, *kn_d = nullptr, *c2_d = nullptr;
-    void *coefA = nullptr, *coefC = nullptr;  // FP32 per-mode coefficients [L][N][K]
+    void *coefA = nullptr, *coefC = nullptr;  // per-mode coefficients [L][N][K] + packed tail [2][L]
     int* active_d = nullptr;  // [B] step mask
     double *erow = nullptr, *ecol[2] = {nullptr, nullptr}, *eout = nullptr;
     int nrow = 0, ncol = 0;
@@ -386,7 +386,8 @@ extern "C" int pfc_create(const pfc_params* p, int batch, pfc_ctx** out) try {
     // forming k^2 per mode: faster at every batch size measured, one
     // instance included (FP32 512^2: 24.8 -> 19.4 us per step).
     if (col_coef(prec == PREC_FP64, c->L)) {
-        const size_t nm = (size_t)c->L * c->N * c->K;
+        // [L][N][K] entries, then the packed columns' k_m = M/2 modes (2 x L)
+        const size_t nm = (size_t)c->L * c->N * c->K + 2 * (size_t)c->L;
         e = hipMalloc(&c->coefA, nm * 16);
         if (e == hipSuccess) e = hipMalloc(&c->coefC, nm * 8);
         if (e == hipSuccess) {

@@ -224,9 +224,9 @@ __global__ void __launch_bounds__((FsCol<R, LL>::NT)) k_col_fs(DevBufs b, PhysAr
 
     // Loads: the twiddle table (staged in LDS for every pass), the
     // coefficients of this thread's modes (tables in the column layout, used
-    // after the forward transform; the packed column forms its own and leaves
-    // its entries unused), and field f of the column (psi, con from S; inn,
-    // inc from IN).  NT divides L.
+    // after the forward transform; for the packed column its k_m = 0 modes),
+    // and field f of the column (psi, con from S; inn, inc from IN).  NT
+    // divides L.
     constexpr int NTW = C::TWL ? LL / NT : 1;
     static_assert(LL % NT == 0, "whole twiddle rows per thread");
     Cx<R> tv[NTW];
@@ -319,18 +319,13 @@ __global__ void __launch_bounds__((FsCol<R, LL>::NT)) k_col_fs(DevBufs b, PhysAr
             a = Cx<R>{ph.x * (R)cc[r].x, ph.y * (R)cc[r].x};
             bb = Cx<R>{ph.x * (R)cc[r].y, ph.y * (R)cc[r].y};
         } else {
+            // k_m = 0 from the column's own entries (cf, cc), k_m = M/2 from the table's tail
             const int im = pidx((LL - i) % LL);
-            Cx<R> ph0, phh, co0, coh, on0, onh, oc0, och, a0, ah, b0, bh;
-            hsplit(ph, pl[0][im], ph0, phh);
-            hsplit(co, pl[1][im], co0, coh);
-            hsplit(on, pl[2][im], on0, onh);
-            hsplit(oc, pl[3][im], oc0, och);
-            col_mode(b, p, COL_STEP, WANT_E, false, LL, i, 0, j, ph0, co0, on0, oc0, z0, a0, b0, g);
-            col_mode(b, p, COL_STEP, WANT_E, false, LL, i, b.M / 2, j, phh, coh, onh, och, z0, ah, bh, g);
-            ph = hjoin(ph0, phh);
-            co = hjoin(co0, coh);
-            a = hjoin(a0, ah);
-            bb = hjoin(b0, bh);
+            const size_t te = coef_packed_tail(b, j) + i;
+            const float4 ch = ((const float4*)b.cA)[te];
+            const float2 cch = ((const float2*)b.cC)[te];
+            packed_modes_coef(b, p, WANT_E, LL, i, j, ph, co, on, oc, pl[0][im], pl[1][im], pl[2][im], pl[3][im],
+                              cf[r], cc[r], ch, cch, a, bb, g);
         }
         nv[r][0] = ph;
         nv[r][1] = co;

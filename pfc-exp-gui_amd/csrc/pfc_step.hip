@@ -505,8 +505,9 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __
 // each mode's mirror -k: it is exchanged through `xch` (PB Vecs, then PB noise
 // values), which no other thread may be using (the call starts and ends the
 // exchange with barriers when the tile is packed).
-// COEF (FP32 plain step): cf/cc hold the precomputed coefficients of the
-// column's modes (plain columns only; the packed column evaluates its own).
+// COEF (plain step with coefficient tables): cf/cc hold the tabulated
+// coefficients of the column's modes (the packed column's: its k_m = 0 modes;
+// its k_m = M/2 modes are read from the table's tail).
 // x and nz hold the column's full-spectrum elements t + G*q (q < E) in
 // threads t < G (GF: the column's thread count; the others hold none).
 template <typename R, int LL, int EM, int E, int G, int GF, bool COEF = false>
@@ -553,6 +554,16 @@ __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs&
             }
         } else if (!packed) {
             col_mode(b, p, mode, want_e, use_nz, LL, i, kmi, kni, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
+        } else if (COEF) {
+            if constexpr (COEF) {
+                // k_m = 0 from the column's own entries (cf, cc), k_m = M/2 from the table's tail
+                const Vec<R, 4> xm = mx4[pidx((LL - i) % LL)];
+                const size_t te = coef_packed_tail(b, j) + i;
+                const float4 ch = ((const float4*)b.cA)[te];
+                const float2 cch = ((const float2*)b.cC)[te];
+                packed_modes_coef(b, p, want_e, LL, i, j, ph, co, x[q].c[2], x[q].c[3], xm.c[0], xm.c[1], xm.c[2],
+                                  xm.c[3], cf[q], cc[q], ch, cch, a, bb, g);
+            }
         } else {
             const int im = pidx((LL - i) % LL);
             const Vec<R, 4> xm = mx4[im];
@@ -814,23 +825,39 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     }
 }
 
-// Per-mode coefficients of the plain FP32 step in the column layout: entry
+// Per-mode coefficients of the plain step in the column layout: entry
 // (l, n slot j, packed column kc) holds the mode k_col gives that column
 // (kc = 0 of slots other than 0 and N/2 carries k_m = 0 or M/2 with k_n
-// remapped); the packed columns' entries are unused.  Same float expressions
-// as update_mode, so the coefficient path is bit-identical.
+// remapped; the packed columns of slots 0 and N/2 carry k_m = 0 and M/2 at
+// k_n = j: their entries hold the k_m = 0 mode), then a tail of 2 x L
+// entries with the packed columns' k_m = M/2 modes (slot 0, then slot N/2;
+// coef_packed_tail).  Same float expressions as update_mode, so the
+// coefficient path is bit-identical.
 __global__ void __launch_bounds__(256) k_coef(DevBufs b, PhysArgs p, int f64) {
     const long long n = (long long)b.L * b.N * b.K;
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
+    if (e >= n + 2LL * b.L) return;
     const int K = b.K, Nh = b.N / 2, Mh = b.M / 2;
-    const int kc = (int)(e % K);
-    const int j = (int)((e / K) % b.N);
-    const int i = (int)(e / ((long long)K * b.N));
-    int kmi = kc, kni = j;
-    if (kc == 0) {
-        kmi = j < Nh ? 0 : Mh;
-        kni = j < Nh ? j : b.N - j;
+    int i, kmi, kni;
+    if (e < n) {
+        const int kc = (int)(e % K);
+        const int j = (int)((e / K) % b.N);
+        i = (int)(e / ((long long)K * b.N));
+        kmi = kc;
+        kni = j;
+        if (kc == 0) {
+            if (j == 0 || j == Nh) {
+                kmi = 0;
+            } else {
+                kmi = j < Nh ? 0 : Mh;
+                kni = j < Nh ? j : b.N - j;
+            }
+        }
+    } else {
+        const int s = (int)((e - n) / b.L);
+        i = (int)((e - n) % b.L);
+        kmi = Mh;
+        kni = s == 0 ? 0 : Nh;
     }
     const float ks = ksq(b.kl[i], b.km[kmi], b.kn[kni]);
     const int il = i <= b.L / 2 ? i : b.L - i;
@@ -842,7 +869,7 @@ __global__ void __launch_bounds__(256) k_coef(DevBufs b, PhysArgs p, int f64) {
 
 hipError_t launch_coef(int prec, const DevBufs& b, const PhysArgs& p, hipStream_t s) {
     if (!b.cA || !b.cC) return hipErrorInvalidValue;
-    const long long n = (long long)b.L * b.N * b.K;
+    const long long n = (long long)b.L * b.N * b.K + 2LL * b.L;
     hipLaunchKernelGGL(k_coef, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, p, prec == PREC_FP64 ? 1 : 0);
     return hipGetLastError();
 }

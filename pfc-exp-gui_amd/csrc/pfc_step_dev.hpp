@@ -323,6 +323,56 @@ __device__ __forceinline__ void col_mode(const DevBufs& b, const PhysArgs& p, in
     bb = Cx<R>{ph.x * cb, ph.y * cb};
 }
 
+// Index of the first coefficient-table entry of the k_m = M/2 modes of the
+// packed column of n slot j (0 or N/2): after the [L][N][K] entries, slot 0
+// then slot N/2, L entries each (k_coef).  The packed column's own entries
+// hold its k_m = 0 modes.
+__device__ __forceinline__ size_t coef_packed_tail(const DevBufs& b, int j) {
+    return (size_t)b.L * b.N * b.K + (j == 0 ? 0 : (size_t)b.L);
+}
+
+// Both modes of a packed-column element (k_m = 0 and M/2 at k_l = i, k_n =
+// j) from the tabulated coefficients (c0, cc0: k_m = 0; ch, cch: k_m = M/2):
+// the element and its mirror -i split into the two Hermitian sequences,
+// each updated as col_mode does (update, DC removal, gradient energy, C2
+// products) and joined back.  Bit-identical to two col_mode calls.
+template <typename R>
+__device__ __forceinline__ void packed_modes_coef(const DevBufs& b, const PhysArgs& p, bool want_e, int LL, int i,
+                                                  int j, Cx<R>& ph, Cx<R>& co, Cx<R> on, Cx<R> oc, Cx<R> phm,
+                                                  Cx<R> com, Cx<R> onm, Cx<R> ocm, float4 c0, float2 cc0, float4 ch,
+                                                  float2 cch, Cx<R>& a, Cx<R>& bb, double& g) {
+    Cx<R> ph0, phh, co0, coh, on0, onh, oc0, och;
+    hsplit(ph, phm, ph0, phh);
+    hsplit(co, com, co0, coh);
+    hsplit(on, onm, on0, onh);
+    hsplit(oc, ocm, oc0, och);
+    const Cx<R> z0{R(0), R(0)};
+    Vec<R, 2> s0, o0, sh, oh;
+    s0.c[0] = ph0; s0.c[1] = co0; o0.c[0] = on0; o0.c[1] = oc0;
+    sh.c[0] = phh; sh.c[1] = coh; oh.c[0] = onh; oh.c[1] = och;
+    if constexpr (sizeof(R) == 4) {
+        update_mode_coef(c0, s0, o0, z0);
+        update_mode_coef(ch, sh, oh, z0);
+    } else {
+        update_mode_coef64(p, c0, s0, o0, z0);
+        update_mode_coef64(p, ch, sh, oh, z0);
+    }
+    ph0 = s0.c[0]; co0 = s0.c[1];
+    phh = sh.c[0]; coh = sh.c[1];
+    if (i == 0 && j == 0) {
+        ph0 = z0;
+        co0 = z0;
+    }
+    if (want_e) g += mode_energy(b, LL, i, 0, j, co0);
+    const Cx<R> a0{ph0.x * (R)cc0.x, ph0.y * (R)cc0.x}, b0{ph0.x * (R)cc0.y, ph0.y * (R)cc0.y};
+    if (want_e) g += mode_energy(b, LL, i, b.M / 2, j, coh);
+    const Cx<R> ah{phh.x * (R)cch.x, phh.y * (R)cch.x}, bh{phh.x * (R)cch.y, phh.y * (R)cch.y};
+    ph = hjoin(ph0, phh);
+    co = hjoin(co0, coh);
+    a = hjoin(a0, ah);
+    bb = hjoin(b0, bh);
+}
+
 // The energy pair of one instance from its partial sums (ENERGYC2,
 // misfit.cpp:1043-1092): the pointwise terms plus alpha/P times the k-space
 // gradient sum, both over the grid's points; the density is the FP64 k = 0
