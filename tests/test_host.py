@@ -130,3 +130,28 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_state_digest_fixture_covers_every_case():
+    """tests/golden/state_sha.json holds a digest set for every case the GPU
+    bit-pin test runs (tests/test_gpu_state_sha.py), with one plain and one
+    energy-traced digest per instance and one (energy, density) pair per
+    instance and traced step."""
+    import importlib.util
+    import json
+
+    spec = importlib.util.spec_from_file_location("make_state_sha_cpu",
+                                                  os.path.join(ROOT, "tests", "golden", "make_state_sha.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "state_sha.json")))
+    assert gold["plain_steps"] == gen.PLAIN and gold["traced_steps"] == gen.TRACED
+    assert set(gold["cases"]) == {gen.case_key(c) for c in gen.CASES}
+    for c in gen.CASES:
+        g = gold["cases"][gen.case_key(c)]
+        B = c[5]
+        assert len(g["plain"]) == B and len(g["traced"]) == B
+        assert all(re.fullmatch(r"[0-9a-f]{64}", h) for h in g["plain"] + g["traced"])
+        assert len(g["energy_trace"]) == 2 * B * gen.TRACED
+        vals = [float.fromhex(x) for x in g["energy_trace"]]
+        assert all(np.isfinite(vals))
