@@ -299,11 +299,16 @@ def accuracy_case(hip, model, L, M, marks, keep_ref=False):
 
 ACC_CASES = [(0, 512, 512), (0, 128, 128), (1, 128, 384), (1, 256, 256), (1, 128, 128)]
 ORACLE_CASES = [(0, 128, 128), (1, 128, 128)]
+# --oracle-curves 2: also the GUI's default grid (Misfit 256^2, 16 x 16
+# grains, added to the accuracy cases for its FP64 envelope) and CVD 256^2
+# (about 2-4 minutes of oracle time each, in parallel)
+ORACLE_CASES_256 = [(0, 256, 256), (1, 256, 256)]
+
 EARLY_CVD = (1, 128, 384)
 EARLY_MARKS = [10, 20, 50, 100, 200, 500, 1000]
 
 
-def accuracy_study(hip, steps, every, with_oracle=True):
+def accuracy_study(hip, steps, every, with_oracle=1):
     """FP32 against the FP64 drop-in (itself pinned to the oracle) from the
     same initial fields, beside the two FP64 envelopes, every `every` steps;
     the FP64 drop-in against the oracle's own trajectory on the 128^2 grids;
@@ -320,18 +325,22 @@ def accuracy_study(hip, steps, every, with_oracle=True):
                            "energy and atoms within spread x the ensemble's largest difference so far; mean phi "
                            "exactly equal"},
            "cases": {}}
+    if with_oracle >= 2:
+        res["workload"] += "; also Misfit 256^2 (16 x 16 grains), and the FP64 drop-in vs the oracle on Misfit and CVD 256^2"
     tmp = tempfile.mkdtemp(prefix="pfc_acc_")
     try:
         procs = {}
+        oracle_cases = ORACLE_CASES + (ORACLE_CASES_256 if with_oracle >= 2 else [])
+        acc_cases = ACC_CASES + ([(0, 256, 256)] if with_oracle >= 2 else [])
         if with_oracle:
-            for c in ORACLE_CASES:
+            for c in oracle_cases:
                 procs[c] = (marks,) + start_oracle_curve(*c, marks, tmp)
             procs["early"] = (EARLY_MARKS,) + start_oracle_curve(*EARLY_CVD, EARLY_MARKS, tmp)
         refs = {}
-        for c in ACC_CASES:
+        for c in acc_cases:
             model, L, M = c
             name = f"{'misfit' if model == 0 else 'cvd'}_{L}x{M}"
-            trace, refs[c] = accuracy_case(hip, model, L, M, marks, keep_ref=c in ORACLE_CASES)
+            trace, refs[c] = accuracy_case(hip, model, L, M, marks, keep_ref=with_oracle and c in oracle_cases)
             res["cases"][name] = {"trace": trace, "bar_ok": all(r["bar_ok"] for r in trace)}
             print(f"accuracy {name}: final fp32 {trace[-1]['linf_fp32']:.2e} envelope "
                   f"{trace[-1]['envelope']['linf']:.2e} bar_ok={res['cases'][name]['bar_ok']}", file=sys.stderr,
@@ -341,7 +350,7 @@ def accuracy_study(hip, steps, every, with_oracle=True):
         res["bar_ok"] = all(c["bar_ok"] for c in res["cases"].values()) and res["cvd_128x384_early"]["bar_ok"]
         if with_oracle:
             res["fp64_vs_oracle"] = {}
-            for c in ORACLE_CASES:
+            for c in oracle_cases:
                 mk, out, proc = procs[c]
                 name = f"{'misfit' if c[0] == 0 else 'cvd'}_{c[1]}x{c[2]}"
                 res["fp64_vs_oracle"][name] = _oracle_rows(proc, out, mk, refs[c])
@@ -644,7 +653,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=10000)
     ap.add_argument("--oracle-curves", type=int, default=1,
-                    help="1: the oracle's own 128^2 trajectories beside the FP64 drop-in (CPU processes)")
+                    help="1: the oracle's own 128^2 trajectories beside the FP64 drop-in (CPU processes); "
+                         "2: also Misfit and CVD 256^2 (the GUI's default grid; minutes of oracle time)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--cvd-steps", type=int, default=20000, help="CVD 1024^2 checkpoint run (0: skip)")
     ap.add_argument("--cvd-mod", type=int, default=500)
@@ -835,7 +845,7 @@ def main():
         s.close()
     if args.accuracy_steps > 0:
         every = max(1, args.accuracy_steps // 10)
-        out["accuracy"] = accuracy_study(hip, args.accuracy_steps, every, with_oracle=bool(args.oracle_curves))
+        out["accuracy"] = accuracy_study(hip, args.accuracy_steps, every, with_oracle=args.oracle_curves)
         print(f"accuracy done: bar_ok={out['accuracy']['bar_ok']}", file=sys.stderr, flush=True)
     if args.serve_seconds > 0:
         out["serving"] = serve_leg(hip, args.serve_seconds)
