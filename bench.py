@@ -200,6 +200,7 @@ for mk in marks:
     t = mk
     phi, con = s.fields()
     snaps[f'phi_{t}'] = phi; snaps[f'con_{t}'] = con
+    print(f'oracle curve: model {kind} {L}x{M} step {t}', file=sys.stderr, flush=True)
 np.savez(out, **snaps)
 """
 
@@ -303,6 +304,9 @@ ORACLE_CASES = [(0, 128, 128), (1, 128, 128)]
 # grains, added to the accuracy cases for its FP64 envelope) and CVD 256^2
 # (about 2-4 minutes of oracle time each, in parallel)
 ORACLE_CASES_256 = [(0, 256, 256), (1, 256, 256)]
+# --oracle-curves 3: also Misfit 512^2, the FP64 drop-in's own benchmark
+# grid (the oracle needs about 8 minutes of one host core for 10^4 steps)
+ORACLE_CASES_512 = [(0, 512, 512)]
 
 EARLY_CVD = (1, 128, 384)
 EARLY_MARKS = [10, 20, 50, 100, 200, 500, 1000]
@@ -327,10 +331,13 @@ def accuracy_study(hip, steps, every, with_oracle=1):
            "cases": {}}
     if with_oracle >= 2:
         res["workload"] += "; also Misfit 256^2 (16 x 16 grains), and the FP64 drop-in vs the oracle on Misfit and CVD 256^2"
+    if with_oracle >= 3:
+        res["workload"] += " and Misfit 512^2"
     tmp = tempfile.mkdtemp(prefix="pfc_acc_")
     try:
         procs = {}
-        oracle_cases = ORACLE_CASES + (ORACLE_CASES_256 if with_oracle >= 2 else [])
+        oracle_cases = (ORACLE_CASES + (ORACLE_CASES_256 if with_oracle >= 2 else []) +
+                        (ORACLE_CASES_512 if with_oracle >= 3 else []))
         acc_cases = ACC_CASES + ([(0, 256, 256)] if with_oracle >= 2 else [])
         if with_oracle:
             for c in oracle_cases:
@@ -654,7 +661,8 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=10000)
     ap.add_argument("--oracle-curves", type=int, default=1,
                     help="1: the oracle's own 128^2 trajectories beside the FP64 drop-in (CPU processes); "
-                         "2: also Misfit and CVD 256^2 (the GUI's default grid; minutes of oracle time)")
+                         "2: also Misfit and CVD 256^2 (the GUI's default grid; minutes of oracle time); "
+                         "3: also Misfit 512^2 (about 8 minutes of oracle time)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--cvd-steps", type=int, default=20000, help="CVD 1024^2 checkpoint run (0: skip)")
     ap.add_argument("--cvd-mod", type=int, default=500)
