@@ -12,9 +12,10 @@ Workload (the headline metric "Mpoint-steps/s batched FP32 vs plain FP64
     steps in FP32 and in the FP64 drop-in from the same initial fields, beside
     an FP64 ensemble (no narrowing; fields one FP32 round-off away, three
     seeds) whose spread is the envelope of the FP32 bar; field L-inf, energy
-    and Phimax atom count every 1000 steps (CVD 128x384 also from step 10);
-    the FP64 drop-in against the oracle's own trajectory on the two 128^2
-    cases.  -> accuracy
+    and Phimax atom count every 1000 steps (CVD 128x384 also from step 10).
+    -> accuracy  (The FP64 drop-in against the oracle's own trajectory over
+    the same horizon is tests/oracle_curves.py: the oracle is the checker of
+    the test suite, and bench.py runs it only as the CPU baseline.)
   * request loop: a 20 s stream of mixed Misfit/CVD jobs (grids 128^2,
     256^2, 512^2, 1024^2, 128x128x64; 500..20000 steps) offering 0.8 of the
     server's measured capacity, Poisson arrivals, through the
@@ -185,45 +186,6 @@ def _acc_sims(hip, model, L, M):
     return sims
 
 
-ORACLE_CURVE = """
-import importlib.util, sys, numpy as np
-spec = importlib.util.spec_from_file_location('o', sys.argv[1])
-o = importlib.util.module_from_spec(spec); spec.loader.exec_module(o)
-kind, L, M, out = int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]), sys.argv[5]
-marks = [int(x) for x in sys.argv[6].split(',')]
-kw = dict(grainx=max(1, L // 16), grainy=max(1, M // 16)) if kind == 0 else dict(u0=0.05, con0=0.25)
-if kind == 0 and L >= 256: kw = dict(grainx=16, grainy=16)
-s = o.Sim(o.model(kind, L, M, 1, narrow=1, **kw))
-snaps, t = {}, 0
-for mk in marks:
-    s.run(mk - t)
-    t = mk
-    phi, con = s.fields()
-    snaps[f'phi_{t}'] = phi; snaps[f'con_{t}'] = con
-    print(f'oracle curve: model {kind} {L}x{M} step {t}', file=sys.stderr, flush=True)
-np.savez(out, **snaps)
-"""
-
-
-def start_oracle_curve(model, L, M, marks, tmp):
-    """The oracle (FP64, reference narrowing) from the reference initial
-    fields in a CPU process of its own, fields saved at each step in marks."""
-    out = os.path.join(tmp, f"oracle_{model}_{L}x{M}_{marks[-1]}.npz")
-    cmd = [sys.executable, "-c", ORACLE_CURVE, os.path.join(ROOT, "oracle", "oracle.py"), str(model), str(L), str(M),
-           out, ",".join(str(m) for m in marks)]
-    return out, subprocess.Popen(cmd, stdout=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS="1"))
-
-
-def _oracle_rows(proc, out, marks, gpu_ref):
-    rc = proc.wait()
-    if rc != 0 or not os.path.exists(out):
-        return {"error": f"oracle process exit {rc}"}
-    snaps = np.load(out)
-    return [{"step": t, "linf": float(max(np.abs(gpu_ref[t][0] - snaps[f"phi_{t}"]).max(),
-                                          np.abs(gpu_ref[t][1] - snaps[f"con_{t}"]).max())),
-             "mean_phi_diff": float(gpu_ref[t][0].mean() - snaps[f"phi_{t}"].mean())} for t in marks]
-
-
 # The FP32 bar.  The FP64 drop-in already rounds every spectral coefficient
 # to float once per step (the reference's narrowing, misfit.cpp:1002-1010);
 # the FP32 path rounds each value about 2 log2(P) times per step in its
@@ -298,29 +260,20 @@ def accuracy_case(hip, model, L, M, marks, keep_ref=False):
     return trace, refs
 
 
-ACC_CASES = [(0, 512, 512), (0, 128, 128), (1, 128, 384), (1, 256, 256), (1, 128, 128)]
-ORACLE_CASES = [(0, 128, 128), (1, 128, 128)]
-# --oracle-curves 2: also the GUI's default grid (Misfit 256^2, 16 x 16
-# grains, added to the accuracy cases for its FP64 envelope) and CVD 256^2
-# (about 2-4 minutes of oracle time each, in parallel)
-ORACLE_CASES_256 = [(0, 256, 256), (1, 256, 256)]
-# --oracle-curves 3: also Misfit 512^2, the FP64 drop-in's own benchmark
-# grid (the oracle needs about 8 minutes of one host core for 10^4 steps)
-ORACLE_CASES_512 = [(0, 512, 512)]
-
+ACC_CASES = [(0, 512, 512), (0, 256, 256), (0, 128, 128), (1, 128, 384), (1, 256, 256), (1, 128, 128)]
 EARLY_CVD = (1, 128, 384)
 EARLY_MARKS = [10, 20, 50, 100, 200, 500, 1000]
 
 
-def accuracy_study(hip, steps, every, with_oracle=1):
-    """FP32 against the FP64 drop-in (itself pinned to the oracle) from the
-    same initial fields, beside the two FP64 envelopes, every `every` steps;
-    the FP64 drop-in against the oracle's own trajectory on the 128^2 grids;
-    and the first 1000 steps of CVD 128x384 at finer checkpoints."""
+def accuracy_study(hip, steps, every):
+    """FP32 against the FP64 drop-in (itself pinned to the oracle by the
+    test suite and tests/oracle_curves.py) from the same initial fields,
+    beside the FP64 envelope, every `every` steps; and the first 1000 steps
+    of CVD 128x384 at finer checkpoints."""
     marks = list(range(every, steps + 1, every))
     res = {"workload": "FP32 path and FP64 envelopes vs the FP64 drop-in (reference narrowing), every %d steps "
-                       "to %d, from the reference initial fields; Misfit 512^2 and 128^2, CVD 128x384, 256^2, "
-                       "128^2; FP64 drop-in vs the oracle on Misfit and CVD 128^2; CVD 128x384 early" % (every, steps),
+                       "to %d, from the reference initial fields; Misfit 512^2, 256^2 and 128^2, CVD 128x384, "
+                       "256^2, 128^2; CVD 128x384 early" % (every, steps),
            "bar": {"K": ACC_K, "floor": ACC_FLOOR, "saturated": ACC_SATURATED, "spread": ACC_SPREAD,
                    "floor_saturated": ACC_FLOOR_SAT,
                    "rule": "per metric: |fp32 - ref| <= max(K * envelope, floor), envelope = the largest "
@@ -329,42 +282,17 @@ def accuracy_study(hip, steps, every, with_oracle=1):
                            "energy and atoms within spread x the ensemble's largest difference so far; mean phi "
                            "exactly equal"},
            "cases": {}}
-    if with_oracle >= 2:
-        res["workload"] += "; also Misfit 256^2 (16 x 16 grains), and the FP64 drop-in vs the oracle on Misfit and CVD 256^2"
-    if with_oracle >= 3:
-        res["workload"] += " and Misfit 512^2"
-    tmp = tempfile.mkdtemp(prefix="pfc_acc_")
-    try:
-        procs = {}
-        oracle_cases = (ORACLE_CASES + (ORACLE_CASES_256 if with_oracle >= 2 else []) +
-                        (ORACLE_CASES_512 if with_oracle >= 3 else []))
-        acc_cases = ACC_CASES + ([(0, 256, 256)] if with_oracle >= 2 else [])
-        if with_oracle:
-            for c in oracle_cases:
-                procs[c] = (marks,) + start_oracle_curve(*c, marks, tmp)
-            procs["early"] = (EARLY_MARKS,) + start_oracle_curve(*EARLY_CVD, EARLY_MARKS, tmp)
-        refs = {}
-        for c in acc_cases:
-            model, L, M = c
-            name = f"{'misfit' if model == 0 else 'cvd'}_{L}x{M}"
-            trace, refs[c] = accuracy_case(hip, model, L, M, marks, keep_ref=with_oracle and c in oracle_cases)
-            res["cases"][name] = {"trace": trace, "bar_ok": all(r["bar_ok"] for r in trace)}
-            print(f"accuracy {name}: final fp32 {trace[-1]['linf_fp32']:.2e} envelope "
-                  f"{trace[-1]['envelope']['linf']:.2e} bar_ok={res['cases'][name]['bar_ok']}", file=sys.stderr,
-                  flush=True)
-        early, early_ref = accuracy_case(hip, *EARLY_CVD, EARLY_MARKS, keep_ref=True)
-        res["cvd_128x384_early"] = {"trace": early, "bar_ok": all(r["bar_ok"] for r in early)}
-        res["bar_ok"] = all(c["bar_ok"] for c in res["cases"].values()) and res["cvd_128x384_early"]["bar_ok"]
-        if with_oracle:
-            res["fp64_vs_oracle"] = {}
-            for c in oracle_cases:
-                mk, out, proc = procs[c]
-                name = f"{'misfit' if c[0] == 0 else 'cvd'}_{c[1]}x{c[2]}"
-                res["fp64_vs_oracle"][name] = _oracle_rows(proc, out, mk, refs[c])
-            mk, out, proc = procs["early"]
-            res["cvd_128x384_early"]["fp64_vs_oracle"] = _oracle_rows(proc, out, mk, early_ref)
-    finally:
-        shutil.rmtree(tmp, ignore_errors=True)
+    for c in ACC_CASES:
+        model, L, M = c
+        name = f"{'misfit' if model == 0 else 'cvd'}_{L}x{M}"
+        trace, _ = accuracy_case(hip, model, L, M, marks)
+        res["cases"][name] = {"trace": trace, "bar_ok": all(r["bar_ok"] for r in trace)}
+        print(f"accuracy {name}: final fp32 {trace[-1]['linf_fp32']:.2e} envelope "
+              f"{trace[-1]['envelope']['linf']:.2e} bar_ok={res['cases'][name]['bar_ok']}", file=sys.stderr,
+              flush=True)
+    early, _ = accuracy_case(hip, *EARLY_CVD, EARLY_MARKS)
+    res["cvd_128x384_early"] = {"trace": early, "bar_ok": all(r["bar_ok"] for r in early)}
+    res["bar_ok"] = all(c["bar_ok"] for c in res["cases"].values()) and res["cvd_128x384_early"]["bar_ok"]
     return res
 
 
@@ -659,10 +587,6 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=10000)
-    ap.add_argument("--oracle-curves", type=int, default=1,
-                    help="1: the oracle's own 128^2 trajectories beside the FP64 drop-in (CPU processes); "
-                         "2: also Misfit and CVD 256^2 (the GUI's default grid; minutes of oracle time); "
-                         "3: also Misfit 512^2 (about 8 minutes of oracle time)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--cvd-steps", type=int, default=20000, help="CVD 1024^2 checkpoint run (0: skip)")
     ap.add_argument("--cvd-mod", type=int, default=500)
@@ -853,7 +777,7 @@ def main():
         s.close()
     if args.accuracy_steps > 0:
         every = max(1, args.accuracy_steps // 10)
-        out["accuracy"] = accuracy_study(hip, args.accuracy_steps, every, with_oracle=args.oracle_curves)
+        out["accuracy"] = accuracy_study(hip, args.accuracy_steps, every)
         print(f"accuracy done: bar_ok={out['accuracy']['bar_ok']}", file=sys.stderr, flush=True)
     if args.serve_seconds > 0:
         out["serving"] = serve_leg(hip, args.serve_seconds)
