@@ -8,7 +8,7 @@ Workload (the headline metric "Mpoint-steps/s batched FP32 vs plain FP64
     "step" advances all 64 instances by one time step.  -> `value`
   * plain FP64 path: the sweep's 64 instances one at a time (one-instance
     FP64 contexts with the reference narrowing, eager launches).  -> fp64_plain
-  * accuracy: Misfit 512^2 and 128^2, CVD 128x384, 256^2 and 128^2, 10^4
+  * accuracy: Misfit 512^2, 256^2 and 128^2, CVD 128x384, 256^2 and 128^2, 10^4
     steps in FP32 and in the FP64 drop-in from the same initial fields, beside
     an FP64 ensemble (no narrowing; fields one FP32 round-off away, three
     seeds) whose spread is the envelope of the FP32 bar; field L-inf, energy
