@@ -7,7 +7,8 @@ process per grid, in parallel) from the same fields; every `every` steps the
 field L-infinity and L2 (RMS) distances, the mean-phi difference and the
 Phimax error (the oracle's Phimax -- misfit.cpp:3261-3556 or
 cvd.cpp:1067-1274 -- on both states: atom counts, atoms matched within half
-a lattice spacing, largest matched position and phi differences), beside
+a lattice spacing, largest matched position and phi differences), the same
+for the FP32 path (FP64 k = 0 mode and reductions) from the same fields, beside
 the FP64 envelope of the same checkpoint: the largest distance from the
 drop-in of the drop-in without the narrowing and of the drop-in from fields
 one FP32 round-off away (the reference's rounding of its FFTW transforms is
@@ -113,15 +114,19 @@ def atom_match(a, b, box):
     return out
 
 
-def gpu_runs(hip, model, L, M, kw, phi, con, marks):
+def gpu_runs(hip, model, L, M, kw, phi, con, marks, fp32=False):
     """The drop-in ('ref'), the drop-in without narrowing ('exact') and the
-    drop-in from fields one FP32 round-off away ('pert'): fields at marks."""
+    drop-in from fields one FP32 round-off away ('pert'), and with fp32 the
+    FP32 path ('fp32'): fields at marks."""
     runs = {}
     rng = np.random.default_rng(11)
     pert = (phi * (1.0 + EPS32 * rng.uniform(-1.0, 1.0, phi.size)),
             con * (1.0 + EPS32 * rng.uniform(-1.0, 1.0, con.size)))
-    for tag, narrow, f in (("ref", 1, (phi, con)), ("exact", 0, (phi, con)), ("pert", 1, pert)):
-        p = hip.default_params(model=model, L=L, M=M, precision=hip.FP64, **kw)
+    cfg = [("ref", 1, (phi, con), hip.FP64), ("exact", 0, (phi, con), hip.FP64), ("pert", 1, pert, hip.FP64)]
+    if fp32:
+        cfg.append(("fp32", 1, (phi, con), hip.FP32))
+    for tag, narrow, f, prec in cfg:
+        p = hip.default_params(model=model, L=L, M=M, precision=prec, **kw)
         p.narrow = narrow
         s = hip.Sim(p)
         try:
@@ -169,7 +174,7 @@ def main():
         model, L, M = c
         kw = grid_kw(*c)
         name = f"{'misfit' if model == 0 else 'cvd'}_{L}x{M}"
-        runs = gpu_runs(hip, model, L, M, kw, *init[c], marks)
+        runs = gpu_runs(hip, model, L, M, kw, *init[c], marks, fp32=True)
         print(f"gpu runs done: {name}", file=sys.stderr, flush=True)
         out, proc = procs[c]
         rc = proc.wait()
@@ -189,11 +194,18 @@ def main():
                    "mean_phi_diff": float(rp.mean() - op.mean()),
                    "envelope_linf": float(env)}
             p = hip.default_params(model=model, L=L, M=M, precision=hip.FP64, **kw)
-            row["phimax"] = atom_match(phimax(orc, model, L, M, p, rp, rc_), phimax(orc, model, L, M, p, op, oc),
-                                       L * p.dx)
+            oatoms = phimax(orc, model, L, M, p, op, oc)
+            row["phimax"] = atom_match(phimax(orc, model, L, M, p, rp, rc_), oatoms, L * p.dx)
+            fp, fc = runs["fp32"][t]
+            row["fp32"] = {"linf": float(max(np.abs(fp - op).max(), np.abs(fc - oc).max())),
+                           "l2_phi": float(np.sqrt(np.mean((fp - op) ** 2))),
+                           "mean_phi_diff": float(fp.mean() - op.mean()),
+                           "phimax": atom_match(phimax(orc, model, L, M, p, fp, fc), oatoms, L * p.dx)}
             rows.append(row)
             print(f"{name} step {t}: linf {row['linf']:.2e} (envelope {env:.2e}) l2 {row['l2_phi']:.2e} "
-                  f"atoms {row['phimax']['atoms_dropin']}/{row['phimax']['atoms_oracle']}", file=sys.stderr, flush=True)
+                  f"atoms {row['phimax']['atoms_dropin']}/{row['phimax']['atoms_oracle']}; FP32 linf "
+                  f"{row['fp32']['linf']:.2e} l2 {row['fp32']['l2_phi']:.2e} atoms "
+                  f"{row['fp32']['phimax']['atoms_dropin']}", file=sys.stderr, flush=True)
         res["cases"][name] = rows
     shutil.rmtree(tmp, ignore_errors=True)
     text = json.dumps(res, indent=1)
