@@ -775,24 +775,36 @@ def main():
                 "its wall time: the sweep's aggregate rate against HBM peak"}
     for s in sims.values():
         s.close()
+    # The legs below are reported beside the headline, not required for it:
+    # a leg that fails records its error and the line is still printed.
+    leg_errors = (subprocess.SubprocessError, OSError, ValueError, IndexError, KeyError, RuntimeError)
     if args.accuracy_steps > 0:
         every = max(1, args.accuracy_steps // 10)
-        out["accuracy"] = accuracy_study(hip, args.accuracy_steps, every)
-        print(f"accuracy done: bar_ok={out['accuracy']['bar_ok']}", file=sys.stderr, flush=True)
+        try:
+            out["accuracy"] = accuracy_study(hip, args.accuracy_steps, every)
+        except leg_errors as e:
+            out["accuracy"] = {"error": f"{type(e).__name__}: {e}"}
+        print(f"accuracy done: bar_ok={out['accuracy'].get('bar_ok')}", file=sys.stderr, flush=True)
     if args.serve_seconds > 0:
-        out["serving"] = serve_leg(hip, args.serve_seconds)
-        print(f"serving done: {out['serving']['fp32_batched']}", file=sys.stderr, flush=True)
+        try:
+            out["serving"] = serve_leg(hip, args.serve_seconds)
+        except leg_errors as e:
+            out["serving"] = {"error": f"{type(e).__name__}: {e}"}
+        print(f"serving done: {out['serving'].get('fp32_batched', out['serving'].get('error'))}", file=sys.stderr,
+              flush=True)
     if args.cvd_steps > 0:
         try:
             out["cvd_checkpoints"] = cvd_checkpoint_leg(hip, args.cvd_steps, args.cvd_mod)
-        except (subprocess.SubprocessError, OSError, ValueError, IndexError, hip.PfcError) as e:
-            # the CLI leg is reported, not required for the headline line
+        except leg_errors as e:
             out["cvd_checkpoints"] = {"error": f"{type(e).__name__}: {e}"}
         print(f"cvd checkpoints done: {out['cvd_checkpoints']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
-        # one job per host core, 16 at most (the host share of one GPU)
-        out["cpu_baseline"]["concurrent"] = cpu_baseline_concurrent(min(16, os.cpu_count() or 1))
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+            # one job per host core, 16 at most (the host share of one GPU)
+            out["cpu_baseline"]["concurrent"] = cpu_baseline_concurrent(min(16, os.cpu_count() or 1))
+        except leg_errors as e:
+            out["cpu_baseline"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out))
 
 
