@@ -155,3 +155,37 @@ def test_state_digest_fixture_covers_every_case():
         assert len(g["energy_trace"]) == 2 * B * gen.TRACED
         vals = [float.fromhex(x) for x in g["energy_trace"]]
         assert all(np.isfinite(vals))
+
+
+def test_oracle_curves_atom_match():
+    """The atom matcher of tests/oracle_curves.py (nearest neighbour in a
+    periodic box, matched within half a lattice spacing): a set against
+    itself shifted by small known offsets, one atom moved across the box
+    edge, one atom far from any other."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("oracle_curves_cpu", os.path.join(ROOT, "tests", "oracle_curves.py"))
+    oc = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(oc)
+    box = 32.0
+    g = np.arange(0.5, box, 1.0)
+    x, y = [a.ravel() for a in np.meshgrid(g, g)]
+    p = np.linspace(0.1, 0.9, x.size)
+    rng = np.random.default_rng(3)
+    d = rng.uniform(-0.05, 0.05, (2, x.size))
+    xb, yb = x + d[0], y + d[1]
+    xb[0] = x[0] - 0.02 + box  # wraps: still 0.02 away from its partner
+    out = oc.atom_match((x, y, p), (xb, yb, p + 1e-3), box)
+    assert out["atoms_dropin"] == out["atoms_oracle"] == out["atoms_matched"] == x.size
+    ddx = d[0].copy()
+    ddx[0] = -0.02
+    want = np.sqrt(ddx ** 2 + d[1] ** 2).max()
+    assert abs(out["max_pos_err"] - want) < 1e-9
+    assert abs(out["max_phi_err"] - 1e-3) < 1e-12
+    # an extra atom halfway between lattice sites is left unmatched
+    xa2, ya2, pa2 = np.append(x, 5.0), np.append(y, 5.0), np.append(p, 0.5)
+    out2 = oc.atom_match((xa2, ya2, pa2), (x, y, p), box)
+    assert out2["atoms_dropin"] == x.size + 1 and out2["atoms_matched"] == x.size
+    assert oc.grid_kw(0, 256, 256) == dict(grainx=16, grainy=16)
+    assert oc.grid_kw(0, 128, 128) == dict(grainx=8, grainy=8)
+    assert oc.grid_kw(1, 256, 256) == dict(u0=0.05, con0=0.25)
