@@ -10,8 +10,13 @@ only; nothing here touches a GPU):
   that its own code reads and writes only what it allocated and has no
   undefined arithmetic;
 * the checkpoint-file writers of pfc-exp-gui_amd/host/pfc_io.cpp
-  (tests/host_io_main.cpp), threads included.
-Any sanitizer report aborts the run (-fno-sanitize-recover=all).
+  (tests/host_io_main.cpp), threads included;
+and a ThreadSanitizer run of the same writers: the background writer
+thread that takes checkpoint tasks in order and the threads that format
+per-atom tables in chunks (the reference has no threads; these are this
+build's own).
+Any sanitizer report aborts the run (-fno-sanitize-recover=all; TSan exits
+with a non-zero status on a report).
 """
 import os
 import shutil
@@ -105,3 +110,38 @@ def test_host_io_sanitized(san_dir, tmp_path):
     assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr
     assert np.array_equal(np.fromfile(str(d / "back.bin"), np.float32), vals["field"])  # VTI round trip
     assert open(d / "order.txt").read().split() == [str(k) for k in range(200)]  # writer task order
+
+
+def test_host_io_thread_sanitized(tmp_path):
+    """ThreadSanitizer on the checkpoint-file writers (pfcio::Writer's task
+    thread and the chunked table formatting threads): no data race, and the
+    files are the sequential formatting's, as in the ASan run."""
+    if shutil.which("g++") is None:
+        pytest.skip("no host compiler")
+    probe = tmp_path / "tsan_probe.cpp"
+    probe.write_text("#include <thread>\nint main(){int x=0; std::thread t([&]{x=1;}); t.join(); return x-1;}\n")
+    if subprocess.run(["g++", "-std=c++17", "-pthread", "-fsanitize=thread", "-o", str(tmp_path / "tsan_probe"),
+                       str(probe)], capture_output=True).returncode != 0:
+        pytest.skip("g++ without the TSan runtime")
+    exe = str(tmp_path / "host_io_main_tsan")
+    subprocess.run(["g++", "-std=c++17", "-pthread", "-fsanitize=thread", "-g", "-O1", "-o", exe,
+                    os.path.join(ROOT, "tests", "host_io_main.cpp"),
+                    os.path.join(ROOT, "pfc-exp-gui_amd", "host", "pfc_io.cpp")], check=True)
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("t_host_io_tsan", os.path.join(ROOT, "tests", "test_host_io.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    d = tmp_path / "io"
+    d.mkdir()
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1:exitcode=66")
+    old = os.environ.get("TSAN_OPTIONS")
+    os.environ["TSAN_OPTIONS"] = env["TSAN_OPTIONS"]
+    try:
+        vals, r = mod.run_case(exe, str(d), 5000, 32, 16, 4, 7)
+    finally:
+        if old is None:
+            os.environ.pop("TSAN_OPTIONS", None)
+        else:
+            os.environ["TSAN_OPTIONS"] = old
+    assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
+    assert open(d / "order.txt").read().split() == [str(k) for k in range(200)]
