@@ -78,8 +78,14 @@ namespace pfc {
 #ifndef PFC_COL_TWO_LDS16
 #define PFC_COL_TWO_LDS16 1  // the two-field step's exchange buffers hold 16 bytes per slot (else 32)
 #endif
+#ifndef PFC_COL_COEF_LAZY
+#define PFC_COL_COEF_LAZY 1  // plain steps read each mode's coefficients where the update uses them (no prefetched array)
+#endif
+#ifndef PFC_COL_COEF_PF
+#define PFC_COL_COEF_PF 2  // with LAZY, the first modes' coefficients loaded before the last forward transform
+#endif
 #ifndef PFC_COL_COEF_LATE
-#define PFC_COL_COEF_LATE 1  // the two-field step loads the mode coefficients after the forward transforms
+#define PFC_COL_COEF_LATE 1  // (prefetched form) the two-field step loads the mode coefficients after the forward transforms
 #endif
 #ifndef PFC_COEF
 #define PFC_COEF 1  // plain steps read per-mode coefficient tables (k_coef) where col_coef holds
@@ -510,12 +516,13 @@ __global__ void __launch_bounds__((RowCfg<R, H>::NT)) k_row_out(DevBufs b, R* __
 // its k_m = M/2 modes are read from the table's tail).
 // x and nz hold the column's full-spectrum elements t + G*q (q < E) in
 // threads t < G (GF: the column's thread count; the others hold none).
-template <typename R, int LL, int EM, int E, int G, int GF, bool COEF = false>
+template <typename R, int LL, int EM, int E, int G, int GF, bool COEF = false, bool LAZY = false, int PF = 0>
 __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs& p, int mode, bool want_e,
                                                bool use_nz, Vec<R, 4> (&x)[EM], const Cx<R> (&nz)[EM], int t, int j,
                                                int kmi, int kni, bool packed_tile, bool packed, char* xch,
                                                double& g, const float4* cf = nullptr, const float2* cc = nullptr,
-                                               const float* klp = nullptr) {
+                                               const float* klp = nullptr, const float4* lca = nullptr,
+                                               const float2* lcc = nullptr, uint32_t lcs = 0) {
     const bool act = G == GF || t < G;
     Vec<R, 4>* mx4 = (Vec<R, 4>*)xch;
     Cx<R>* nzx = (Cx<R>*)(mx4 + padded(LL));
@@ -541,16 +548,18 @@ __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs&
                 Vec<R, 2> sv, ov;
                 sv.c[0] = ph; sv.c[1] = co;
                 ov.c[0] = x[q].c[2]; ov.c[1] = x[q].c[3];
-                if constexpr (sizeof(R) == 4) update_mode_coef(cf[q], sv, ov, Cx<float>{0.0f, 0.0f});
-                else update_mode_coef64(p, cf[q], sv, ov, Cx<double>{0.0, 0.0});
+                const float4 cfq = (LAZY && q >= PF) ? lca[(size_t)i * lcs] : cf[q];
+                const float2 ccq = (LAZY && q >= PF) ? lcc[(size_t)i * lcs] : cc[q];
+                if constexpr (sizeof(R) == 4) update_mode_coef(cfq, sv, ov, Cx<float>{0.0f, 0.0f});
+                else update_mode_coef64(p, cfq, sv, ov, Cx<double>{0.0, 0.0});
                 ph = sv.c[0]; co = sv.c[1];
                 if (i == 0 && kmi == 0 && kni == 0) {
                     ph = Cx<R>{R(0), R(0)};
                     co = Cx<R>{R(0), R(0)};
                 }
                 if (want_e) g += klp ? mode_energy_kl(b, LL, i, klp[q], kmi, kni, co) : mode_energy(b, LL, i, kmi, kni, co);
-                a = Cx<R>{ph.x * (R)cc[q].x, ph.y * (R)cc[q].x};
-                bb = Cx<R>{ph.x * (R)cc[q].y, ph.y * (R)cc[q].y};
+                a = Cx<R>{ph.x * (R)ccq.x, ph.y * (R)ccq.x};
+                bb = Cx<R>{ph.x * (R)ccq.y, ph.y * (R)ccq.y};
             }
         } else if (!packed) {
             col_mode(b, p, mode, want_e, use_nz, LL, i, kmi, kni, ph, co, x[q].c[2], x[q].c[3], nz[q], a, bb, g);
@@ -562,7 +571,8 @@ __device__ __forceinline__ void col_tile_modes(const DevBufs& b, const PhysArgs&
                 const float4 ch = ((const float4*)b.cA)[te];
                 const float2 cch = ((const float2*)b.cC)[te];
                 packed_modes_coef(b, p, want_e, LL, i, j, ph, co, x[q].c[2], x[q].c[3], xm.c[0], xm.c[1], xm.c[2],
-                                  xm.c[3], cf[q], cc[q], ch, cch, a, bb, g);
+                                  xm.c[3], (LAZY && q >= PF) ? lca[(size_t)i * lcs] : cf[q],
+                                  (LAZY && q >= PF) ? lcc[(size_t)i * lcs] : cc[q], ch, cch, a, bb, g);
             }
         } else {
             const int im = pidx((LL - i) % LL);
@@ -703,22 +713,31 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
 #pragma unroll
         for (int q = 0; q < C::EB; q++) klv[q] = (C::GB == G || t < C::GB) ? b.kl[t + C::GB * q] : 0.0f;
     }
-    float4 cf[COEF ? EBc : 1];
-    float2 cc[COEF ? EBc : 1];
+    // LAZY: each mode's coefficients are read where the update uses them
+    // (L2 hits: one table serves the batch) except the first PF modes', which
+    // are loaded before the last forward transform, so no full array of them
+    // stays live across the transforms: 16 x 512^2 column kernel 190 -> 160
+    // VGPRs, three workgroups per CU instead of two, 34.5 -> 33.8 us alone
+    // (bit-identical).  Not at 1024 (one 1024^2 instance's energy-traced
+    // column pass 22.5 -> 23.0 us with every mode read lazily).
+    constexpr bool LAZY = COEF && PFC_COL_COEF_LAZY && LL != 1024;
+    constexpr int PF = LAZY ? (PFC_COL_COEF_PF < EBc ? PFC_COL_COEF_PF : EBc) : EBc;  // modes prefetched
+    float4 cf[COEF && PF > 0 ? PF : 1];
+    float2 cc[COEF && PF > 0 ? PF : 1];
     auto load_coef = [&]() {
-        if constexpr (COEF) {
+        if constexpr (COEF && PF > 0) {
             if (GBc == G || t < GBc) {
                 const Buf CA = mkbuf((const float4*)b.cA + tile0, 0), CC = mkbuf((const float2*)b.cC + tile0, 0);
                 const uint32_t ca_off = (uint32_t)(col + t * CS) * 16u, cc_off = (uint32_t)(col + t * CS) * 8u;
 #pragma unroll
-                for (int q = 0; q < EBc; q++) {
+                for (int q = 0; q < PF; q++) {
                     cf[q] = bload<float4>(CA, ca_off, (uint32_t)(GBc * q * CS) * 16u);
                     cc[q] = bload<float2>(CC, cc_off, (uint32_t)(GBc * q * CS) * 8u);
                 }
             }
         }
     };
-    constexpr bool COEF_LATE = TWO && PFC_COL_COEF_LATE;
+    constexpr bool COEF_LATE = TWO && (PFC_COL_COEF_LATE || LAZY);
     if constexpr (!COEF_LATE) load_coef();
 
     // Forward transform of {psi, con, inn, inc} along l.
@@ -731,8 +750,9 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
         }
         __syncthreads();
         fft_ab<R, 2, LL, E, -1, true, C::GEN, C::PLAN>(xs, lds2, t, tws);
+        if constexpr (LAZY) load_coef();  // the first modes' coefficients arrive during the next transform
         fft_ab<R, 2, LL, E, -1, true, C::GEN, C::PLAN>(xi, lds2, t, tws);
-        if constexpr (COEF_LATE) load_coef();
+        if constexpr (COEF_LATE && !LAZY) load_coef();
 #pragma unroll
         for (int q = 0; q < E; q++) {
             x[q].c[0] = xs[q].c[0];
@@ -772,8 +792,10 @@ k_col(DevBufs b, PhysArgs p, int mode, int flags) {
     }
     const bool want_e = KIND == COL_LEAN_E || (KIND == COL_GENERAL && (flags & COL_ENERGY) != 0);
     double g = 0.0;
-    col_tile_modes<R, LL, E, C::EB, C::GB, G, COEF>(b, p, mode, want_e, use_nz, x, nz, t, j, kmi, kni, packed_tile,
-                                                    packed, smem, g, cf, cc, KLP ? klv : nullptr);
+    col_tile_modes<R, LL, E, C::EB, C::GB, G, COEF, LAZY, PF>(
+        b, p, mode, want_e, use_nz, x, nz, t, j, kmi, kni, packed_tile, packed, smem, g, cf, cc, KLP ? klv : nullptr,
+        LAZY ? (const float4*)b.cA + tile0 + col : nullptr, LAZY ? (const float2*)b.cC + tile0 + col : nullptr,
+        (uint32_t)CS);
     if (step && b.tcount && blockIdx.x == 0 && tid == 0) b.tcount[inst] += 1;
     if (want_e) {
         const double gs = block_sum<NT>(g, red);
