@@ -505,26 +505,27 @@ void batch_events(pfc_server* s, std::unique_lock<std::mutex>& lk, Batch& b, std
         }
         if (j->cancel || !err.empty() || j->steps_done == target) ev.push_back(i);
     }
-    std::vector<int> release;
+    std::vector<int> release, stepped;  // stepped: slots whose jobs took their last step here
     std::map<int, std::string> failed;  // slot -> its job's failure
+    // Cancellations as of this event, read under the lock: the unlocked work
+    // below and the release after it act on the same answer (a cancel that
+    // arrives meanwhile takes effect at the job's next event).
+    std::vector<char> cancelled(b.slots, 0);
+    for (int i : ev) cancelled[i] = b.slot_job[i]->cancel;
+    std::vector<double> e(2 * (size_t)b.slots, NAN);
     if (!ev.empty()) {
         lk.unlock();
-        std::vector<double> e(2 * (size_t)b.slots, NAN);
         std::string berr = err;  // failure of the whole batch
         if (berr.empty() && pfc_energy(b.ctx, e.data())) berr = pfc_last_error();
         for (int i : ev) {
             Job* j = b.slot_job[i];
-            if (j->cancel || !berr.empty()) {
+            if (cancelled[i] || !berr.empty()) {
                 release.push_back(i);
-                if (!j->cancel) failed[i] = berr;
+                if (!cancelled[i]) failed[i] = berr;
                 continue;
             }
             const bool final = j->steps_done >= j->spec.steps;
-            if (final) {
-                j->t_stepped = t_done;
-                j->energy = e[2 * i];
-                j->density = e[2 * i + 1];
-            }
+            if (final) stepped.push_back(i);
             std::string jerr;
             try {  // host buffers: a failed allocation fails this job only
                 if (j->spec.keep_fields && final) {
@@ -561,11 +562,17 @@ void batch_events(pfc_server* s, std::unique_lock<std::mutex>& lk, Batch& b, std
         for (int i : release) pfc_set_active(b.ctx, i, 0);
         lk.lock();
     }
+    for (int i : stepped) {  // under the lock: pfc_server_poll reads these
+        Job* j = b.slot_job[i];
+        j->t_stepped = t_done;
+        j->energy = e[2 * i];
+        j->density = e[2 * i + 1];
+    }
     for (int i : release) {
         Job* j = b.slot_job[i];
         b.slot_job[i] = nullptr;
         auto f = failed.find(i);
-        if (j->cancel) {
+        if (cancelled[i]) {
             finish(s, j, PFC_JOB_CANCELLED, s->now());
         } else if (f != failed.end()) {
             if (j->outdir.empty()) {
