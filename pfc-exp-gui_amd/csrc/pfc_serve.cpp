@@ -658,21 +658,29 @@ extern "C" int pfc_server_create(const pfc_server_opts* o, pfc_server** out) try
     d.quantum = 500;
     if (o) d = *o;
     if (d.max_slots <= 0 || d.graph_chunk < 0 || d.quantum <= 0) return pfc::api_fail(PFC_E_ARG, "bad server options");
-    pfc_server* s = new pfc_server();
+    std::unique_ptr<pfc_server> owner(new pfc_server());
+    pfc_server* s = owner.get();
     s->o = d;
     if (d.hbm_budget > 0) {
         s->budget = d.hbm_budget;
     } else {
         size_t fr = 0, tot = 0;
-        if (pfc_device_memory(&fr, &tot)) {
-            delete s;
-            return PFC_E_HIP;
-        }
+        if (pfc_device_memory(&fr, &tot)) return PFC_E_HIP;
         s->budget = 0.8 * (double)fr;
     }
     s->sched = std::thread(sched_loop, s);
-    s->writer = std::thread(writer_loop, s);
-    *out = s;
+    try {
+        s->writer = std::thread(writer_loop, s);
+    } catch (...) {  // no writer thread: stop the scheduler before the server is freed
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->stop = true;
+            s->cv_sched.notify_all();
+        }
+        s->sched.join();
+        throw;
+    }
+    *out = owner.release();
     return PFC_OK;
 } PFC_SERVE_CATCH
 
