@@ -270,6 +270,9 @@ int pfc_server_poll(pfc_server* s, long long id, pfc_job_status* st);
 int pfc_server_result(pfc_server* s, long long id, double* phi, double* con);
 /* Failure text of a FAILED job. */
 int pfc_server_error(pfc_server* s, long long id, char* buf, int cap);
+/* Asks to cancel a job: a queued job ends CANCELLED when the scheduler next
+ * admits jobs, a running one at its next event (checkpoint, quantum or end);
+ * a job that completes first stays DONE. */
 int pfc_server_cancel(pfc_server* s, long long id);
 /* Wait until no job is queued, running or awaiting writes (timeout_s < 0: forever). */
 int pfc_server_drain(pfc_server* s, double timeout_s);
