@@ -637,15 +637,23 @@ def main():
 
     # Warm-up: group-graph launches for WARM_S seconds first (from idle the
     # device runs the sweep 10-25% slower until it has been loaded for some
-    # tens of ms), then the requested W steps.
+    # tens of ms), then the requested W steps.  Each warm-up launch is waited
+    # for before the next is queued: with a backlog of a few hundred queued
+    # launches the first graph launch after it ran 2-7% slower (20-step
+    # regions, first after the warm-up, seven processes: 70.8-77.9 G queued
+    # vs 75.6-80.1 G waited for).
+    def warm_group(n):
+        for _ in range(n):
+            group.step(1)
+            for s in sims.values():
+                s.sync()
+
     t_w = time.perf_counter()
-    group.step(1)
-    for s in sims.values():
-        s.sync()
+    warm_group(1)
     first = time.perf_counter() - t_w
     warm_launches = 1 + max(0, int(WARM_S / max(first, 1e-4)))
-    group.step(warm_launches - 1)
-    group.step(args.warmup // chunk)
+    warm_group(warm_launches - 1)
+    warm_group(args.warmup // chunk)
     sweep(args.warmup % chunk, True)
     for s in sims.values():
         s.sync()
