@@ -215,3 +215,23 @@ def test_request_loop_serving_shapes(hip, oracle):
     print(f"served CVD 1024^2 FP32 vs oracle after {steps} steps: {err:.3e}", flush=True)
     assert err < 1e-5
     srv.close()
+
+
+def test_cancel_after_done_keeps_result(hip):
+    """A cancellation that arrives after a job has completed leaves it DONE,
+    with its steps and final fields (those of the same job run alone)."""
+    srv = hip.Server(max_slots=2, graph_chunk=5, quantum=10)
+    p = hip.default_params(model=hip.MISFIT, L=64, M=64, precision=hip.FP32, grainx=4, grainy=4, seed=41)
+    a = srv.submit(p, 30, keep_fields=True)
+    srv.drain(120)
+    assert srv.poll(a).state == hip.JOB_DONE
+    phi0, con0 = srv.result(a)
+    srv.cancel(a)
+    srv.drain(120)
+    st = srv.poll(a)
+    assert st.state == hip.JOB_DONE and st.steps_done == 30
+    phi1, con1 = srv.result(a)
+    assert np.array_equal(phi0, phi1) and np.array_equal(con0, con1)
+    (ref_phi, ref_con), _ = _alone(hip, p, 30)
+    assert np.array_equal(phi1, ref_phi) and np.array_equal(con1, ref_con)
+    srv.close()
